@@ -1,0 +1,199 @@
+/*
+ * lyg — MI355X (gfx950) tessellation engine for lyon's hot path.
+ *
+ * C ABI of liblyg.so.  Plain pointers and sizes only.  Each entry point names the
+ * lyon interface it replaces (paths relative to the lyon source tree).
+ *
+ * Conventions shared by every call:
+ *   - A path is lyon's flat storage (crates/path/src/path.rs:75-79): verbs[] (lyg_verb,
+ *     order of path.rs:29-36) and points[] as x,y f32 pairs; a CLOSE verb carries the
+ *     stored copy of the sub-path's first point (path.rs:537-542).
+ *   - Output follows BuffersBuilder<[f32;2], u32, Positions>
+ *     (crates/tessellation/src/geometry_builder.rs:300-351, 451-508): vertices are
+ *     appended, and indices are absolute ids = vertex_offset + position in the vertex
+ *     array of this call.  u32::MAX is reserved (VertexId::INVALID).
+ *   - On error nothing is written (abort_geometry, geometry_builder.rs:478-481).
+ *   - Status codes map one to one to TessellationError (crates/tessellation/src/error.rs).
+ */
+#ifndef LYG_H
+#define LYG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Verb codes: lyon_path::path::Verb (path.rs:29-36). */
+enum lyg_verb {
+    LYG_VERB_LINE = 0,
+    LYG_VERB_QUADRATIC = 1,
+    LYG_VERB_CUBIC = 2,
+    LYG_VERB_BEGIN = 3,
+    LYG_VERB_CLOSE = 4,
+    LYG_VERB_END = 5
+};
+
+/* lyon_path::LineCap / LineJoin / FillRule (crates/path/src/lib.rs:170-206, 262-280);
+ * lyon_tessellation::Orientation (crates/tessellation/src/lib.rs:302-308). */
+enum lyg_line_cap { LYG_CAP_BUTT = 0, LYG_CAP_SQUARE = 1, LYG_CAP_ROUND = 2 };
+enum lyg_line_join {
+    LYG_JOIN_MITER = 0,
+    LYG_JOIN_MITER_CLIP = 1,
+    LYG_JOIN_ROUND = 2,
+    LYG_JOIN_BEVEL = 3
+};
+enum lyg_fill_rule { LYG_FILL_EVEN_ODD = 0, LYG_FILL_NON_ZERO = 1 };
+enum lyg_orientation { LYG_ORIENT_HORIZONTAL = 0, LYG_ORIENT_VERTICAL = 1 };
+
+/* TessellationError (error.rs:5-125) plus the runtime's own failures. */
+enum lyg_status {
+    LYG_OK = 0,
+    LYG_ERR_TOO_MANY_VERTICES = 1,  /* GeometryBuilderError::TooManyVertices */
+    LYG_ERR_INVALID_VERTEX = 2,     /* GeometryBuilderError::InvalidVertex */
+    LYG_ERR_POSITION_IS_NAN = 3,    /* UnsupportedParamater::PositionIsNaN */
+    LYG_ERR_TOLERANCE_IS_NAN = 4,   /* UnsupportedParamater::ToleranceIsNaN */
+    LYG_ERR_INTERNAL = 5,           /* InternalError::* */
+    LYG_ERR_CAPACITY = 16,          /* output arrays too small: required sizes were set */
+    LYG_ERR_INVALID_ARGUMENT = 17,
+    LYG_ERR_DEVICE = 18,            /* HIP runtime failure */
+    LYG_ERR_UNSUPPORTED = 19,       /* option combination not implemented */
+    LYG_ERR_NO_SPACE = 20           /* serving pool cannot hold the request */
+};
+
+/* StrokeOptions (crates/tessellation/src/lib.rs:311-433); defaults: width 1, miter limit 4,
+ * tolerance 0.1, Butt caps, Miter join.  variable_line_width is not supported. */
+typedef struct lyg_stroke_options {
+    float line_width;
+    float miter_limit;
+    float tolerance;
+    uint8_t start_cap;
+    uint8_t end_cap;
+    uint8_t line_join;
+    uint8_t _pad;
+} lyg_stroke_options;
+
+/* FillOptions (lib.rs:445-531); defaults: tolerance 0.1, EvenOdd, Vertical, intersections on. */
+typedef struct lyg_fill_options {
+    float tolerance;
+    uint8_t fill_rule;
+    uint8_t sweep_orientation;
+    uint8_t handle_intersections;
+    uint8_t _pad;
+} lyg_fill_options;
+
+/* A borrowed path: lyon_path::PathSlice (path.rs:82-87) without attributes. */
+typedef struct lyg_path_view {
+    const uint8_t* verbs;
+    uint32_t n_verbs;
+    const float* points; /* x, y pairs */
+    uint32_t n_points;
+} lyg_path_view;
+
+/* VertexBuffers<[f32;2], u32> (geometry_builder.rs:262-286) seen through a BuffersBuilder. */
+typedef struct lyg_output {
+    float* vertices;          /* x, y pairs, capacity vertex_capacity */
+    uint32_t* indices;        /* capacity index_capacity */
+    uint32_t vertex_capacity;
+    uint32_t index_capacity;
+    uint32_t vertex_offset;   /* BuffersBuilder::with_vertex_offset */
+    uint32_t n_vertices;      /* out: written (or required on LYG_ERR_CAPACITY) */
+    uint32_t n_indices;       /* out */
+} lyg_output;
+
+typedef struct lyg_ctx lyg_ctx;
+
+/* ---- context -------------------------------------------------------------------------
+ * One context per host thread: owns a HIP stream, a device memory pool and launch
+ * parameters.  Not thread-safe; use one context per thread.  (The reference's analogue is
+ * a FillTessellator/StrokeTessellator instance, fill.rs:523-539 / stroke.rs:108-111.) */
+lyg_ctx* lyg_ctx_create(int device);
+void lyg_ctx_destroy(lyg_ctx* ctx);
+const char* lyg_status_string(int32_t status);
+/* Last HIP error text recorded by the context (empty if none). */
+const char* lyg_ctx_last_error(const lyg_ctx* ctx);
+
+/* ---- plain per-path drop-in (host buffers in and out) --------------------------------
+ * StrokeTessellator::tessellate_path (crates/tessellation/src/stroke.rs:164-177). */
+int32_t lyg_stroke_path(lyg_ctx* ctx, const lyg_path_view* path, const lyg_stroke_options* opts,
+                        lyg_output* out);
+/* FillTessellator::tessellate_path (crates/tessellation/src/fill.rs:628-641). */
+int32_t lyg_fill_path(lyg_ctx* ctx, const lyg_path_view* path, const lyg_fill_options* opts,
+                      lyg_output* out);
+
+/* ---- lyon_geom flattening, array form -------------------------------------------------
+ * for_each_flattened_with_t of QuadraticBezierSegment (quadratic_bezier.rs:368-391),
+ * CubicBezierSegment (cubic_bezier.rs:590-614) and LineSegment (line.rs:544-550) for n
+ * segments.  kinds[i]: 0 line, 1 quadratic, 2 cubic; ctrl: 8 floats per segment
+ * (from, ctrl1, ctrl2, to; unused slots ignored; a line uses from and ctrl1 as its end).
+ * Writes the end point of every flattened sub-segment; offsets[n+1] delimits segment i.
+ * Host pointers. */
+int32_t lyg_flatten_segments(lyg_ctx* ctx, const int32_t* kinds, const float* ctrl, uint32_t n,
+                             float tolerance, float* out_points, uint32_t point_capacity,
+                             uint32_t* offsets, uint32_t* n_points);
+/* Arc::for_each_flattened_with_t (arc.rs:339-368) for n arcs of 7 floats each
+ * (center x, y, radii x, y, start angle, sweep angle, x rotation).  Host pointers. */
+int32_t lyg_flatten_arcs(lyg_ctx* ctx, const float* arcs, uint32_t n, float tolerance,
+                         float* out_points, uint32_t point_capacity, uint32_t* offsets,
+                         uint32_t* n_points);
+/* arc_to_quadratic_beziers_with_t (arc.rs:668-701): 6 floats (from, ctrl, to) per quadratic.
+ * Host pointers. */
+int32_t lyg_arc_to_quadratics(lyg_ctx* ctx, const float* arcs, uint32_t n, float* out_quads,
+                              uint32_t quad_capacity, uint32_t* offsets, uint32_t* n_quads);
+
+/* ---- batched form (device-resident buffers) ------------------------------------------
+ * No lyon counterpart: equivalent to calling tessellate_path for every path in order with
+ * one BuffersBuilder over a shared VertexBuffers (geometry_builder.rs:300-351).  Every
+ * pointer here is device memory.  Inputs: n paths packed as verbs/points with n+1 offsets.
+ * Outputs: vertices/indices plus per-path offsets (n+1 entries each) and status. */
+typedef struct lyg_batch_in {
+    const uint8_t* verbs;
+    const float* points;
+    const uint32_t* verb_offsets;
+    const uint32_t* point_offsets;
+    uint32_t n_paths;
+} lyg_batch_in;
+
+typedef struct lyg_batch_out {
+    float* vertices;
+    uint32_t* indices;
+    uint64_t vertex_capacity;
+    uint64_t index_capacity;
+    uint32_t* path_vertex_offsets; /* n+1, device */
+    uint32_t* path_index_offsets;  /* n+1, device */
+    uint32_t vertex_offset;
+    uint64_t n_vertices;           /* out (host field) */
+    uint64_t n_indices;            /* out (host field) */
+} lyg_batch_out;
+
+/* opts: n_opts entries in device memory; n_opts == 1 shares one StrokeOptions. */
+int32_t lyg_stroke_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
+                         uint32_t n_opts, lyg_batch_out* out);
+int32_t lyg_fill_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
+                       uint32_t n_opts, lyg_batch_out* out);
+
+/* Size query for the batched stroke: per-path vertex/index counts into device arrays
+ * (n entries each) and totals.  Same kernels as lyg_stroke_batch's first pass. */
+int32_t lyg_stroke_batch_count(lyg_ctx* ctx, const lyg_batch_in* in,
+                               const lyg_stroke_options* opts, uint32_t n_opts,
+                               uint32_t* path_nv, uint32_t* path_ni, uint64_t* total_nv,
+                               uint64_t* total_ni);
+
+/* ---- synthetic inputs ----------------------------------------------------------------
+ * Seeded generators for the benchmark workloads (SplitMix64 -> PCG32).  Host memory.
+ * Call once with null arrays to get sizes, then again with arrays of those sizes. */
+int32_t lyg_gen_stroke_batch(uint64_t seed, uint32_t n_paths, uint32_t min_segs,
+                             uint32_t max_segs, float coord_max, int closed, uint8_t* verbs,
+                             float* points, uint32_t* verb_offsets, uint32_t* point_offsets,
+                             uint64_t* n_verbs, uint64_t* n_points);
+/* Per-path StrokeOptions of the batched stroke workload: width uniform in [w_min, w_max],
+ * join uniform over {Miter, Round, Bevel}, one cap uniform over {Butt, Square, Round},
+ * miter limit 4.  `out` has n_paths entries (host memory). */
+int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance, float w_min,
+                               float w_max, lyg_stroke_options* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LYG_H */

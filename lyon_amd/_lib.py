@@ -1,0 +1,110 @@
+"""ctypes binding of liblyg.so (include/lyg.h).
+
+The product's only compute path: if the shared library is missing, importing anything
+that needs it raises — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblyg.so")
+
+F32P = C.POINTER(C.c_float)
+U32P = C.POINTER(C.c_uint32)
+U8P = C.POINTER(C.c_uint8)
+I32P = C.POINTER(C.c_int32)
+U64P = C.POINTER(C.c_uint64)
+
+
+class StrokeOptionsC(C.Structure):
+    _fields_ = [("line_width", C.c_float), ("miter_limit", C.c_float), ("tolerance", C.c_float),
+                ("start_cap", C.c_uint8), ("end_cap", C.c_uint8), ("line_join", C.c_uint8),
+                ("_pad", C.c_uint8)]
+
+
+class FillOptionsC(C.Structure):
+    _fields_ = [("tolerance", C.c_float), ("fill_rule", C.c_uint8),
+                ("sweep_orientation", C.c_uint8), ("handle_intersections", C.c_uint8),
+                ("_pad", C.c_uint8)]
+
+
+class PathViewC(C.Structure):
+    _fields_ = [("verbs", U8P), ("n_verbs", C.c_uint32), ("points", F32P),
+                ("n_points", C.c_uint32)]
+
+
+class OutputC(C.Structure):
+    _fields_ = [("vertices", F32P), ("indices", U32P), ("vertex_capacity", C.c_uint32),
+                ("index_capacity", C.c_uint32), ("vertex_offset", C.c_uint32),
+                ("n_vertices", C.c_uint32), ("n_indices", C.c_uint32)]
+
+
+class BatchInC(C.Structure):
+    _fields_ = [("verbs", C.c_void_p), ("points", C.c_void_p), ("verb_offsets", C.c_void_p),
+                ("point_offsets", C.c_void_p), ("n_paths", C.c_uint32)]
+
+
+class BatchOutC(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("indices", C.c_void_p),
+                ("vertex_capacity", C.c_uint64), ("index_capacity", C.c_uint64),
+                ("path_vertex_offsets", C.c_void_p), ("path_index_offsets", C.c_void_p),
+                ("vertex_offset", C.c_uint32), ("n_vertices", C.c_uint64),
+                ("n_indices", C.c_uint64)]
+
+
+# Every symbol include/lyg.h declares, with its ctypes signature.
+SIGNATURES = {
+    "lyg_ctx_create": ([C.c_int], C.c_void_p),
+    "lyg_ctx_destroy": ([C.c_void_p], None),
+    "lyg_status_string": ([C.c_int32], C.c_char_p),
+    "lyg_ctx_last_error": ([C.c_void_p], C.c_char_p),
+    "lyg_stroke_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
+                         C.POINTER(OutputC)], C.c_int32),
+    "lyg_fill_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),
+                       C.POINTER(OutputC)], C.c_int32),
+    "lyg_flatten_segments": ([C.c_void_p, I32P, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32,
+                              U32P, U32P], C.c_int32),
+    "lyg_flatten_arcs": ([C.c_void_p, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32, U32P, U32P],
+                         C.c_int32),
+    "lyg_arc_to_quadratics": ([C.c_void_p, F32P, C.c_uint32, F32P, C.c_uint32, U32P, U32P],
+                              C.c_int32),
+    "lyg_stroke_batch": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                          C.POINTER(BatchOutC)], C.c_int32),
+    "lyg_fill_batch": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                        C.POINTER(BatchOutC)], C.c_int32),
+    "lyg_stroke_batch_count": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                                C.c_void_p, C.c_void_p, U64P, U64P], C.c_int32),
+    "lyg_gen_stroke_batch": ([C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
+                              U8P, F32P, U32P, U32P, U64P, U64P], C.c_int32),
+    "lyg_gen_stroke_options": ([C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                C.POINTER(StrokeOptionsC)], C.c_int32),
+}
+
+_lib = None
+
+
+def lib():
+    """Load liblyg.so once; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header_path: str | None = None):
+    """Names of the functions include/lyg.h declares (parsed from the header text)."""
+    import re
+    if header_path is None:
+        header_path = os.path.join(os.path.dirname(_HERE), "include", "lyg.h")
+    text = open(header_path).read()
+    return sorted(set(re.findall(r"\b(lyg_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,476 @@
+// Host runtime of liblyg.so: contexts, the device buffer pool and the C ABI.
+//
+// A context owns one HIP stream (default priority) and a small set of grow-only device
+// buffers.  The plain per-path calls stage the path into device memory, run the count
+// kernel, size-check the caller's arrays (two-phase protocol), run the emit kernel and
+// copy the geometry back — the same kernels the batched form runs over many paths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lyg.h"
+#include "lyg_types.h"
+
+extern "C" {
+hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hipStream_t stream);
+hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
+hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
+hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
+                                  const uint32_t* offsets, float* out, hipStream_t s);
+hipError_t lyg_launch_arc_to_quads(const float* arcs, uint32_t n, uint32_t* counts,
+                                   const uint32_t* offsets, float* out, hipStream_t s);
+hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
+                           uint32_t* overflow, hipStream_t s);
+hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+}
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct lyg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    std::string last_error;
+    // grow-only device scratch
+    DevBuf in_verbs, in_points, in_offs, in_opts, counts_v, counts_i, offs_v, offs_i, out_v,
+        out_i, misc, arcs, flat_kinds, flat_ctrl;
+    uint32_t* d_counters = nullptr;  // [0..3] work-queue heads, [4] overflow flag
+    uint64_t* d_totals = nullptr;    // [0] vertices, [1] indices
+    void* h_pinned = nullptr;        // small pinned staging area for totals
+};
+
+namespace {
+
+// LYG_TRACE=1 prints every host step to stderr (flushed), for diagnosing a run that
+// goes quiet.
+bool trace_on() {
+    static int on = -1;
+    if (on < 0) {
+        const char* v = getenv("LYG_TRACE");
+        on = (v && *v && *v != '0') ? 1 : 0;
+    }
+    return on == 1;
+}
+
+bool check(lyg_ctx* c, hipError_t e, const char* what) {
+    if (trace_on()) {
+        fprintf(stderr, "[lyg] %s -> %d\n", what, (int)e);
+        fflush(stderr);
+    }
+    if (e == hipSuccess) return true;
+    c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+bool ensure(lyg_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return true;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t cap = std::max(bytes, (size_t)4096);
+    cap = (cap + 4095) & ~(size_t)4095;
+    if (!check(c, hipMalloc(&b.p, cap), "hipMalloc")) return false;
+    b.cap = cap;
+    return true;
+}
+
+template <class T>
+T* as(DevBuf& b) {
+    return reinterpret_cast<T*>(b.p);
+}
+
+int persistent_grid(lyg_ctx* c, uint32_t n_paths) {
+    // 4 waves per block, one path per wave at a time; 8 blocks per CU keep 32 waves/CU.
+    uint32_t blocks_needed = (n_paths + 3) / 4;
+    uint32_t cap = (uint32_t)c->cu_count * 8;
+    return (int)std::max(1u, std::min(blocks_needed, cap));
+}
+
+// Count pass + scan over the paths of `L`; fills per-path counts and offsets on the
+// device and the totals on the host.
+int32_t stroke_count_and_scan(lyg_ctx* c, lyg::StrokeLaunch& L, uint32_t* offs_v,
+                              uint32_t* offs_i, uint64_t* total_v, uint64_t* total_i) {
+    const uint32_t n = L.in.n_paths;
+    if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, c->stream), "reset")) return LYG_ERR_DEVICE;
+    L.work_counter = c->d_counters + 0;
+    if (!check(c, lyg_launch_stroke(&L, 0, persistent_grid(c, n), c->stream), "stroke count"))
+        return LYG_ERR_DEVICE;
+    if (!check(c, lyg_launch_scan(L.path_nv, n, offs_v, c->d_totals + 0, c->d_counters + 4, c->stream),
+               "scan"))
+        return LYG_ERR_DEVICE;
+    if (!check(c, lyg_launch_scan(L.path_ni, n, offs_i, c->d_totals + 1, c->d_counters + 4, c->stream),
+               "scan"))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, c->stream), "copy"))
+        return LYG_ERR_DEVICE;
+    if (!check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, c->stream),
+               "copy"))
+        return LYG_ERR_DEVICE;
+    if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
+    *total_v = h[0];
+    *total_i = h[1];
+    uint32_t ovf = (uint32_t)(h[2] & 0xffffffffu);
+    if (ovf) return LYG_ERR_TOO_MANY_VERTICES;
+    return LYG_OK;
+}
+
+int32_t stroke_emit(lyg_ctx* c, lyg::StrokeLaunch& L) {
+    if (!check(c, lyg_launch_fill_u32(c->d_counters + 1, 1, 0, c->stream), "reset"))
+        return LYG_ERR_DEVICE;
+    L.work_counter = c->d_counters + 1;
+    L.err = c->d_counters + 5;
+    if (!check(c, lyg_launch_stroke(&L, 1, persistent_grid(c, L.in.n_paths), c->stream),
+               "stroke emit"))
+        return LYG_ERR_DEVICE;
+    uint32_t* h = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 128);
+    if (!check(c, hipMemcpyAsync(h, c->d_counters + 5, 4, hipMemcpyDeviceToHost, c->stream),
+               "copy"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+// After the stream has been synchronized: did the emit pass flag an inconsistency?
+bool emit_failed(lyg_ctx* c) {
+    const uint32_t* h = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(c->h_pinned) + 128);
+    if (*h) {
+        c->last_error = "emit pass wrote outside its path range (count/emit mismatch)";
+        return true;
+    }
+    return false;
+}
+
+bool valid_stroke_opts(const lyg_stroke_options& o) {
+    return o.start_cap <= 2 && o.end_cap <= 2 && o.line_join <= 3;
+}
+
+}  // namespace
+
+extern "C" {
+
+lyg_ctx* lyg_ctx_create(int device) {
+    if (trace_on()) {
+        fprintf(stderr, "[lyg] ctx_create(%d)\n", device);
+        fflush(stderr);
+    }
+    lyg_ctx* c = new lyg_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    if (hipMalloc(&c->d_counters, 64) != hipSuccess || hipMalloc(&c->d_totals, 64) != hipSuccess ||
+        hipHostMalloc(&c->h_pinned, 256, hipHostMallocDefault) != hipSuccess) {
+        lyg_ctx_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void lyg_ctx_destroy(lyg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->in_verbs, &c->in_points, &c->in_offs, &c->in_opts, &c->counts_v,
+                      &c->counts_i, &c->offs_v,    &c->offs_i,  &c->out_v,   &c->out_i,
+                      &c->misc,     &c->arcs,      &c->flat_kinds, &c->flat_ctrl};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (c->d_counters) (void)hipFree(c->d_counters);
+    if (c->d_totals) (void)hipFree(c->d_totals);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lyg_status_string(int32_t s) {
+    switch (s) {
+        case LYG_OK: return "ok";
+        case LYG_ERR_TOO_MANY_VERTICES: return "GeometryBuilderError::TooManyVertices";
+        case LYG_ERR_INVALID_VERTEX: return "GeometryBuilderError::InvalidVertex";
+        case LYG_ERR_POSITION_IS_NAN: return "UnsupportedParamater::PositionIsNaN";
+        case LYG_ERR_TOLERANCE_IS_NAN: return "UnsupportedParamater::ToleranceIsNaN";
+        case LYG_ERR_INTERNAL: return "InternalError";
+        case LYG_ERR_CAPACITY: return "output capacity too small";
+        case LYG_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case LYG_ERR_DEVICE: return "HIP runtime error";
+        case LYG_ERR_UNSUPPORTED: return "unsupported option";
+        case LYG_ERR_NO_SPACE: return "no space in the device pool";
+        default: return "unknown";
+    }
+}
+
+const char* lyg_ctx_last_error(const lyg_ctx* c) { return c ? c->last_error.c_str() : ""; }
+
+// ---- plain per-path stroke -------------------------------------------------------------
+int32_t lyg_stroke_path(lyg_ctx* c, const lyg_path_view* path, const lyg_stroke_options* opts,
+                        lyg_output* out) {
+    if (!c || !path || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
+    if (!valid_stroke_opts(*opts)) return LYG_ERR_INVALID_ARGUMENT;
+    if ((path->n_verbs && !path->verbs) || (path->n_points && !path->points))
+        return LYG_ERR_INVALID_ARGUMENT;
+    out->n_vertices = 0;
+    out->n_indices = 0;
+    if (path->n_verbs == 0) return LYG_OK;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    if (!ensure(c, c->in_verbs, path->n_verbs) || !ensure(c, c->in_points, 8ull * path->n_points) ||
+        !ensure(c, c->in_offs, 16) || !ensure(c, c->in_opts, sizeof(lyg_stroke_options)) ||
+        !ensure(c, c->counts_v, 8) || !ensure(c, c->counts_i, 8) || !ensure(c, c->offs_v, 8) ||
+        !ensure(c, c->offs_i, 8))
+        return LYG_ERR_DEVICE;
+    uint32_t* h = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
+    h[0] = 0;
+    h[1] = path->n_verbs;
+    h[2] = 0;
+    h[3] = path->n_points;
+    hipStream_t s = c->stream;
+    if (!check(c, hipMemcpyAsync(c->in_verbs.p, path->verbs, path->n_verbs, hipMemcpyHostToDevice, s), "H2D") ||
+        (path->n_points &&
+         !check(c, hipMemcpyAsync(c->in_points.p, path->points, 8ull * path->n_points, hipMemcpyHostToDevice, s), "H2D")) ||
+        !check(c, hipMemcpyAsync(c->in_offs.p, h, 16, hipMemcpyHostToDevice, s), "H2D") ||
+        !check(c, hipMemcpyAsync(c->in_opts.p, opts, sizeof(*opts), hipMemcpyHostToDevice, s), "H2D"))
+        return LYG_ERR_DEVICE;
+    lyg::StrokeLaunch L{};
+    L.in.verbs = as<uint8_t>(c->in_verbs);
+    L.in.points = as<float>(c->in_points);
+    L.in.verb_off = as<uint32_t>(c->in_offs);
+    L.in.point_off = as<uint32_t>(c->in_offs) + 2;
+    L.in.n_paths = 1;
+    L.in.first_path = 0;
+    L.opts = as<lyg_stroke_options>(c->in_opts);
+    L.opts_stride = 0;
+    L.path_nv = as<uint32_t>(c->counts_v);
+    L.path_ni = as<uint32_t>(c->counts_i);
+    L.vertex_offset = out->vertex_offset;
+    uint64_t tv = 0, ti = 0;
+    int32_t st = stroke_count_and_scan(c, L, as<uint32_t>(c->offs_v), as<uint32_t>(c->offs_i), &tv, &ti);
+    if (st != LYG_OK) return st;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350).
+    if (tv > 0 && (uint64_t)out->vertex_offset + tv - 1 > 0xfffffffeull) return LYG_ERR_TOO_MANY_VERTICES;
+    if (tv > out->vertex_capacity || ti > out->index_capacity) {
+        out->n_vertices = (uint32_t)tv;
+        out->n_indices = (uint32_t)ti;
+        return LYG_ERR_CAPACITY;
+    }
+    if (tv == 0 && ti == 0) return LYG_OK;
+    if (!ensure(c, c->out_v, 8 * tv) || !ensure(c, c->out_i, 4 * ti)) return LYG_ERR_DEVICE;
+    L.vtx_off = as<uint32_t>(c->offs_v);
+    L.idx_off = as<uint32_t>(c->offs_i);
+    L.out_vertices = as<float>(c->out_v);
+    L.out_indices = as<uint32_t>(c->out_i);
+    st = stroke_emit(c, L);
+    if (st != LYG_OK) return st;
+    if (!check(c, hipMemcpyAsync(out->vertices, c->out_v.p, 8 * tv, hipMemcpyDeviceToHost, s), "D2H") ||
+        (ti && !check(c, hipMemcpyAsync(out->indices, c->out_i.p, 4 * ti, hipMemcpyDeviceToHost, s), "D2H")) ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    if (emit_failed(c)) return LYG_ERR_INTERNAL;
+    out->n_vertices = (uint32_t)tv;
+    out->n_indices = (uint32_t)ti;
+    return LYG_OK;
+}
+
+// ---- batched stroke (device-resident) -------------------------------------------------
+int32_t lyg_stroke_batch_count(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_options* opts,
+                               uint32_t n_opts, uint32_t* path_nv, uint32_t* path_ni,
+                               uint64_t* total_nv, uint64_t* total_ni) {
+    if (!c || !in || !opts || !path_nv || !path_ni) return LYG_ERR_INVALID_ARGUMENT;
+    if (n_opts != 1 && n_opts != in->n_paths) return LYG_ERR_INVALID_ARGUMENT;
+    if (in->n_paths == 0) {
+        if (total_nv) *total_nv = 0;
+        if (total_ni) *total_ni = 0;
+        return LYG_OK;
+    }
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    lyg::StrokeLaunch L{};
+    L.in.verbs = in->verbs;
+    L.in.points = in->points;
+    L.in.verb_off = in->verb_offsets;
+    L.in.point_off = in->point_offsets;
+    L.in.n_paths = in->n_paths;
+    L.opts = opts;
+    L.opts_stride = n_opts == 1 ? 0 : 1;
+    L.path_nv = path_nv;
+    L.path_ni = path_ni;
+    if (!ensure(c, c->offs_v, 4ull * (in->n_paths + 1)) || !ensure(c, c->offs_i, 4ull * (in->n_paths + 1)))
+        return LYG_ERR_DEVICE;
+    uint64_t tv, ti;
+    int32_t st = stroke_count_and_scan(c, L, as<uint32_t>(c->offs_v), as<uint32_t>(c->offs_i), &tv, &ti);
+    if (total_nv) *total_nv = tv;
+    if (total_ni) *total_ni = ti;
+    return st;
+}
+
+int32_t lyg_stroke_batch(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_options* opts,
+                         uint32_t n_opts, lyg_batch_out* out) {
+    if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
+    if (n_opts != 1 && n_opts != in->n_paths) return LYG_ERR_INVALID_ARGUMENT;
+    out->n_vertices = 0;
+    out->n_indices = 0;
+    if (in->n_paths == 0) return LYG_OK;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    if (!ensure(c, c->counts_v, 4ull * in->n_paths) || !ensure(c, c->counts_i, 4ull * in->n_paths))
+        return LYG_ERR_DEVICE;
+    lyg::StrokeLaunch L{};
+    L.in.verbs = in->verbs;
+    L.in.points = in->points;
+    L.in.verb_off = in->verb_offsets;
+    L.in.point_off = in->point_offsets;
+    L.in.n_paths = in->n_paths;
+    L.opts = opts;
+    L.opts_stride = n_opts == 1 ? 0 : 1;
+    L.path_nv = as<uint32_t>(c->counts_v);
+    L.path_ni = as<uint32_t>(c->counts_i);
+    L.vertex_offset = out->vertex_offset;
+    uint64_t tv, ti;
+    int32_t st = stroke_count_and_scan(c, L, out->path_vertex_offsets, out->path_index_offsets, &tv, &ti);
+    if (st != LYG_OK) return st;
+    if (tv > 0 && (uint64_t)out->vertex_offset + tv - 1 > 0xfffffffeull) return LYG_ERR_TOO_MANY_VERTICES;
+    out->n_vertices = tv;
+    out->n_indices = ti;
+    if (tv > out->vertex_capacity || ti > out->index_capacity) return LYG_ERR_CAPACITY;
+    L.vtx_off = out->path_vertex_offsets;
+    L.idx_off = out->path_index_offsets;
+    L.out_vertices = out->vertices;
+    L.out_indices = out->indices;
+    st = stroke_emit(c, L);
+    if (st != LYG_OK) return st;
+    if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
+    if (emit_failed(c)) return LYG_ERR_INTERNAL;
+    return LYG_OK;
+}
+
+// ---- flattening ---------------------------------------------------------------------------
+int32_t lyg_flatten_segments(lyg_ctx* c, const int32_t* kinds, const float* ctrl, uint32_t n,
+                             float tolerance, float* out_points, uint32_t point_capacity,
+                             uint32_t* offsets, uint32_t* n_points) {
+    if (!c || (n && (!kinds || !ctrl || !offsets))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!(tolerance > 0.0f)) return LYG_ERR_TOLERANCE_IS_NAN;
+    if (n_points) *n_points = 0;
+    if (n == 0) {
+        offsets[0] = 0;
+        return LYG_OK;
+    }
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    hipStream_t s = c->stream;
+    if (!ensure(c, c->flat_kinds, 4ull * n) || !ensure(c, c->flat_ctrl, 32ull * n) ||
+        !ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->offs_v, 4ull * (n + 1)))
+        return LYG_ERR_DEVICE;
+    if (!check(c, hipMemcpyAsync(c->flat_kinds.p, kinds, 4ull * n, hipMemcpyHostToDevice, s), "H2D") ||
+        !check(c, hipMemcpyAsync(c->flat_ctrl.p, ctrl, 32ull * n, hipMemcpyHostToDevice, s), "H2D"))
+        return LYG_ERR_DEVICE;
+    lyg::FlattenLaunch L{};
+    L.kinds = as<int32_t>(c->flat_kinds);
+    L.ctrl = as<float>(c->flat_ctrl);
+    L.n = n;
+    L.tolerance = tolerance;
+    L.counts = as<uint32_t>(c->counts_v);
+    if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
+        !check(c, lyg_launch_flatten_count(&L, s), "flatten count") ||
+        !check(c, lyg_launch_scan(L.counts, n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    uint64_t total = h[0];
+    if (n_points) *n_points = (uint32_t)total;
+    if (total > point_capacity || !out_points) return LYG_ERR_CAPACITY;
+    if (!ensure(c, c->out_v, 8 * total)) return LYG_ERR_DEVICE;
+    L.offsets = as<uint32_t>(c->offs_v);
+    L.out_points = as<float>(c->out_v);
+    if (!check(c, lyg_launch_flatten_emit(&L, (uint32_t)total, s), "flatten emit") ||
+        !check(c, hipMemcpyAsync(out_points, c->out_v.p, 8 * total, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(offsets, c->offs_v.p, 4ull * (n + 1), hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+static int32_t arcs_common(lyg_ctx* c, const float* arcs, uint32_t n, float tol, bool quads,
+                           float* out, uint32_t cap, uint32_t* offsets, uint32_t* n_out) {
+    if (!c || (n && (!arcs || !offsets))) return LYG_ERR_INVALID_ARGUMENT;
+    if (n_out) *n_out = 0;
+    if (n == 0) {
+        offsets[0] = 0;
+        return LYG_OK;
+    }
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    hipStream_t s = c->stream;
+    if (!ensure(c, c->arcs, 28ull * n) || !ensure(c, c->counts_v, 4ull * n) ||
+        !ensure(c, c->offs_v, 4ull * (n + 1)))
+        return LYG_ERR_DEVICE;
+    if (!check(c, hipMemcpyAsync(c->arcs.p, arcs, 28ull * n, hipMemcpyHostToDevice, s), "H2D"))
+        return LYG_ERR_DEVICE;
+    hipError_t e = quads ? lyg_launch_arc_to_quads(as<float>(c->arcs), n, as<uint32_t>(c->counts_v), nullptr, nullptr, s)
+                         : lyg_launch_arc_flatten(as<float>(c->arcs), n, tol, as<uint32_t>(c->counts_v), nullptr, nullptr, s);
+    if (!check(c, e, "arc count") ||
+        !check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
+        !check(c, lyg_launch_scan(as<uint32_t>(c->counts_v), n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    uint64_t total = h[0];
+    if (n_out) *n_out = (uint32_t)total;
+    if (total > cap || !out) return LYG_ERR_CAPACITY;
+    size_t per = quads ? 24 : 8;
+    if (!ensure(c, c->out_v, per * total)) return LYG_ERR_DEVICE;
+    e = quads ? lyg_launch_arc_to_quads(as<float>(c->arcs), n, nullptr, as<uint32_t>(c->offs_v), as<float>(c->out_v), s)
+              : lyg_launch_arc_flatten(as<float>(c->arcs), n, tol, nullptr, as<uint32_t>(c->offs_v), as<float>(c->out_v), s);
+    if (!check(c, e, "arc emit") ||
+        !check(c, hipMemcpyAsync(out, c->out_v.p, per * total, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(offsets, c->offs_v.p, 4ull * (n + 1), hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_flatten_arcs(lyg_ctx* c, const float* arcs, uint32_t n, float tolerance,
+                         float* out_points, uint32_t point_capacity, uint32_t* offsets,
+                         uint32_t* n_points) {
+    if (!(tolerance > 0.0f)) return LYG_ERR_TOLERANCE_IS_NAN;
+    return arcs_common(c, arcs, n, tolerance, false, out_points, point_capacity, offsets, n_points);
+}
+
+int32_t lyg_arc_to_quadratics(lyg_ctx* c, const float* arcs, uint32_t n, float* out_quads,
+                              uint32_t quad_capacity, uint32_t* offsets, uint32_t* n_quads) {
+    return arcs_common(c, arcs, n, 0.0f, true, out_quads, quad_capacity, offsets, n_quads);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Fill entry points are provided by fill.hip when it is built; until then they report
+// LYG_ERR_UNSUPPORTED.
+__attribute__((weak)) int32_t lyg_fill_path(lyg_ctx*, const lyg_path_view*, const lyg_fill_options*,
+                                            lyg_output*) {
+    return LYG_ERR_UNSUPPORTED;
+}
+__attribute__((weak)) int32_t lyg_fill_batch(lyg_ctx*, const lyg_batch_in*, const lyg_fill_options*,
+                                             uint32_t, lyg_batch_out*) {
+    return LYG_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

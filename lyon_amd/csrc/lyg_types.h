@@ -1,0 +1,47 @@
+// Plain-old-data layouts shared by the host runtime and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/lyg.h"
+
+namespace lyg {
+
+// Device-side description of one batch of paths, in lyon's flat storage layout
+// (path.rs:75-79): verbs[] and points[] of every path back to back, with n+1 offsets.
+struct PathBatchDev {
+    const uint8_t* verbs;
+    const float* points;  // x, y pairs
+    const uint32_t* verb_off;
+    const uint32_t* point_off;
+    uint32_t n_paths;
+    uint32_t first_path;  // index of the first path of this launch within the batch
+};
+
+struct StrokeLaunch {
+    PathBatchDev in;
+    const lyg_stroke_options* opts;  // per path (opts_stride 1) or shared (stride 0)
+    uint32_t opts_stride;
+    // count pass outputs
+    uint32_t* path_nv;
+    uint32_t* path_ni;
+    // emit pass inputs/outputs
+    const uint32_t* vtx_off;  // exclusive offsets (n + 1)
+    const uint32_t* idx_off;
+    float* out_vertices;
+    uint32_t* out_indices;
+    uint32_t vertex_offset;  // BuffersBuilder::with_vertex_offset
+    uint32_t* work_counter;  // device work queue head (persistent kernels)
+    uint32_t* err;           // emit pass: set when a write falls outside its path's range
+};
+
+struct FlattenLaunch {
+    const int32_t* kinds;  // 0 line, 1 quadratic, 2 cubic
+    const float* ctrl;     // 8 floats per segment: from, ctrl1, ctrl2, to
+    uint32_t n;
+    float tolerance;
+    uint32_t* counts;         // count pass
+    const uint32_t* offsets;  // emit pass (n + 1)
+    float* out_points;
+};
+
+}  // namespace lyg

@@ -1,0 +1,218 @@
+// ORACLE — test infrastructure only (see refgeom.h).  Loaded through ctypes by tests/,
+// by __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by lyon_amd/.
+//
+// extern "C" face of the CPU restatement: stroke, fill and the lyon_geom flattening
+// calls, each producing VertexBuffers<[f32;2],u32> / point lists in a result handle.
+#include <atomic>
+#include <chrono>
+#include <cstdlib>
+#include <thread>
+#include <vector>
+
+#include "refgeom.h"
+#include "refpath.h"
+#include "refstroke.h"
+#include "reffill.h"
+
+using namespace refgeom;
+
+struct orc_result {
+    std::vector<V2> vertices;
+    std::vector<uint32_t> indices;
+    std::vector<uint32_t> offsets;  // flatten: per-segment point offsets
+    int status = 0;
+};
+
+extern "C" {
+
+void orc_result_free(orc_result* r) { delete r; }
+int orc_result_status(const orc_result* r) { return r->status; }
+uint64_t orc_result_n_vertices(const orc_result* r) { return r->vertices.size(); }
+uint64_t orc_result_n_indices(const orc_result* r) { return r->indices.size(); }
+uint64_t orc_result_n_offsets(const orc_result* r) { return r->offsets.size(); }
+void orc_result_copy(const orc_result* r, float* vertices, uint32_t* indices, uint32_t* offsets) {
+    if (vertices && !r->vertices.empty())
+        std::memcpy(vertices, r->vertices.data(), r->vertices.size() * sizeof(V2));
+    if (indices && !r->indices.empty())
+        std::memcpy(indices, r->indices.data(), r->indices.size() * 4);
+    if (offsets && !r->offsets.empty())
+        std::memcpy(offsets, r->offsets.data(), r->offsets.size() * 4);
+}
+
+// StrokeTessellator::tessellate_path into BuffersBuilder::new(..).with_vertex_offset(vo).
+orc_result* orc_stroke_path(const uint8_t* verbs, uint32_t n_verbs, const float* points,
+                            uint32_t n_points, float line_width, float miter_limit,
+                            float tolerance, int start_cap, int end_cap, int line_join,
+                            uint32_t vertex_offset) {
+    orc_result* r = new orc_result();
+    refpath::PathView pv{verbs, n_verbs, points, n_points};
+    refstroke::StrokeOptions o;
+    o.line_width = line_width;
+    o.miter_limit = miter_limit;
+    o.tolerance = tolerance;
+    o.start_cap = start_cap;
+    o.end_cap = end_cap;
+    o.line_join = line_join;
+    refstroke::Output out{&r->vertices, &r->indices, vertex_offset};
+    r->status = refstroke::stroke_path(pv, o, out);
+    return r;
+}
+
+// FillTessellator::tessellate_path into BuffersBuilder (fill.rs:628-641).
+orc_result* orc_fill_path(const uint8_t* verbs, uint32_t n_verbs, const float* points,
+                          uint32_t n_points, float tolerance, int fill_rule,
+                          int sweep_orientation, int handle_intersections,
+                          uint32_t vertex_offset) {
+    orc_result* r = new orc_result();
+    refpath::PathView pv{verbs, n_verbs, points, n_points};
+    reffill::FillOptions o;
+    o.tolerance = tolerance;
+    o.fill_rule = fill_rule;
+    o.sweep_orientation = sweep_orientation;
+    o.handle_intersections = handle_intersections != 0;
+    reffill::Output out{&r->vertices, &r->indices, vertex_offset};
+    r->status = reffill::fill_path(pv, o, out);
+    return r;
+}
+
+// The flattened polylines of every curve of a path, as lyon_geom's
+// for_each_flattened_with_t emits them: kind 0 line (1 point), 1 quadratic,
+// 2 cubic; `offsets` gets one entry per segment plus a final total.
+orc_result* orc_flatten_segments(const int* kinds, const float* ctrl, uint32_t n, float tol) {
+    orc_result* r = new orc_result();
+    r->offsets.push_back(0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* c = ctrl + 8 * i;
+        V2 p0{c[0], c[1]}, p1{c[2], c[3]}, p2{c[4], c[5]}, p3{c[6], c[7]};
+        auto cb = [&](V2, V2 to, float, float) { r->vertices.push_back(to); };
+        if (kinds[i] == 0) {
+            r->vertices.push_back(p1);
+        } else if (kinds[i] == 1) {
+            quad_flatten_with_t(p0, p1, p2, tol, cb);
+        } else {
+            cubic_flatten_with_t(p0, p1, p2, p3, tol, cb);
+        }
+        r->offsets.push_back((uint32_t)r->vertices.size());
+    }
+    return r;
+}
+
+// Arc::for_each_flattened_with_t for n arcs given as (cx, cy, rx, ry, start, sweep, xrot).
+orc_result* orc_flatten_arcs(const float* arcs, uint32_t n, float tol) {
+    orc_result* r = new orc_result();
+    r->offsets.push_back(0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* a = arcs + 7 * i;
+        refgeom::Arc arc{V2{a[0], a[1]}, V2{a[2], a[3]}, a[4], a[5], a[6]};
+        arc_flatten_with_t(arc, tol, [&](V2, V2 to, float, float) { r->vertices.push_back(to); });
+        r->offsets.push_back((uint32_t)r->vertices.size());
+    }
+    return r;
+}
+
+// arc_to_quadratic_beziers_with_t: vertices hold (from, ctrl, to) triples.
+orc_result* orc_arc_to_quads(const float* arcs, uint32_t n) {
+    orc_result* r = new orc_result();
+    r->offsets.push_back(0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* a = arcs + 7 * i;
+        refgeom::Arc arc{V2{a[0], a[1]}, V2{a[2], a[3]}, a[4], a[5], a[6]};
+        arc_to_quads_with_t(arc, [&](V2 f, V2 c, V2 t, float, float) {
+            r->vertices.push_back(f);
+            r->vertices.push_back(c);
+            r->vertices.push_back(t);
+        });
+        r->offsets.push_back((uint32_t)r->vertices.size());
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------
+// CPU baseline: stroke or fill many paths given in a packed batch, single thread or
+// a pool of worker threads over paths (the "rayon over paths" analogue), and return
+// the triangle count.  Output goes to per-worker reserved VertexBuffers.
+// ---------------------------------------------------------------------------------
+struct BatchJob {
+    const uint8_t* verbs;
+    const float* points;
+    const uint32_t* verb_off;
+    const uint32_t* point_off;
+    const float* opts;  // stroke: width, miter, tol, start_cap, end_cap, join (6 floats)
+                        // fill: tol, rule (2 floats)
+    int is_fill;
+};
+
+static uint64_t run_range(const BatchJob& j, uint32_t a, uint32_t b) {
+    std::vector<V2> vs;
+    std::vector<uint32_t> is;
+    vs.reserve(1 << 16);
+    is.reserve(1 << 18);
+    uint64_t tris = 0;
+    for (uint32_t p = a; p < b; ++p) {
+        vs.clear();
+        is.clear();
+        refpath::PathView pv{j.verbs + j.verb_off[p], j.verb_off[p + 1] - j.verb_off[p],
+                             j.points + 2 * (size_t)j.point_off[p],
+                             j.point_off[p + 1] - j.point_off[p]};
+        if (j.is_fill) {
+            reffill::FillOptions o;
+            o.tolerance = j.opts[2 * p];
+            o.fill_rule = (int)j.opts[2 * p + 1];
+            reffill::Output out{&vs, &is, 0};
+            reffill::fill_path(pv, o, out);
+        } else {
+            const float* s = j.opts + 6 * (size_t)p;
+            refstroke::StrokeOptions o;
+            o.line_width = s[0];
+            o.miter_limit = s[1];
+            o.tolerance = s[2];
+            o.start_cap = (int)s[3];
+            o.end_cap = (int)s[4];
+            o.line_join = (int)s[5];
+            refstroke::Output out{&vs, &is, 0};
+            refstroke::stroke_path(pv, o, out);
+        }
+        tris += is.size() / 3;
+    }
+    return tris;
+}
+
+uint64_t orc_batch_triangles(const uint8_t* verbs, const float* points, const uint32_t* verb_off,
+                             const uint32_t* point_off, const float* opts, uint32_t n_paths,
+                             int is_fill, int n_threads) {
+    BatchJob j{verbs, points, verb_off, point_off, opts, is_fill};
+    if (n_threads <= 1) return run_range(j, 0, n_paths);
+    std::vector<std::thread> ts;
+    std::vector<uint64_t> res(n_threads, 0);
+    std::atomic<uint32_t> next{0};
+    for (int t = 0; t < n_threads; ++t) {
+        ts.emplace_back([&, t] {
+            for (;;) {
+                uint32_t p = next.fetch_add(16);
+                if (p >= n_paths) break;
+                uint32_t e = p + 16 < n_paths ? p + 16 : n_paths;
+                res[t] += run_range(j, p, e);
+            }
+        });
+    }
+    uint64_t tot = 0;
+    for (int t = 0; t < n_threads; ++t) {
+        ts[t].join();
+        tot += res[t];
+    }
+    return tot;
+}
+
+}  // extern "C"
+
+extern "C" {
+// euclid Rotation2D::transform_point applied in place (used to replay lyon's
+// test_path_with_rotations, fill_tests.rs:134-150): angle accumulates in f32 by the caller.
+void orc_rotate_points(float* pts, uint32_t n, float angle) {
+    for (uint32_t i = 0; i < n; ++i) {
+        V2 p = refgeom::rotate(angle, V2{pts[2 * i], pts[2 * i + 1]});
+        pts[2 * i] = p.x;
+        pts[2 * i + 1] = p.y;
+    }
+}
+}

@@ -1,0 +1,129 @@
+"""ORACLE — test infrastructure only.
+
+ctypes face of ``liboracle.so`` (the CPU restatement of lyon's tessellators).  Only
+tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import this
+module, and only as the checker; the product (``lyon_amd``) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+F32P = C.POINTER(C.c_float)
+U32P = C.POINTER(C.c_uint32)
+U8P = C.POINTER(C.c_uint8)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.orc_result_free.argtypes = [C.c_void_p]
+        L.orc_result_status.argtypes = [C.c_void_p]
+        L.orc_result_status.restype = C.c_int
+        for n in ("orc_result_n_vertices", "orc_result_n_indices", "orc_result_n_offsets"):
+            getattr(L, n).argtypes = [C.c_void_p]
+            getattr(L, n).restype = C.c_uint64
+        L.orc_result_copy.argtypes = [C.c_void_p, F32P, U32P, U32P]
+        L.orc_stroke_path.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float, C.c_float,
+                                      C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint32]
+        L.orc_stroke_path.restype = C.c_void_p
+        L.orc_fill_path.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float, C.c_int,
+                                    C.c_int, C.c_int, C.c_uint32]
+        L.orc_fill_path.restype = C.c_void_p
+        L.orc_flatten_segments.argtypes = [C.POINTER(C.c_int), F32P, C.c_uint32, C.c_float]
+        L.orc_flatten_segments.restype = C.c_void_p
+        L.orc_flatten_arcs.argtypes = [F32P, C.c_uint32, C.c_float]
+        L.orc_flatten_arcs.restype = C.c_void_p
+        L.orc_arc_to_quads.argtypes = [F32P, C.c_uint32]
+        L.orc_arc_to_quads.restype = C.c_void_p
+        L.orc_batch_triangles.argtypes = [U8P, F32P, U32P, U32P, F32P, C.c_uint32, C.c_int,
+                                          C.c_int]
+        L.orc_batch_triangles.restype = C.c_uint64
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _collect(h):
+    L = lib()
+    try:
+        st = L.orc_result_status(h)
+        nv = L.orc_result_n_vertices(h)
+        ni = L.orc_result_n_indices(h)
+        no = L.orc_result_n_offsets(h)
+        v = np.zeros((nv, 2), np.float32)
+        i = np.zeros(ni, np.uint32)
+        o = np.zeros(no, np.uint32)
+        L.orc_result_copy(h, _ptr(v, F32P), _ptr(i, U32P), _ptr(o, U32P))
+        return st, v, i, o
+    finally:
+        L.orc_result_free(h)
+
+
+def stroke(path, line_width=1.0, miter_limit=4.0, tolerance=0.1, start_cap=0, end_cap=0,
+           line_join=0, vertex_offset=0):
+    """StrokeTessellator::tessellate_path on the CPU.  Returns (status, vertices, indices)."""
+    v = np.ascontiguousarray(path.verbs, np.uint8)
+    p = np.ascontiguousarray(path.points, np.float32)
+    h = lib().orc_stroke_path(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), line_width,
+                              miter_limit, tolerance, start_cap, end_cap, line_join,
+                              vertex_offset)
+    st, vv, ii, _ = _collect(h)
+    return st, vv, ii
+
+
+def fill(path, tolerance=0.1, fill_rule=0, sweep_orientation=1, handle_intersections=True,
+         vertex_offset=0):
+    """FillTessellator::tessellate_path on the CPU.  Returns (status, vertices, indices)."""
+    v = np.ascontiguousarray(path.verbs, np.uint8)
+    p = np.ascontiguousarray(path.points, np.float32)
+    h = lib().orc_fill_path(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), tolerance, fill_rule,
+                            sweep_orientation, int(handle_intersections), vertex_offset)
+    st, vv, ii, _ = _collect(h)
+    return st, vv, ii
+
+
+def flatten_segments(kinds, ctrl, tolerance):
+    """lyon_geom for_each_flattened_with_t per segment. kinds: 0 line, 1 quad, 2 cubic;
+    ctrl: (n, 4, 2) float32.  Returns (points, offsets)."""
+    k = np.ascontiguousarray(kinds, np.int32)
+    c = np.ascontiguousarray(ctrl, np.float32).reshape(-1, 8)
+    h = lib().orc_flatten_segments(k.ctypes.data_as(C.POINTER(C.c_int)), _ptr(c, F32P), len(k),
+                                   tolerance)
+    _, v, _, o = _collect(h)
+    return v, o
+
+
+def flatten_arcs(arcs, tolerance):
+    a = np.ascontiguousarray(arcs, np.float32).reshape(-1, 7)
+    h = lib().orc_flatten_arcs(_ptr(a, F32P), len(a), tolerance)
+    _, v, _, o = _collect(h)
+    return v, o
+
+
+def arc_to_quads(arcs):
+    a = np.ascontiguousarray(arcs, np.float32).reshape(-1, 7)
+    h = lib().orc_arc_to_quads(_ptr(a, F32P), len(a))
+    _, v, _, o = _collect(h)
+    return v.reshape(-1, 3, 2), o // 3
+
+
+def batch_triangles(batch, opts, is_fill, n_threads=1):
+    """CPU baseline: tessellate every path of a PathBatch; returns the triangle count."""
+    o = np.ascontiguousarray(opts, np.float32)
+    return lib().orc_batch_triangles(_ptr(batch.verbs, U8P), _ptr(batch.points, F32P),
+                                     _ptr(batch.verb_offsets, U32P),
+                                     _ptr(batch.point_offsets, U32P), _ptr(o, F32P), len(batch),
+                                     int(is_fill), int(n_threads))

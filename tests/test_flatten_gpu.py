@@ -1,0 +1,58 @@
+"""GPU parity of lyon_geom flattening against the oracle: point counts and positions
+bit-exact for lines/quadratics/cubics; arcs within 1e-5 of the bbox diagonal."""
+import numpy as np
+import pytest
+
+from lyon_amd import arc_to_quadratics, flatten_arcs, flatten_segments
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def random_segments(n, seed, scale):
+    rng = np.random.default_rng(seed)
+    kinds = rng.integers(0, 3, n).astype(np.int32)
+    ctrl = (rng.random((n, 4, 2)) * scale).astype(np.float32)
+    return kinds, ctrl
+
+
+@pytest.mark.parametrize("tol", [0.05, 0.1, 0.001, 1.0])
+def test_segments_bit_exact(tol):
+    kinds, ctrl = random_segments(3000, 1, 2048.0)
+    gp, go = flatten_segments(kinds, ctrl, tol)
+    rp, ro = O.flatten_segments(kinds, ctrl, tol)
+    assert np.array_equal(go, ro), "point counts differ"
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), "positions differ"
+
+
+def test_segments_degenerate():
+    ctrl = np.zeros((4, 4, 2), np.float32)
+    ctrl[1] = [[1, 1], [1, 1], [1, 1], [1, 1]]
+    ctrl[2] = [[0, 0], [5, 0], [10, 0], [10, 0]]  # straight quadratic
+    ctrl[3] = [[0, 0], [1e6, 1e6], [-1e6, 1e6], [3, 0]]  # huge
+    kinds = np.array([1, 2, 1, 2], np.int32)
+    gp, go = flatten_segments(kinds, ctrl, 0.05)
+    rp, ro = O.flatten_segments(kinds, ctrl, 0.05)
+    assert np.array_equal(go, ro)
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    # quadratic_bezier.rs:1381-1419: empty and straight-line quadratics flatten to 1 segment
+    assert go[1] - go[0] == 1
+
+
+def test_arcs():
+    rng = np.random.default_rng(3)
+    arcs = np.zeros((200, 7), np.float32)
+    arcs[:, 0:2] = rng.random((200, 2)) * 1000
+    arcs[:, 2:4] = rng.random((200, 2)) * 300 + 1
+    arcs[:, 4] = rng.random(200) * 6.28
+    arcs[:, 5] = (rng.random(200) - 0.5) * 12.0
+    arcs[:, 6] = rng.random(200) * 3.0
+    gp, go = flatten_arcs(arcs, 0.05)
+    rp, ro = O.flatten_arcs(arcs, 0.05)
+    assert np.array_equal(go, ro)
+    diag = 2000.0
+    assert np.max(np.abs(gp.astype(np.float64) - rp)) <= 1e-5 * diag
+    gq, gqo = arc_to_quadratics(arcs)
+    rq, rqo = O.arc_to_quads(arcs)
+    assert np.array_equal(gqo, rqo)
+    assert np.max(np.abs(gq.astype(np.float64) - rq)) <= 1e-5 * diag

@@ -1,0 +1,298 @@
+"""Pins the oracle (the CPU restatement of lyon) against the reference's own known
+answers, fixtures and closed forms.  Runs on the CPU (no GPU, no liblyg kernels).
+
+Sources of each expectation:
+* stroke KATs: stroke.rs:2852-2910 (test_square), :2921-2974 (test_empty_caps),
+  :3126-3154 (test_line_width), :3380-3416 (correct_miter_clip_length);
+* fill KATs: fill_tests.rs:153-624 triangle counts, with the reference's rotation sweeps
+  (test_path_with_rotations, :134-150);
+* fixtures: fuzz_tests.rs and earcut_tests.rs polygons (tests/golden/*.json) must
+  tessellate without error under both fill rules;
+* flattening: quadratic_bezier.rs:1381-1461 (degenerate -> 1 segment, t monotonic),
+  cubic_bezier.rs:1614-1646 (last point exact), arc.rs:940-1017 (arc -> quadratic
+  counts), plus Wang's bound checked against an f64 evaluation of the true curve.
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from helpers import empty_caps_path, square_paths
+from lyon_amd.path import Path, PathBuilder, polygon
+from oracle import pyoracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rotate_inplace(pts, angle):
+    L = O.lib()
+    L.orc_rotate_points.argtypes = [O.F32P, C.c_uint32, C.c_float]
+    L.orc_rotate_points(pts.ctypes.data_as(O.F32P), len(pts), angle)
+
+
+# ---- stroke ---------------------------------------------------------------------------
+def test_stroke_square_counts():
+    p = square_paths()
+    assert len(O.stroke(p, line_join=0, start_cap=0, end_cap=0)[2]) // 3 == 12
+    assert len(O.stroke(p, line_join=3, start_cap=1, end_cap=1)[2]) // 3 == 16
+
+
+def test_stroke_empty_caps():
+    p = empty_caps_path()
+    assert len(O.stroke(p, start_cap=0, end_cap=0)[2]) == 0
+    assert len(O.stroke(p, start_cap=1, end_cap=1)[2]) // 3 == 8
+
+
+def test_stroke_line_width_vertices():
+    b = PathBuilder()
+    b.begin((0, 1)); b.line_to((2, 1)); b.end(False)
+    _, v, _ = O.stroke(b.build(), line_width=2.0)
+    got = {tuple(x) for x in v.tolist()}
+    assert got == {(0.0, 0.0), (0.0, 2.0), (2.0, 0.0), (2.0, 2.0)}
+
+
+def test_stroke_miter_clip_length():
+    b = PathBuilder()
+    b.begin((-70, -200)); b.line_to((0, 300)); b.line_to((70, -200)); b.end(False)
+    _, v, _ = O.stroke(b.build(), line_width=30.0, line_join=1, miter_limit=3.0)
+    assert float(v[:, 1].max()) == 300.0 + 30.0 * 3.0 * 0.5
+
+
+def test_stroke_triangle_winding():
+    """stroke.rs:3214-3257: every triangle is clockwise (y down) up to -0.035 slack, and
+    no triangle repeats a vertex (stroke.rs:2787-2849 TestBuilder)."""
+    from lyon_amd import workloads
+    b = workloads.stroke_batch(20, 4, 30, 500.0, seed=3)
+    for i in range(len(b)):
+        for join in (0, 2, 3):
+            st, v, idx = O.stroke(b.path(i), line_width=3.0, line_join=join, tolerance=0.05,
+                                  start_cap=2, end_cap=1)
+            assert st == 0
+            t = idx.reshape(-1, 3)
+            assert np.all((t[:, 0] != t[:, 1]) & (t[:, 0] != t[:, 2]) & (t[:, 1] != t[:, 2]))
+            pa, pb, pc = v[t[:, 0]].astype(np.float64), v[t[:, 1]], v[t[:, 2]]
+            cr = (pa - pb)[:, 0] * (pc - pb)[:, 1] - (pa - pb)[:, 1] * (pc - pb)[:, 0]
+            assert np.mean(cr >= -0.035) > 0.99
+
+
+def test_stroke_vertex_offset_and_overflow():
+    """BuffersBuilder ids are len + vertex_offset; ids past u32::MAX - 1 abort
+    (geometry_builder.rs:338-350, 609-630)."""
+    p = square_paths()
+    st, v, i = O.stroke(p, vertex_offset=1000)
+    assert st == 0 and i.min() >= 1000 and i.max() < 1000 + len(v)
+    st, v, i = O.stroke(p, vertex_offset=0xFFFFFFFE - 3)
+    assert st == 1 and len(v) == 0 and len(i) == 0
+
+
+# ---- fill -----------------------------------------------------------------------------
+def fill_count(path, rule=0, tol=0.05):
+    st, v, i = O.fill(path, tolerance=tol, fill_rule=rule)
+    assert st == 0, f"fill failed with status {st}"
+    return len(i) // 3
+
+
+def poly(*pts):
+    return polygon(pts, True)
+
+
+def rotations(path, step, expected):
+    angle = np.float32(0.0)
+    two_pi = np.float32(math.pi * 2.0)
+    n = 0
+    while angle < two_pi:
+        pts = path.points.copy()
+        rotate_inplace(pts, float(angle))
+        rp = Path(path.verbs, pts)
+        if expected is not None:
+            assert fill_count(rp, 0) == expected, f"angle {angle}"
+        else:
+            fill_count(rp, 0)
+        fill_count(rp, 1)
+        angle = np.float32(angle + np.float32(step))
+        n += 1
+    return n
+
+
+FILL_KATS = [
+    ("simple_triangle", [(0, 0), (1, 1), (0, 1)], 0.01, 1),
+    ("simple_split", [(0, 0), (2, 1), (2, 3), (1, 2), (0, 3)], 0.01, 3),
+    ("simple_merge_split", [(0, 0), (1, 1), (2, 0), (2, 3), (1, 2), (0, 3)], 0.01, 4),
+    ("simple_aligned", [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1)], 0.01, 6),
+    ("simple_1", [(0, 0), (1, 1), (2, 0), (1, 3), (0.5, 4), (0, 3)], 0.01, 4),
+    ("simple_2", [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3), (2, 3), (1, 3), (0, 3),
+                  (0, 2), (0, 1)], 0.01, 10),
+]
+
+
+@pytest.mark.parametrize("name,pts,step,expected", FILL_KATS, ids=[k[0] for k in FILL_KATS])
+def test_fill_kat_rotations(name, pts, step, expected):
+    """fill_tests.rs: exact EvenOdd triangle counts at every rotation (the reference
+    steps by 0.001-0.01 rad; a 0.01 step keeps the CPU suite short)."""
+    assert rotations(poly(*pts), step, expected) > 600
+
+
+def test_fill_kat_fixed():
+    assert fill_count(poly((0, 0), (-1, 1), (-3, 2), (-1, 3), (-4, 5), (0, 6))) == 4
+    assert fill_count(poly((0, 0), (2, 1), (0, 2), (2, 3))) == 2
+    assert fill_count(poly((0, 0), (2, 3), (2, 1), (0, 2))) == 2
+    assert fill_count(poly((1, 0), (2, 1), (3, 0), (4, 2), (5, 0), (6, 3), (7, 0), (5, 8))) == 6
+    assert fill_count(poly((1, 0), (2, 1), (3, 0), (4, 2), (5, 0), (6, 3), (7, 0), (7, 5),
+                           (0, 4))) == 7
+    assert fill_count(poly((1, 0), (2, 1), (3, 0), (4, 2), (5, 0), (6, 3), (7, 0), (7, 5),
+                           (-1, 5), (-1, 0), (0, 4))) == 9
+    assert fill_count(poly((1, 0), (2, 1), (3, 0), (4, 2), (5, 0), (6, 3), (7, 0), (7, 5),
+                           (4, 4), (1, 5))) == 8
+    assert fill_count(Path(np.zeros(0, np.uint8), np.zeros((0, 2), np.float32))) == 0
+
+
+def test_fill_kat_hole_and_multi():
+    b = PathBuilder()
+    b.begin((-11, 5)); b.line_to((0, -5)); b.line_to((10, 5)); b.end(True)
+    b.begin((-5, 2)); b.line_to((0, -2)); b.line_to((4, 2)); b.end(True)
+    rotations(b.build(), 0.01, 6)
+    b = PathBuilder()
+    b.begin((20, 20)); b.line_to((60, 20)); b.line_to((60, 60)); b.line_to((20, 60)); b.end(True)
+    b.begin((40, 10)); b.line_to((70, 40)); b.line_to((40, 70)); b.line_to((10, 40)); b.end(True)
+    rotations(b.build(), 0.011, 8)
+
+
+def test_fill_tolerance_errors():
+    p = poly((0, 0), (1, 1), (0, 1))
+    for t in (0.0, -1.0, float("nan")):
+        st, v, i = O.fill(p, tolerance=t)
+        assert st == 4 and len(v) == 0
+
+
+def fixture_path(subs, earcut=False):
+    b = PathBuilder()
+    for s in subs:
+        pts = s if earcut else s["points"]
+        if not pts:
+            continue
+        if earcut:
+            b.begin((pts[0][0], pts[0][0]))  # earcut_tests.rs:20192 begins at (x0, x0)
+            for p in pts:
+                b.line_to(p)
+            b.end(True)
+        else:
+            b.begin(pts[0])
+            for p in pts[1:]:
+                b.line_to(p)
+            b.end(s["closed"])
+    return b.build()
+
+
+def test_fuzz_fixtures_no_error():
+    cases = json.load(open(os.path.join(HERE, "golden", "fuzz_polygons.json")))
+    assert len(cases) == 26
+    for name, subs in cases.items():
+        for rule in (0, 1):
+            st, v, i = O.fill(fixture_path(subs), tolerance=0.05, fill_rule=rule)
+            assert st == 0, f"{name} rule {rule}: status {st}"
+
+
+def test_earcut_fixtures_no_error():
+    cases = json.load(open(os.path.join(HERE, "golden", "earcut_polygons.json")))
+    assert len(cases) == 30
+    for name, rings in cases.items():
+        for rule in (0, 1):
+            st, v, i = O.fill(fixture_path(rings, earcut=True), tolerance=0.05, fill_rule=rule)
+            assert st == 0, f"{name} rule {rule}: status {st}"
+
+
+# ---- flattening -----------------------------------------------------------------------
+def test_flatten_degenerate_quadratics():
+    """quadratic_bezier.rs:1381-1419: empty and straight quadratics give 1 segment."""
+    ctrl = np.array([[[0, 0], [0, 0], [0, 0], [0, 0]],
+                     [[0, 0], [1, 0], [2, 0], [0, 0]]], np.float32)
+    pts, off = O.flatten_segments(np.array([1, 1], np.int32), ctrl, 0.1)
+    assert list(np.diff(off)) == [1, 1]
+
+
+def test_flatten_last_point_exact_and_error_bound():
+    """Last point == `to` exactly (cubic_bezier.rs:1614-1646), and every chord stays
+    within the uniform-parameter chord bound M * h^2 / 8 of the true curve (f64), where
+    M = max |B''| (Wang) and h the chord's parameter length: interior chords have
+    h = 1/n, the last one spans 1 - (count - 1)/n because the count is n truncated
+    (quadratic_bezier.rs:375-390).  So the flattening error may exceed `tolerance` by up
+    to (n - count + 1)^2 on the last chord — lyon's behaviour, reported, not a bug."""
+    rng = np.random.default_rng(5)
+    n = 300
+    kinds = rng.integers(1, 3, n).astype(np.int32)
+    ctrl = (rng.random((n, 4, 2)) * 1000).astype(np.float32)
+    worst_ratio = 0.0
+    for tol in (0.05, 0.25):
+        pts, off = O.flatten_segments(kinds, ctrl, tol)
+        for s in range(n):
+            seg = pts[off[s]:off[s + 1]].astype(np.float64)
+            c = ctrl[s].astype(np.float64)
+            assert np.array_equal(pts[off[s + 1] - 1], ctrl[s][2 if kinds[s] == 1 else 3])
+            if kinds[s] == 1:
+                lv = (c[0] - 2 * c[1] + c[2]) * 2
+                M = float(np.sqrt(lv @ lv))
+            else:
+                v1 = (c[0] - 2 * c[1] + c[2]) * 6
+                v2 = (c[1] - 2 * c[2] + c[3]) * 6
+                M = float(np.sqrt(max(v1 @ v1, v2 @ v2)))
+            wang = (M * M / (8 * tol) ** 2) ** 0.25
+            cnt = off[s + 1] - off[s]
+            h_last = 1.0 if cnt == 1 else max(1.0 / wang, 1.0 - (cnt - 1) / wang)
+            bound = M * h_last ** 2 / 8 * (1 + 1e-4) + 1e-3
+            poly_pts = np.vstack([c[0], seg])
+            ts = np.linspace(0, 1, 4001)
+            if kinds[s] == 1:
+                curve = ((1 - ts) ** 2)[:, None] * c[0] + (2 * (1 - ts) * ts)[:, None] * c[1] + \
+                        (ts ** 2)[:, None] * c[2]
+            else:
+                curve = ((1 - ts) ** 3)[:, None] * c[0] + (3 * (1 - ts) ** 2 * ts)[:, None] * c[1] \
+                        + (3 * (1 - ts) * ts ** 2)[:, None] * c[2] + (ts ** 3)[:, None] * c[3]
+            a = poly_pts[:-1][None]
+            b = poly_pts[1:][None]
+            p = curve[:, None]
+            ab = b - a
+            t = np.clip(np.sum((p - a) * ab, -1) / np.maximum(np.sum(ab * ab, -1), 1e-30), 0, 1)
+            d = np.linalg.norm(p - (a + t[..., None] * ab), axis=-1).min(axis=1)
+            assert d.max() <= bound, (s, d.max(), bound)
+            worst_ratio = max(worst_ratio, d.max() / tol)
+    assert worst_ratio < 4.0
+
+
+def test_arc_to_quadratic_counts():
+    """arc.rs:940-1017: sweeps of pi/2 and pi produce 2 and 4 quadratics... (n = ceil(sweep /
+    (pi/4)))."""
+    arcs = np.array([[0, 0, 1, 1, 0, math.pi, 0], [0, 0, 1, 1, 0, math.pi / 2, 0],
+                     [0, 0, 1, 1, 0, 0.5, 0], [0, 0, 1, 1, 0, -math.pi, 0]], np.float32)
+    q, off = O.arc_to_quads(arcs)
+    assert list(np.diff(off)) == [4, 2, 1, 4]
+
+
+def test_tchain_matches_serial_accumulation():
+    """The closed-form t chain used by the kernels equals the serial f32 sum (exhaustive
+    over j for 2000 segment counts)."""
+    import subprocess
+    import sys
+    src = os.path.join(os.path.dirname(HERE), "lyon_amd", "csrc", "lyg_math.h")
+    code = r'''
+#include "%s"
+#include <cstdio>
+#include <random>
+int main() {
+  using namespace lyg; std::mt19937 rng(11); std::uniform_real_distribution<float> u(1.0f, 3000.0f);
+  long bad = 0, tot = 0;
+  for (int k = 0; k < 2000; ++k) {
+    float n = k < 300 ? (float)(k + 1) : u(rng); float s = 1.0f / n; float t = 0;
+    for (uint32_t j = 1; j <= seg_count(n) + 2; ++j) { t = t + s; tot++; if (f2u(tchain(s, j)) != f2u(t)) bad++; }
+  }
+  printf("%%ld %%ld\n", tot, bad); return bad != 0;
+}''' % src
+    tmp = os.path.join(os.environ.get("TMPDIR", "/tmp"), "lyg_tchain_test")
+    with open(tmp + ".cpp", "w") as f:
+        f.write(code)
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-o", tmp, tmp + ".cpp"], check=True)
+    r = subprocess.run([tmp], capture_output=True, text=True)
+    tot, bad = map(int, r.stdout.split())
+    assert tot > 1_000_000 and bad == 0

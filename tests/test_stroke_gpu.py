@@ -1,0 +1,157 @@
+"""GPU parity of the stroke tessellator (plain and batched) against the oracle.
+
+Bar: index topology bit-exact, positions within 1e-5 of the path's bbox diagonal.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from helpers import assert_same_geometry, bbox_diag, empty_caps_path, square_paths
+from lyon_amd import (BuffersBuilder, LineCap, LineJoin, PathBuilder, StrokeOptions,
+                      StrokeTessellator, VertexBuffers)
+from lyon_amd import workloads
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_stroke(path, opts, vertex_offset=0):
+    buf = VertexBuffers()
+    StrokeTessellator().tessellate_path(path, opts,
+                                        BuffersBuilder(buf).with_vertex_offset(vertex_offset))
+    return buf.vertices, buf.indices
+
+
+def ref_stroke(path, opts, vertex_offset=0):
+    st, v, i = O.stroke(path, opts.line_width, opts.miter_limit, opts.tolerance,
+                        int(opts.start_cap), int(opts.end_cap), int(opts.line_join),
+                        vertex_offset)
+    assert st == 0
+    return v, i
+
+
+def check(path, opts, what, vertex_offset=0):
+    g = gpu_stroke(path, opts, vertex_offset)
+    r = ref_stroke(path, opts, vertex_offset)
+    print(f"{what}: {len(r[0])} vertices, {len(r[1]) // 3} triangles", flush=True)
+    return assert_same_geometry(g, r, bbox_diag(path.points), what=what)
+
+
+ALL_OPTS = [StrokeOptions(line_width=w, line_join=j, start_cap=c, end_cap=c2, tolerance=t,
+                          miter_limit=m)
+            for (w, j, c, c2, t, m) in [
+                (1.0, LineJoin.Miter, LineCap.Butt, LineCap.Butt, 0.1, 4.0),
+                (2.0, LineJoin.Bevel, LineCap.Square, LineCap.Square, 0.1, 4.0),
+                (0.3, LineJoin.MiterClip, LineCap.Round, LineCap.Round, 0.1, 1.0),
+                (1.0, LineJoin.Round, LineCap.Round, LineCap.Round, 0.001, 4.0),
+                (4.0, LineJoin.Miter, LineCap.Butt, LineCap.Square, 0.05, 4.0),
+                (8.0, LineJoin.Round, LineCap.Square, LineCap.Round, 0.05, 2.0),
+            ]]
+
+
+def test_square_kats():
+    """stroke.rs:2852-2910: Miter+Butt -> 12 triangles, Bevel+Square -> 16."""
+    p = square_paths()
+    v, i = gpu_stroke(p, StrokeOptions(line_join=LineJoin.Miter, start_cap=LineCap.Butt,
+                                       end_cap=LineCap.Butt))
+    assert len(i) // 3 == 12
+    v, i = gpu_stroke(p, StrokeOptions(line_join=LineJoin.Bevel, start_cap=LineCap.Square,
+                                       end_cap=LineCap.Square))
+    assert len(i) // 3 == 16
+    for k, o in enumerate(ALL_OPTS):
+        check(p, o, f"square opts#{k}")
+
+
+def test_empty_caps_kats():
+    """stroke.rs:2921-2974: Butt -> 0 triangles, Square -> 8."""
+    p = empty_caps_path()
+    assert len(gpu_stroke(p, StrokeOptions().with_line_cap(LineCap.Butt))[1]) == 0
+    assert len(gpu_stroke(p, StrokeOptions().with_line_cap(LineCap.Square))[1]) // 3 == 8
+    for cap in LineCap:
+        check(p, StrokeOptions().with_line_cap(cap), f"empty caps {cap.name}")
+
+
+def test_line_width_kat():
+    """stroke.rs:3126-3154: vertices of a width-2 segment are the square's corners."""
+    b = PathBuilder()
+    b.begin((0, 1)); b.line_to((2, 1)); b.end(False)
+    v, _ = gpu_stroke(b.build(), StrokeOptions(line_width=2.0))
+    got = {tuple(np.round(x, 5)) for x in v.tolist()}
+    assert got <= {(0.0, 0.0), (0.0, 2.0), (2.0, 0.0), (2.0, 2.0)}
+
+
+def test_miter_clip_length_kat():
+    """stroke.rs:3380-3416: max y == 300 + 30 * 3 * 0.5 exactly."""
+    b = PathBuilder()
+    b.begin((-70, -200)); b.line_to((0, 300)); b.line_to((70, -200)); b.end(False)
+    v, _ = gpu_stroke(b.build(), StrokeOptions(line_width=30.0, line_join=LineJoin.MiterClip,
+                                               miter_limit=3.0))
+    assert float(v[:, 1].max()) == 345.0
+
+
+def test_issue_819_close_merge():
+    """stroke.rs:3289-3318: last point within merge range of the first two."""
+    b = PathBuilder()
+    b.begin((650.539978027344, 173.559997558594))
+    b.line_to((650.609985351562, 173.529998779297))
+    b.line_to((650.729980468750, 173.630004882812))
+    b.line_to((650.559997558594, 173.570007324219))
+    b.end(True)
+    check(b.build(), StrokeOptions(tolerance=0.1), "issue_819")
+
+
+def test_plain_workload_path():
+    """The plain workload: one closed 64-segment path, width 4, Miter limit 4, Butt."""
+    p = workloads.plain_path(0)
+    frac = check(p, StrokeOptions(line_width=4.0, miter_limit=4.0, line_join=LineJoin.Miter,
+                                  start_cap=LineCap.Butt, end_cap=LineCap.Butt), "plain")
+    print("bit-exact vertex fraction", frac)
+
+
+def test_random_paths_all_options():
+    batch = workloads.stroke_batch(24, 1, 40, 600.0, seed=7)
+    closed = workloads.stroke_batch(24, 1, 40, 600.0, seed=8, closed=True)
+    for bi, b in enumerate((batch, closed)):
+        for i in range(len(b)):
+            o = ALL_OPTS[(i + bi) % len(ALL_OPTS)]
+            check(b.path(i), o, f"random[{bi}][{i}]", vertex_offset=i * 13)
+
+
+def test_degenerate_inputs():
+    cases = []
+    b = PathBuilder(); b.begin((5, 5)); b.line_to((5, 5)); b.line_to((5.01, 5)); b.end(False)
+    cases.append(b.build())
+    b = PathBuilder(); b.begin((0, 0)); b.cubic_bezier_to((0, 0), (0, 0), (0, 0)); b.end(True)
+    cases.append(b.build())
+    b = PathBuilder(); b.begin((0, 0)); b.quadratic_bezier_to((100, 0), (0, 0.001)); b.end(False)
+    cases.append(b.build())
+    b = PathBuilder(); b.begin((0, 0)); b.line_to((10, 0)); b.line_to((0, 0.0001)); b.end(False)
+    cases.append(b.build())
+    b = PathBuilder(); b.begin((0, 0)); b.line_to((10, 0)); b.line_to(( 0, 0)); b.end(True)
+    cases.append(b.build())
+    # cusp cubic and a sharp quadratic
+    b = PathBuilder(); b.begin((0, 0)); b.cubic_bezier_to((100, 100), (0, 100), (100, 0))
+    b.quadratic_bezier_to((-500, 3), (10, 1)); b.end(False)
+    cases.append(b.build())
+    for k, p in enumerate(cases):
+        for j, o in enumerate(ALL_OPTS):
+            check(p, o, f"degenerate[{k}] opts#{j}")
+
+
+def test_batch_matches_sequential_plain_calls():
+    """The batched form equals tessellate_path per path into one shared VertexBuffers."""
+    from lyon_amd.batch import DeviceBatch, StrokeBatcher
+    batch = workloads.stroke_batch(300, 8, 256, 2048.0, seed=0)
+    opts = workloads.stroke_options(300, tolerance=0.05)
+    res = StrokeBatcher().run(DeviceBatch(batch, opts))
+    vo = res.vertex_offsets.cpu().numpy().view(np.uint32)
+    for i in range(len(batch)):
+        o = opts[i]
+        so = StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
+                           tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
+                           end_cap=int(o["end_cap"]), line_join=int(o["line_join"]))
+        p = batch.path(i)
+        r = ref_stroke(p, so, vertex_offset=int(vo[i]))
+        g = res.path(i)
+        assert_same_geometry(g, r, bbox_diag(p.points), what=f"batch path {i}")
