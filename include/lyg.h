@@ -113,6 +113,11 @@ void lyg_ctx_destroy(lyg_ctx* ctx);
 const char* lyg_status_string(int32_t status);
 /* Last HIP error text recorded by the context (empty if none). */
 const char* lyg_ctx_last_error(const lyg_ctx* ctx);
+/* Measurement support (no lyon counterpart): when enabled, HIP events are recorded on the
+ * context's stream around the count and emit kernels of every stroke call;
+ * lyg_ctx_kernel_times returns the durations of the last call in milliseconds. */
+int32_t lyg_ctx_enable_timing(lyg_ctx* ctx, int enable);
+int32_t lyg_ctx_kernel_times(lyg_ctx* ctx, float* count_ms, float* emit_ms);
 
 /* ---- plain per-path drop-in (host buffers in and out) --------------------------------
  * StrokeTessellator::tessellate_path (crates/tessellation/src/stroke.rs:164-177). */

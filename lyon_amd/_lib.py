@@ -60,6 +60,8 @@ SIGNATURES = {
     "lyg_ctx_destroy": ([C.c_void_p], None),
     "lyg_status_string": ([C.c_int32], C.c_char_p),
     "lyg_ctx_last_error": ([C.c_void_p], C.c_char_p),
+    "lyg_ctx_enable_timing": ([C.c_void_p, C.c_int], C.c_int32),
+    "lyg_ctx_kernel_times": ([C.c_void_p, F32P, F32P], C.c_int32),
     "lyg_stroke_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
                          C.POINTER(OutputC)], C.c_int32),
     "lyg_fill_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),

@@ -49,6 +49,9 @@ struct lyg_ctx {
     uint32_t* d_counters = nullptr;  // [0..3] work-queue heads, [4] overflow flag
     uint64_t* d_totals = nullptr;    // [0] vertices, [1] indices
     void* h_pinned = nullptr;        // small pinned staging area for totals
+    // optional kernel timing: events around the count and emit kernels on `stream`
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -106,8 +109,10 @@ int32_t stroke_count_and_scan(lyg_ctx* c, lyg::StrokeLaunch& L, uint32_t* offs_v
     const uint32_t n = L.in.n_paths;
     if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, c->stream), "reset")) return LYG_ERR_DEVICE;
     L.work_counter = c->d_counters + 0;
+    if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
     if (!check(c, lyg_launch_stroke(&L, 0, persistent_grid(c, n), c->stream), "stroke count"))
         return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
     if (!check(c, lyg_launch_scan(L.path_nv, n, offs_v, c->d_totals + 0, c->d_counters + 4, c->stream),
                "scan"))
         return LYG_ERR_DEVICE;
@@ -133,9 +138,11 @@ int32_t stroke_emit(lyg_ctx* c, lyg::StrokeLaunch& L) {
         return LYG_ERR_DEVICE;
     L.work_counter = c->d_counters + 1;
     L.err = c->d_counters + 5;
+    if (c->timing) (void)hipEventRecord(c->ev[2], c->stream);
     if (!check(c, lyg_launch_stroke(&L, 1, persistent_grid(c, L.in.n_paths), c->stream),
                "stroke emit"))
         return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[3], c->stream);
     uint32_t* h = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 128);
     if (!check(c, hipMemcpyAsync(h, c->d_counters + 5, 4, hipMemcpyDeviceToHost, c->stream),
                "copy"))
@@ -198,6 +205,8 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_totals) (void)hipFree(c->d_totals);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    for (int i = 0; i < 4; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -220,6 +229,28 @@ const char* lyg_status_string(int32_t s) {
 }
 
 const char* lyg_ctx_last_error(const lyg_ctx* c) { return c ? c->last_error.c_str() : ""; }
+
+int32_t lyg_ctx_enable_timing(lyg_ctx* c, int enable) {
+    if (!c) return LYG_ERR_INVALID_ARGUMENT;
+    if (enable && !c->ev[0]) {
+        for (int i = 0; i < 4; ++i)
+            if (!check(c, hipEventCreate(&c->ev[i]), "hipEventCreate")) return LYG_ERR_DEVICE;
+    }
+    c->timing = enable != 0;
+    return LYG_OK;
+}
+
+int32_t lyg_ctx_kernel_times(lyg_ctx* c, float* count_ms, float* emit_ms) {
+    if (!c || !c->timing) return LYG_ERR_INVALID_ARGUMENT;
+    if (!check(c, hipEventSynchronize(c->ev[3]), "hipEventSynchronize")) return LYG_ERR_DEVICE;
+    float a = 0, b = 0;
+    if (!check(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "elapsed") ||
+        !check(c, hipEventElapsedTime(&b, c->ev[2], c->ev[3]), "elapsed"))
+        return LYG_ERR_DEVICE;
+    if (count_ms) *count_ms = a;
+    if (emit_ms) *emit_ms = b;
+    return LYG_OK;
+}
 
 // ---- plain per-path stroke -------------------------------------------------------------
 int32_t lyg_stroke_path(lyg_ctx* c, const lyg_path_view* path, const lyg_stroke_options* opts,

@@ -34,6 +34,31 @@ struct StrokeLaunch {
     uint32_t* err;           // emit pass: set when a write falls outside its path's range
 };
 
+struct FillLaunch {
+    PathBatchDev in;
+    const lyg_fill_options* opts;  // per path (opts_stride 1) or shared (stride 0)
+    uint32_t opts_stride;
+    uint8_t* work;                 // per-wave working memory (wave slot * work_stride)
+    uint64_t work_stride;
+    uint32_t ev_cap, heap_cap, act_cap;
+    float* stage_v;                // per-path staging, path i at stage_v_off[i] (vertices)
+    uint32_t* stage_t;             // and stage_t_off[i] (triangles)
+    const uint64_t* stage_v_off;
+    const uint64_t* stage_t_off;
+    uint32_t* path_nv;
+    uint32_t* path_ni;
+    uint32_t* path_status;
+    const uint32_t* path_list;     // paths to process (nullptr: first_path + 0..n_list)
+    uint32_t n_list;
+    uint32_t* work_counter;
+    // copy pass
+    const uint32_t* vtx_off;
+    const uint32_t* idx_off;
+    float* out_vertices;
+    uint32_t* out_indices;
+    uint32_t vertex_offset;
+};
+
 struct FlattenLaunch {
     const int32_t* kinds;  // 0 line, 1 quadratic, 2 cubic
     const float* ctrl;     // 8 floats per segment: from, ctrl1, ctrl2, to

@@ -35,7 +35,9 @@ def check(path, opts, what, vertex_offset=0):
     g = gpu_stroke(path, opts, vertex_offset)
     r = ref_stroke(path, opts, vertex_offset)
     print(f"{what}: {len(r[0])} vertices, {len(r[1]) // 3} triangles", flush=True)
-    return assert_same_geometry(g, r, bbox_diag(path.points), what=what)
+    # Scale of the 1e-5 position bound: the path's bbox diagonal, or the line width for a
+    # path of zero extent (its stroke is a cap of that size).
+    return assert_same_geometry(g, r, max(bbox_diag(path.points), opts.line_width), what=what)
 
 
 ALL_OPTS = [StrokeOptions(line_width=w, line_join=j, start_cap=c, end_cap=c2, tolerance=t,
@@ -154,4 +156,5 @@ def test_batch_matches_sequential_plain_calls():
         p = batch.path(i)
         r = ref_stroke(p, so, vertex_offset=int(vo[i]))
         g = res.path(i)
-        assert_same_geometry(g, r, bbox_diag(p.points), what=f"batch path {i}")
+        assert_same_geometry(g, r, max(bbox_diag(p.points), so.line_width),
+                             what=f"batch path {i}")
