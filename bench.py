@@ -1,19 +1,25 @@
-"""Benchmark: batched stroke tessellation on one MI355X.
+"""Benchmark: batched stroke and batched fill tessellation on one MI355X.
 
-Workload (batched stroke): 10,000 open paths with log-uniform segment counts in
+Headline workload (batched stroke): 10,000 open paths with log-uniform segment counts in
 [8, 256] (25% lines / 25% quadratics / 50% cubics), points uniform in [0, 2048]^2,
 tolerance 0.05, per-path width U[0.5, 8], join uniform over {Miter, Round, Bevel}, caps
-uniform over {Butt, Square, Round}; output f32 positions + u32 indices.  Data is
-synthetic (seeded generator), resident in HBM before the timed region.
+uniform over {Butt, Square, Round}; output f32 positions + u32 indices.  A step = one
+batched pass over the 10,000 paths (count kernel, offset scan, emit kernel); ``value`` =
+output triangles per second over the timed steps.  Inputs are synthetic (seeded
+generator) and resident in HBM before the timed region.
 
-A step = one batched pass over the 10,000 paths: count kernel, offset scan, emit kernel.
-``value`` = output triangles per second over the timed steps.  The same JSON line carries:
-  * plain: the plain per-path drop-in (host buffers in/out, one call per path) on a
-    sample of the paths, as triangles/s, and the batched/plain ratio;
-  * roofline: algorithmic bytes of the emit kernel (1 B/verb + 8 B/point + 8 B/vertex +
-    4 B/index) over its average duration (HIP events on the library's stream);
-  * cpu_baseline: the oracle (C++ restatement of lyon's StrokeTessellator) on the box's
-    host cores over a bounded sample of the same workload.
+The same JSON line carries:
+  * plain_per_path: the plain per-path drop-in (host buffers in/out, one call per path) on
+    a sample of the paths, and the batched/plain ratio;
+  * fill: the batched fill workload (4,096 closed paths, half self-intersecting star
+    polygons with 5-512 points, half loops of 4-256 cubics, EvenOdd/NonZero 50/50,
+    tolerance 0.05, coordinates in [0, 4096]^2) timed the same way, with its plain
+    per-path sample;
+  * roofline: algorithmic bytes of the stroke emit kernel (1 B/verb + 8 B/point +
+    8 B/vertex + 4 B/index) over its average duration (HIP events on the library's
+    stream); traffic from rocprofv3 counters when given (--traffic-json);
+  * cpu_baseline: the oracle (C++ restatement of lyon's tessellators) on the box's host
+    cores over a bounded sample of the same workloads, single thread and all cores.
 """
 import argparse
 import ctypes as C
@@ -32,115 +38,185 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def kernel_times(L, ctx):
+    a = C.c_float(0)
+    b = C.c_float(0)
+    L.lyg_ctx_kernel_times(ctx, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def plain_rate(tess, paths, opts, make_buffers):
+    """Triangles/s of one tessellate_path call per path (host buffers in and out)."""
+    for i in range(min(20, len(paths))):
+        tess.tessellate_path(paths[i], opts[i], make_buffers()[1])
+    tris = 0
+    t0 = time.perf_counter()
+    for i in range(len(paths)):
+        vb, bb = make_buffers()
+        tess.tessellate_path(paths[i], opts[i], bb)
+        tris += len(vb.indices) // 3
+    return tris / (time.perf_counter() - t0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--paths", type=int, default=10_000)
+    ap.add_argument("--fill-paths", type=int, default=4096)
+    ap.add_argument("--fill-steps", type=int, default=5)
     ap.add_argument("--plain-sample", type=int, default=300)
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fill", action="store_true")
+    ap.add_argument("--traffic-json", default=None,
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the emit kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding of this path is described in DESIGN.md but not run.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharding of this "
+              "path across GPUs is described in DESIGN.md and not run.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
-    import torch
 
-    from lyon_amd import (BuffersBuilder, StrokeOptions, StrokeTessellator, VertexBuffers,
-                          workloads)
+    from lyon_amd import (BuffersBuilder, FillOptions, FillRule, FillTessellator, StrokeOptions,
+                          StrokeTessellator, VertexBuffers, workloads)
     from lyon_amd import _lib as B
-    from lyon_amd.batch import DeviceBatch, StrokeBatcher
+    from lyon_amd.batch import DeviceBatch, FillBatcher, StrokeBatcher, synchronize
     from lyon_amd.tessellation import _Ctx
 
-    log("generating workload")
-    batch = workloads.stroke_batch(args.paths, 8, 256, 2048.0, seed=0)
-    opts = workloads.stroke_options(args.paths, tolerance=0.05, w_min=0.5, w_max=8.0)
-    db = DeviceBatch(batch, opts)
-    batcher = StrokeBatcher()
     L = B.lib()
     ctx = _Ctx.get()
     L.lyg_ctx_enable_timing(ctx, 1)
 
-    log("warmup")
+    # ---- batched stroke (headline) ----------------------------------------------------
+    log("generating stroke workload")
+    batch = workloads.stroke_batch(args.paths, 8, 256, 2048.0, seed=0)
+    opts = workloads.stroke_options(args.paths, tolerance=0.05, w_min=0.5, w_max=8.0)
+    db = DeviceBatch(batch, opts)
+    batcher = StrokeBatcher()
+    log("stroke warmup")
     res = None
     for _ in range(max(args.warmup, 1)):
         res = batcher.run(db)
-    torch.cuda.synchronize()
+    synchronize()
     tris_per_step = res.n_indices // 3
-    n_vertices = res.n_vertices
-    n_indices = res.n_indices
-
-    log(f"timing {args.steps} steps")
-    emit_ms = []
-    count_ms = []
-    torch.cuda.synchronize()
+    n_vertices, n_indices = res.n_vertices, res.n_indices
+    log(f"stroke: timing {args.steps} steps")
+    emit_ms, count_ms = [], []
+    synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batcher.run(db)
-        a = C.c_float(0)
-        b = C.c_float(0)
-        L.lyg_ctx_kernel_times(ctx, C.byref(a), C.byref(b))
-        count_ms.append(a.value)
-        emit_ms.append(b.value)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+        a, b = kernel_times(L, ctx)
+        count_ms.append(a)
+        emit_ms.append(b)
+    synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     value = tris_per_step / (ms_per_step / 1e3)
-
-    # roofline of the emit kernel: algorithmic bytes per launch / average duration
-    alg_bytes = (len(batch.verbs) * 1 + len(batch.points) * 8 + n_vertices * 8 + n_indices * 4)
+    alg_bytes = len(batch.verbs) + 8 * len(batch.points) + 8 * n_vertices + 4 * n_indices
     emit_avg = float(np.mean(emit_ms))
     achieved = alg_bytes / (emit_avg / 1e3) / 1e9
 
-    # plain per-path drop-in on a sample (host in/out, one call per path)
-    log("plain per-path drop-in")
-    tess = StrokeTessellator()
+    log("stroke: plain per-path drop-in")
     ns = min(args.plain_sample, args.paths)
     sos = [StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
                          tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
                          end_cap=int(o["end_cap"]), line_join=int(o["line_join"]))
            for o in opts[:ns]]
-    paths = [batch.path(i) for i in range(ns)]
-    for i in range(min(20, ns)):
-        tess.tessellate_path(paths[i], sos[i], BuffersBuilder(VertexBuffers()))
-    plain_tris = 0
-    tp0 = time.perf_counter()
-    for i in range(ns):
-        vb = VertexBuffers()
-        tess.tessellate_path(paths[i], sos[i], BuffersBuilder(vb))
-        plain_tris += len(vb.indices) // 3
-    tp1 = time.perf_counter()
-    plain_value = plain_tris / (tp1 - tp0)
 
+    def buffers():
+        vb = VertexBuffers()
+        return vb, BuffersBuilder(vb)
+
+    plain_value = plain_rate(StrokeTessellator(), [batch.path(i) for i in range(ns)], sos,
+                             buffers)
+
+    # ---- batched fill -------------------------------------------------------------------
+    fill = None
+    fbatch = fopts = None
+    if not args.no_fill:
+        log("generating fill workload")
+        fbatch = workloads.fill_batch(args.fill_paths, seed=0)
+        fopts = workloads.fill_options(args.fill_paths, tolerance=0.05)
+        fdb = DeviceBatch(fbatch, fopts)
+        fb = FillBatcher()
+        log("fill warmup")
+        fres = fb.run(fdb, check=False)
+        synchronize()
+        ftris = fres.n_indices // 3
+        fstatus = fres.status
+        log(f"fill: timing {args.fill_steps} steps")
+        sweep_ms, copy_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.fill_steps):
+            fb.run(fdb, check=False)
+            a, b = kernel_times(L, ctx)
+            copy_ms.append(a)
+            sweep_ms.append(b)
+        synchronize()
+        fms = (time.perf_counter() - t0) * 1e3 / args.fill_steps
+        fns = min(args.plain_sample, args.fill_paths)
+        fos = [FillOptions(tolerance=0.05, fill_rule=FillRule(int(o["fill_rule"])))
+               for o in fopts[:fns]]
+        log("fill: plain per-path drop-in")
+        fplain = plain_rate(FillTessellator(), [fbatch.path(i) for i in range(fns)], fos,
+                            buffers)
+        fill = {"workload": f"batched fill: {args.fill_paths} closed paths, half "
+                            "self-intersecting star polygons {n/k} (5-512 points), half loops "
+                            "of 4-256 cubics, EvenOdd/NonZero 50/50, tol 0.05, coords in "
+                            "[0,4096]^2",
+                "value": ftris / (fms / 1e3), "unit": "triangles/s", "ms_per_step": fms,
+                "paths_per_s": args.fill_paths / (fms / 1e3),
+                "triangles_per_step": ftris, "vertices_per_step": fres.n_vertices,
+                "failed_paths": int(np.count_nonzero(fstatus)),
+                "kernel_ms": {"sweep_round0_avg": float(np.mean(sweep_ms)),
+                              "copy_avg": float(np.mean(copy_ms))},
+                "plain_per_path": {"value": fplain, "unit": "triangles/s",
+                                   "sample_paths": fns},
+                "batched_over_plain": (ftris / (fms / 1e3)) / fplain}
+
+    # ---- CPU baseline ---------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         from oracle import pyoracle as O
         from lyon_amd.path import PathBatch
+        ncores = len(os.sched_getaffinity(0))
         m = min(args.cpu_sample, args.paths)
         sub = PathBatch.from_paths([batch.path(i) for i in range(m)])
         so = np.zeros((m, 6), np.float32)
-        so[:, 0] = opts["line_width"][:m]
-        so[:, 1] = opts["miter_limit"][:m]
-        so[:, 2] = opts["tolerance"][:m]
-        so[:, 3] = opts["start_cap"][:m]
-        so[:, 4] = opts["end_cap"][:m]
-        so[:, 5] = opts["line_join"][:m]
+        for j, k in enumerate(("line_width", "miter_limit", "tolerance", "start_cap", "end_cap",
+                               "line_join")):
+            so[:, j] = opts[k][:m]
         O.batch_triangles(sub, so, False, 1)
         c0 = time.perf_counter()
         ctris = O.batch_triangles(sub, so, False, 1)
         c1 = time.perf_counter()
-        ncores = len(os.sched_getaffinity(0))
-        a0 = time.perf_counter()
         atris = O.batch_triangles(sub, so, False, ncores)
         a1 = time.perf_counter()
         cpu = {"value": ctris / (c1 - c0), "unit": "triangles/s", "cores": 1, "kind": "port",
                "sample": f"first {m} of the {args.paths} batched-stroke paths, single thread",
-               "all_cores_value": atris / (a1 - a0), "all_cores": ncores}
+               "all_cores_value": atris / (a1 - c1), "all_cores": ncores}
+        if fbatch is not None:
+            fm = min(args.cpu_sample, args.fill_paths)
+            fsub = PathBatch.from_paths([fbatch.path(i) for i in range(fm)])
+            fo = np.zeros((fm, 2), np.float32)  # tolerance, fill rule per path
+            fo[:, 0] = 0.05
+            fo[:, 1] = fopts["fill_rule"][:fm]
+            f0 = time.perf_counter()
+            fct = O.batch_triangles(fsub, fo, True, 1)
+            f1 = time.perf_counter()
+            fat = O.batch_triangles(fsub, fo, True, ncores)
+            f2 = time.perf_counter()
+            cpu["fill"] = {"value": fct / (f1 - f0), "unit": "triangles/s", "cores": 1,
+                           "sample": f"first {fm} of the {args.fill_paths} batched-fill paths",
+                           "all_cores_value": fat / (f2 - f1), "all_cores": ncores}
+
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        traffic = json.load(open(args.traffic_json)).get("bytes_per_launch")
 
     out = {
         "metric": "output triangles/s batched fill+stroke vs plain per-path; p50/p99 req latency",
@@ -160,13 +236,15 @@ def main():
                                "tol 0.05, width U[0.5,8], join {Miter,Round,Bevel}, caps "
                                "{Butt,Square,Round}",
                    "paths": args.paths, "triangles_per_step": tris_per_step,
-                   "vertices_per_step": n_vertices, "parallelism": "single GPU"},
+                   "vertices_per_step": n_vertices, "paths_per_s": args.paths / (ms_per_step / 1e3),
+                   "parallelism": "single GPU"},
         "plain_per_path": {"value": plain_value, "unit": "triangles/s",
                            "sample_paths": ns, "batched_over_plain": value / plain_value},
         "kernel_ms": {"count_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "stroke_kernel<emit>", "algorithmic_bytes": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
+        "fill": fill,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -114,10 +114,22 @@ const char* lyg_status_string(int32_t status);
 /* Last HIP error text recorded by the context (empty if none). */
 const char* lyg_ctx_last_error(const lyg_ctx* ctx);
 /* Measurement support (no lyon counterpart): when enabled, HIP events are recorded on the
- * context's stream around the count and emit kernels of every stroke call;
- * lyg_ctx_kernel_times returns the durations of the last call in milliseconds. */
+ * context's stream around two phases of every call; lyg_ctx_kernel_times returns their
+ * durations for the last call in milliseconds: for a stroke, the count pass (first) and
+ * the emit pass (second); for a fill, the compaction copy (first) and the first sweep
+ * round (second). */
 int32_t lyg_ctx_enable_timing(lyg_ctx* ctx, int enable);
 int32_t lyg_ctx_kernel_times(lyg_ctx* ctx, float* count_ms, float* emit_ms);
+
+/* Device memory for the batched form (no lyon counterpart: lyon's buffers are host Vecs).
+ * Allocations belong to the context's device; copies run on the context's stream and are
+ * complete when the call returns. */
+int32_t lyg_device_alloc(lyg_ctx* ctx, uint64_t bytes, void** ptr);
+int32_t lyg_device_free(lyg_ctx* ctx, void* ptr);
+int32_t lyg_copy_to_device(lyg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int32_t lyg_copy_to_host(lyg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Waits for all work queued on the context's stream. */
+int32_t lyg_ctx_synchronize(lyg_ctx* ctx);
 
 /* ---- plain per-path drop-in (host buffers in and out) --------------------------------
  * StrokeTessellator::tessellate_path (crates/tessellation/src/stroke.rs:164-177). */
@@ -170,6 +182,7 @@ typedef struct lyg_batch_out {
     uint32_t vertex_offset;
     uint64_t n_vertices;           /* out (host field) */
     uint64_t n_indices;            /* out (host field) */
+    int32_t* path_status;          /* n, device, optional: each path's lyg_status */
 } lyg_batch_out;
 
 /* opts: n_opts entries in device memory; n_opts == 1 shares one StrokeOptions. */
@@ -197,6 +210,17 @@ int32_t lyg_gen_stroke_batch(uint64_t seed, uint32_t n_paths, uint32_t min_segs,
  * miter limit 4.  `out` has n_paths entries (host memory). */
 int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance, float w_min,
                                float w_max, lyg_stroke_options* out);
+
+/* The batched fill workload: closed paths in [0, 4096]^2, even paths self-intersecting
+ * star polygons {n/k} (n log-uniform in [5, 512], radii R*U[0.2,1], R in [16, 512]), odd
+ * paths loops of m cubics (m log-uniform in [4, 256]).  Same two-call protocol. */
+int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, float* points,
+                           uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
+                           uint64_t* n_points);
+/* Per-path FillOptions of the batched fill workload: EvenOdd/NonZero 50/50, vertical
+ * sweep, intersections handled. */
+int32_t lyg_gen_fill_options(uint64_t seed, uint32_t n_paths, float tolerance,
+                             lyg_fill_options* out);
 
 #ifdef __cplusplus
 }

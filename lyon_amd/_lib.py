@@ -51,7 +51,7 @@ class BatchOutC(C.Structure):
                 ("vertex_capacity", C.c_uint64), ("index_capacity", C.c_uint64),
                 ("path_vertex_offsets", C.c_void_p), ("path_index_offsets", C.c_void_p),
                 ("vertex_offset", C.c_uint32), ("n_vertices", C.c_uint64),
-                ("n_indices", C.c_uint64)]
+                ("n_indices", C.c_uint64), ("path_status", C.c_void_p)]
 
 
 # Every symbol include/lyg.h declares, with its ctypes signature.
@@ -62,6 +62,11 @@ SIGNATURES = {
     "lyg_ctx_last_error": ([C.c_void_p], C.c_char_p),
     "lyg_ctx_enable_timing": ([C.c_void_p, C.c_int], C.c_int32),
     "lyg_ctx_kernel_times": ([C.c_void_p, F32P, F32P], C.c_int32),
+    "lyg_device_alloc": ([C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)], C.c_int32),
+    "lyg_device_free": ([C.c_void_p, C.c_void_p], C.c_int32),
+    "lyg_copy_to_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
+    "lyg_copy_to_host": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
+    "lyg_ctx_synchronize": ([C.c_void_p], C.c_int32),
     "lyg_stroke_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
                          C.POINTER(OutputC)], C.c_int32),
     "lyg_fill_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),
@@ -80,6 +85,10 @@ SIGNATURES = {
                                 C.c_void_p, C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_gen_stroke_batch": ([C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
                               U8P, F32P, U32P, U32P, U64P, U64P], C.c_int32),
+    "lyg_gen_fill_batch": ([C.c_uint64, C.c_uint32, U8P, F32P, U32P, U32P, U64P, U64P],
+                           C.c_int32),
+    "lyg_gen_fill_options": ([C.c_uint64, C.c_uint32, C.c_float, C.POINTER(FillOptionsC)],
+                             C.c_int32),
     "lyg_gen_stroke_options": ([C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float,
                                 C.POINTER(StrokeOptionsC)], C.c_int32),
 }

@@ -1,9 +1,11 @@
 """Batched form: many paths per launch, device-resident inputs and outputs.
 
-Device memory comes from PyTorch (plumbing only: the kernels are liblyg.so's).  The
-result of ``stroke_batch`` equals calling ``StrokeTessellator.tessellate_path`` for every
-path in order into one shared ``VertexBuffers`` (geometry_builder.rs:300-351): path i's
-vertices start at ``vertex_offsets[i]`` and its indices are absolute ids.
+Device memory is allocated through liblyg.so's own entry points (``lyg_device_alloc`` and
+friends), so the batched form needs nothing but the library.  The result of a batch equals
+calling ``StrokeTessellator.tessellate_path`` / ``FillTessellator.tessellate_path`` for
+every path in order into one shared ``VertexBuffers`` (geometry_builder.rs:300-351): path
+i's vertices start at ``vertex_offsets[i]`` and its indices are absolute ids.  A fill path
+that fails contributes nothing and its status is reported per path (fill.rs:788-793).
 """
 from __future__ import annotations
 
@@ -15,105 +17,186 @@ from . import _lib as B
 from .path import PathBatch
 from .tessellation import TessellationError, _Ctx
 
+LYG_ERR_CAPACITY = 16
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("the batched form needs a HIP device")
-    return torch
+
+class DeviceBuffer:
+    """A device allocation owned by the library's context (freed with the object)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(max(nbytes, 16))
+        p = C.c_void_p(0)
+        st = B.lib().lyg_device_alloc(_Ctx.get(), self.nbytes, C.byref(p))
+        if st != 0:
+            raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(a)
+        if offset + a.nbytes > self.nbytes:
+            raise ValueError("upload beyond the buffer")
+        st = B.lib().lyg_copy_to_device(_Ctx.get(), self.ptr + offset, a.ctypes.data, a.nbytes)
+        if st != 0:
+            raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+
+    def download(self, dtype, count: int, offset: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("download beyond the buffer")
+        st = B.lib().lyg_copy_to_host(_Ctx.get(), out.ctypes.data, self.ptr + offset, out.nbytes)
+        if st != 0:
+            raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                B.lib().lyg_device_free(_Ctx.get(), self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def synchronize():
+    st = B.lib().lyg_ctx_synchronize(_Ctx.get())
+    if st != 0:
+        raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
 
 
 class DeviceBatch:
-    """A PathBatch copied to device memory once (inputs stay resident in HBM)."""
+    """A PathBatch copied to device memory once (inputs stay resident in HBM), with
+    per-path options: a structured array of lyg_stroke_options or lyg_fill_options
+    records (or one record shared by every path)."""
 
-    def __init__(self, batch: PathBatch, stroke_opts=None, device="cuda:0"):
-        torch = _torch()
+    def __init__(self, batch: PathBatch, opts=None):
         self.n = len(batch)
-        self.verbs = torch.from_numpy(batch.verbs).to(device)
-        self.points = torch.from_numpy(batch.points.reshape(-1)).to(device)
-        self.verb_offsets = torch.from_numpy(batch.verb_offsets.view(np.int32)).to(device)
-        self.point_offsets = torch.from_numpy(batch.point_offsets.view(np.int32)).to(device)
+        self.batch = batch
+        self.verbs = DeviceBuffer.from_numpy(batch.verbs)
+        self.points = DeviceBuffer.from_numpy(batch.points.reshape(-1))
+        self.verb_offsets = DeviceBuffer.from_numpy(batch.verb_offsets)
+        self.point_offsets = DeviceBuffer.from_numpy(batch.point_offsets)
         self.opts = None
         self.n_opts = 0
-        if stroke_opts is not None:
-            raw = np.ascontiguousarray(stroke_opts).view(np.uint8).reshape(-1)
-            self.opts = torch.from_numpy(raw.copy()).to(device)
-            self.n_opts = len(stroke_opts)
+        if opts is not None:
+            raw = np.ascontiguousarray(opts)
+            self.opts = DeviceBuffer.from_numpy(raw.view(np.uint8).reshape(-1))
+            self.n_opts = len(raw)
 
     def c_in(self) -> B.BatchInC:
-        return B.BatchInC(self.verbs.data_ptr(), self.points.data_ptr(),
-                          self.verb_offsets.data_ptr(), self.point_offsets.data_ptr(), self.n)
+        return B.BatchInC(self.verbs.ptr, self.points.ptr, self.verb_offsets.ptr,
+                          self.point_offsets.ptr, self.n)
 
 
-class StrokeBatchResult:
-    def __init__(self, vertices, indices, vertex_offsets, index_offsets, n_vertices, n_indices):
-        self.vertices = vertices            # device tensor (capacity, 2) float32
-        self.indices = indices              # device tensor (capacity,) int32 view of u32
-        self.vertex_offsets = vertex_offsets  # device (n+1,) int32 view of u32
-        self.index_offsets = index_offsets
+class BatchResult:
+    """Device-resident output of one batch call."""
+
+    def __init__(self, vertices, indices, voffs, ioffs, status, n_paths, n_vertices, n_indices):
+        self.vertices = vertices      # DeviceBuffer, (capacity, 2) f32
+        self.indices = indices        # DeviceBuffer, (capacity,) u32
+        self.voffs = voffs            # DeviceBuffer, (n+1,) u32
+        self.ioffs = ioffs
+        self.status_buf = status      # DeviceBuffer, (n,) i32
+        self.n = n_paths
         self.n_vertices = n_vertices
         self.n_indices = n_indices
 
+    @property
+    def vertex_offsets(self) -> np.ndarray:
+        return self.voffs.download(np.uint32, self.n + 1)
+
+    @property
+    def index_offsets(self) -> np.ndarray:
+        return self.ioffs.download(np.uint32, self.n + 1)
+
+    @property
+    def status(self) -> np.ndarray:
+        return self.status_buf.download(np.int32, self.n)
+
+    def all_vertices(self) -> np.ndarray:
+        return self.vertices.download(np.float32, 2 * self.n_vertices).reshape(-1, 2)
+
+    def all_indices(self) -> np.ndarray:
+        return self.indices.download(np.uint32, self.n_indices)
+
     def path(self, i: int):
         """(vertices, indices) of path i as numpy arrays (indices absolute, u32)."""
-        vo = self.vertex_offsets.cpu().numpy().view(np.uint32)
-        io = self.index_offsets.cpu().numpy().view(np.uint32)
-        v = self.vertices[int(vo[i]):int(vo[i + 1])].cpu().numpy()
-        idx = self.indices[int(io[i]):int(io[i + 1])].cpu().numpy().view(np.uint32)
-        return v, idx
+        vo = self.vertex_offsets
+        io = self.index_offsets
+        v = self.vertices.download(np.float32, 2 * int(vo[i + 1] - vo[i]), 8 * int(vo[i]))
+        idx = self.indices.download(np.uint32, int(io[i + 1] - io[i]), 4 * int(io[i]))
+        return v.reshape(-1, 2), idx
 
 
-class StrokeBatcher:
-    """Reusable output arena for repeated batched strokes (no allocation per call once
-    the arena is large enough)."""
+class _Batcher:
+    """Reusable output arena for repeated batch calls (no allocation per call once the
+    arena is large enough)."""
 
-    def __init__(self, device="cuda:0"):
-        self.device = device
+    _entry = None
+
+    def __init__(self):
         self.vertices = None
         self.indices = None
         self.voffs = None
         self.ioffs = None
+        self.status = None
+        self.n_cap = 0
 
     def _ensure(self, n_paths, nv, ni):
-        torch = _torch()
-        if self.voffs is None or self.voffs.numel() < n_paths + 1:
-            self.voffs = torch.empty(n_paths + 1, dtype=torch.int32, device=self.device)
-            self.ioffs = torch.empty(n_paths + 1, dtype=torch.int32, device=self.device)
-        if self.vertices is None or self.vertices.shape[0] < nv:
-            self.vertices = torch.empty((max(nv, 1), 2), dtype=torch.float32, device=self.device)
-        if self.indices is None or self.indices.shape[0] < ni:
-            self.indices = torch.empty(max(ni, 1), dtype=torch.int32, device=self.device)
+        if self.voffs is None or self.n_cap < n_paths:
+            self.voffs = DeviceBuffer(4 * (n_paths + 1))
+            self.ioffs = DeviceBuffer(4 * (n_paths + 1))
+            self.status = DeviceBuffer(4 * max(n_paths, 1))
+            self.n_cap = n_paths
+        if self.vertices is None or self.vertices.nbytes < 8 * nv:
+            self.vertices = DeviceBuffer(8 * max(nv, 1))
+        if self.indices is None or self.indices.nbytes < 4 * ni:
+            self.indices = DeviceBuffer(4 * max(ni, 1))
+
+    def run(self, db: DeviceBatch, vertex_offset: int = 0, check=True) -> BatchResult:
+        if db.opts is None:
+            raise ValueError("DeviceBatch needs options")
+        self._ensure(db.n, 0, 0)
+        cin = db.c_in()
+        fn = getattr(B.lib(), self._entry)
+        for _ in range(3):
+            out = B.BatchOutC(self.vertices.ptr, self.indices.ptr, self.vertices.nbytes // 8,
+                              self.indices.nbytes // 4, self.voffs.ptr, self.ioffs.ptr,
+                              vertex_offset, 0, 0, self.status.ptr)
+            st = fn(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts, C.byref(out))
+            if st == LYG_ERR_CAPACITY:
+                self._ensure(db.n, out.n_vertices, out.n_indices)
+                continue
+            if st != 0 and (check or st in (17, 18, 19, 20)):
+                raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+            return BatchResult(self.vertices, self.indices, self.voffs, self.ioffs, self.status,
+                               db.n, out.n_vertices, out.n_indices)
+        raise RuntimeError("batch did not converge")
+
+
+class StrokeBatcher(_Batcher):
+    _entry = "lyg_stroke_batch"
 
     def count(self, db: DeviceBatch):
-        torch = _torch()
-        nv = torch.empty(db.n, dtype=torch.int32, device=self.device)
-        ni = torch.empty(db.n, dtype=torch.int32, device=self.device)
+        nv = DeviceBuffer(4 * db.n)
+        ni = DeviceBuffer(4 * db.n)
         tv = C.c_uint64(0)
         ti = C.c_uint64(0)
         cin = db.c_in()
-        st = B.lib().lyg_stroke_batch_count(_Ctx.get(), C.byref(cin), db.opts.data_ptr(),
-                                            db.n_opts, nv.data_ptr(), ni.data_ptr(),
-                                            C.byref(tv), C.byref(ti))
+        st = B.lib().lyg_stroke_batch_count(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts,
+                                            nv.ptr, ni.ptr, C.byref(tv), C.byref(ti))
         if st != 0:
             raise TessellationError(st)
         return tv.value, ti.value
 
-    def run(self, db: DeviceBatch, vertex_offset: int = 0) -> StrokeBatchResult:
-        if db.opts is None:
-            raise ValueError("DeviceBatch needs stroke options")
-        self._ensure(db.n, 0, 0)
-        cin = db.c_in()
-        for _ in range(2):
-            out = B.BatchOutC(self.vertices.data_ptr(), self.indices.data_ptr(),
-                              self.vertices.shape[0], self.indices.shape[0],
-                              self.voffs.data_ptr(), self.ioffs.data_ptr(), vertex_offset, 0, 0)
-            st = B.lib().lyg_stroke_batch(_Ctx.get(), C.byref(cin), db.opts.data_ptr(),
-                                          db.n_opts, C.byref(out))
-            if st == 16:
-                self._ensure(db.n, out.n_vertices, out.n_indices)
-                continue
-            if st != 0:
-                raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
-            return StrokeBatchResult(self.vertices, self.indices, self.voffs[:db.n + 1],
-                                     self.ioffs[:db.n + 1], out.n_vertices, out.n_indices)
-        raise RuntimeError("stroke batch did not converge")
+
+class FillBatcher(_Batcher):
+    """Batched fill; with check=False a path's failure is reported in ``status`` only."""
+
+    _entry = "lyg_fill_batch"

@@ -1,0 +1,87 @@
+// Internal to liblyg.so: the context object and the helpers every host file shares.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/lyg.h"
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct lyg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 256;
+    std::string last_error;
+    // grow-only device scratch
+    DevBuf in_verbs, in_points, in_offs, in_opts, counts_v, counts_i, offs_v, offs_i, out_v,
+        out_i, misc, arcs, flat_kinds, flat_ctrl;
+    uint32_t* d_counters = nullptr;  // [0..3] work-queue heads, [4] overflow flag
+    uint64_t* d_totals = nullptr;    // [0] vertices, [1] indices
+    void* h_pinned = nullptr;        // small pinned staging area for totals
+    // optional kernel timing: events around the count and emit kernels on `stream`
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // fill: per-path arrays, per-wave working memory and per-round staging
+    DevBuf f_bound, f_status, f_round, f_work;
+    std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
+};
+
+inline void free_buf(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// LYG_TRACE=1 prints every host step to stderr (flushed), for diagnosing a run that
+// goes quiet.
+inline bool trace_on() {
+    static int on = -1;
+    if (on < 0) {
+        const char* v = getenv("LYG_TRACE");
+        on = (v && *v && *v != '0') ? 1 : 0;
+    }
+    return on == 1;
+}
+
+inline bool check(lyg_ctx* c, hipError_t e, const char* what) {
+    if (trace_on()) {
+        fprintf(stderr, "[lyg] %s -> %d\n", what, (int)e);
+        fflush(stderr);
+    }
+    if (e == hipSuccess) return true;
+    c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+inline bool ensure(lyg_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return true;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t cap = std::max(bytes, (size_t)4096);
+    cap = (cap + 4095) & ~(size_t)4095;
+    if (!check(c, hipMalloc(&b.p, cap), "hipMalloc")) return false;
+    b.cap = cap;
+    return true;
+}
+
+template <class T>
+T* as(DevBuf& b) {
+    return reinterpret_cast<T*>(b.p);
+}
+
+inline int persistent_grid(lyg_ctx* c, uint32_t n_paths) {
+    // 4 waves per block, one path per wave at a time; 8 blocks per CU keep 32 waves/CU.
+    uint32_t blocks_needed = (n_paths + 3) / 4;
+    uint32_t cap = (uint32_t)c->cu_count * 8;
+    return (int)std::max(1u, std::min(blocks_needed, cap));
+}

@@ -1,0 +1,395 @@
+// Host side of the fill: sizing, launch rounds and compaction.
+//
+// A fill call runs in three steps on the context's stream:
+//   1. fill_bound_kernel writes an upper bound of every path's sweep events;
+//   2. rounds of fill_sweep_kernel: each path is swept by one wave into a staging area
+//      sized from its bound.  Paths are grouped into size classes so that the per-wave
+//      working memory of a launch fits its largest path; a path that outgrows its
+//      staging or working memory ends with a retry status and is swept again in the next
+//      round with four times the room;
+//   3. an exclusive scan over the per-path vertex/index counts, then fill_copy_kernel
+//      moves every round's staged output to its final place, rebasing the indices.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ctx.h"
+#include "lyg_types.h"
+
+extern "C" {
+size_t lyg_fill_work_bytes(uint32_t ev_cap, uint32_t heap_cap, uint32_t act_cap);
+hipError_t lyg_launch_fill_bound(const lyg::FillLaunch* L, uint32_t n, uint32_t* bound,
+                                 hipStream_t s);
+hipError_t lyg_launch_fill_sweep(const lyg::FillLaunch* L, int grid, hipStream_t s);
+hipError_t lyg_launch_fill_copy(const lyg::FillLaunch* L, hipStream_t s);
+hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
+                           uint32_t* overflow, hipStream_t s);
+hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+}
+
+namespace {
+
+constexpr uint32_t ST_RETRY = 100;
+constexpr int MAX_ROUNDS = 5;
+constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO_SPACE
+constexpr size_t WORK_BUDGET = 4ull << 30;           // per-launch working memory
+constexpr size_t STAGE_BUDGET = 64ull << 30;         // staging memory of one round
+constexpr int MAX_CLASSES = 8;
+constexpr int FILL_WAVES_PER_BLOCK = 4;
+
+struct Caps {
+    uint32_t ev, heap, act;
+};
+
+// Working-memory capacities of one wave for paths of at most `bound` events at `scale`.
+Caps class_caps(uint64_t bound, uint64_t scale) {
+    Caps k;
+    k.ev = (uint32_t)bound;
+    k.heap = (uint32_t)std::min<uint64_t>(scale * (2 * bound + 256), 0x7fffffffu);
+    k.act = (uint32_t)std::min<uint64_t>(scale * (bound + 64), 0x7fffffffu);
+    return k;
+}
+
+// Staging capacity (vertices, triangles) of one path.
+void stage_caps(uint64_t bound, uint64_t scale, uint64_t* v, uint64_t* t) {
+    *v = std::min<uint64_t>(scale * (2 * bound + 256), 0x7fffffffu);
+    *t = std::min<uint64_t>(2 * *v, 0x7fffffffu);
+}
+
+int size_class(uint64_t bound) {
+    int k = 0;
+    uint64_t lim = 1u << 12;
+    while (bound > lim && k < MAX_CLASSES - 1) {
+        lim <<= 3;
+        ++k;
+    }
+    return k;
+}
+
+bool ensure_at(lyg_ctx* c, std::vector<DevBuf>& v, size_t i, size_t bytes) {
+    if (v.size() <= i) v.resize(i + 1);
+    return ensure(c, v[i], bytes);
+}
+
+struct FillJob {
+    lyg::PathBatchDev in;
+    const lyg_fill_options* opts;
+    uint32_t opts_stride;
+    uint32_t vertex_offset;
+};
+
+struct FillResult {
+    std::vector<uint32_t> status;  // per path, final
+    uint64_t total_v = 0, total_i = 0;
+    int rounds = 0;
+    std::vector<uint32_t> round_len;  // paths swept in each round
+};
+
+// Steps 1-2 and the scan of step 3; per-path offsets go to vtx_off/idx_off (n+1 each).
+int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* idx_off,
+                   FillResult* R) {
+    const uint32_t n = J.in.n_paths;
+    hipStream_t s = c->stream;
+    if (!ensure(c, c->f_bound, 4ull * n) || !ensure(c, c->f_status, 4ull * n) ||
+        !ensure(c, c->f_round, 4ull * n) || !ensure(c, c->counts_v, 4ull * n) ||
+        !ensure(c, c->counts_i, 4ull * n))
+        return LYG_ERR_DEVICE;
+    lyg::FillLaunch L{};
+    L.in = J.in;
+    L.opts = J.opts;
+    L.opts_stride = J.opts_stride;
+    L.path_nv = as<uint32_t>(c->counts_v);
+    L.path_ni = as<uint32_t>(c->counts_i);
+    L.path_status = as<uint32_t>(c->f_status);
+    L.path_round = as<uint32_t>(c->f_round);
+    L.vertex_offset = J.vertex_offset;
+    std::vector<uint32_t> bound(n);
+    if (!check(c, lyg_launch_fill_bound(&L, n, as<uint32_t>(c->f_bound), s), "fill bound") ||
+        !check(c, hipMemcpyAsync(bound.data(), c->f_bound.p, 4ull * n, hipMemcpyDeviceToHost, s),
+               "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    R->status.assign(n, 0);
+    // paths still to sweep, with their scale (4^round)
+    std::vector<uint32_t> todo;
+    todo.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (bound[i] > MAX_PATH_EVENTS) R->status[i] = LYG_ERR_NO_SPACE;
+        else todo.push_back(i);
+    }
+    // paths rejected up front produce nothing
+    if (todo.size() != n) {
+        if (!check(c, lyg_launch_fill_u32(L.path_nv, n, 0, s), "zero") ||
+            !check(c, lyg_launch_fill_u32(L.path_ni, n, 0, s), "zero") ||
+            !check(c, lyg_launch_fill_u32(L.path_round, n, 0xffffffffu, s), "zero"))
+            return LYG_ERR_DEVICE;
+    }
+    uint64_t scale = 1;
+    int round = 0;
+    for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale *= 4) {
+        // order by size class, then by path
+        std::vector<int> cls(todo.size());
+        for (size_t k = 0; k < todo.size(); ++k) cls[k] = size_class(bound[todo[k]]);
+        std::vector<size_t> order(todo.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] < cls[b]; });
+        std::vector<uint32_t> list(todo.size());
+        std::vector<uint64_t> offs(2 * (todo.size() + 1));
+        uint64_t* sv = offs.data();
+        uint64_t* st = offs.data() + todo.size() + 1;
+        sv[0] = st[0] = 0;
+        for (size_t k = 0; k < order.size(); ++k) {
+            uint32_t p = todo[order[k]];
+            list[k] = p;
+            uint64_t v, t;
+            stage_caps(bound[p], scale, &v, &t);
+            sv[k + 1] = sv[k] + v;
+            st[k + 1] = st[k] + t;
+        }
+        const size_t m = list.size();
+        if (8 * sv[m] + 12 * st[m] > STAGE_BUDGET) {
+            c->last_error = "fill staging exceeds the per-round budget";
+            return LYG_ERR_NO_SPACE;
+        }
+        if (!ensure_at(c, c->f_list, round, 4 * m) ||
+            !ensure_at(c, c->f_stage_off, round, 8 * offs.size()) ||
+            !ensure_at(c, c->f_stage_v, round, 8 * sv[m]) ||
+            !ensure_at(c, c->f_stage_t, round, 12 * st[m]))
+            return LYG_ERR_DEVICE;
+        uint32_t* d_list = as<uint32_t>(c->f_list[round]);
+        uint64_t* d_off = as<uint64_t>(c->f_stage_off[round]);
+        if (!check(c, hipMemcpyAsync(d_list, list.data(), 4 * m, hipMemcpyHostToDevice, s), "H2D") ||
+            !check(c, hipMemcpyAsync(d_off, offs.data(), 8 * offs.size(), hipMemcpyHostToDevice, s),
+                   "H2D") ||
+            !check(c, lyg_launch_fill_u32(c->d_counters + 8, 8, 0, s), "reset"))
+            return LYG_ERR_DEVICE;
+        L.round = (uint32_t)round;
+        L.stage_v = as<float>(c->f_stage_v[round]);
+        L.stage_t = as<uint32_t>(c->f_stage_t[round]);
+        // one launch per size class; the classes share the working memory, sized once
+        struct Launch {
+            size_t k0, k1, blocks, stride;
+            Caps cap;
+        };
+        std::vector<Launch> launches;
+        size_t work_bytes = 0;
+        for (size_t k0 = 0; k0 < m;) {
+            int kc = cls[order[k0]];
+            size_t k1 = k0;
+            uint64_t maxb = 0;
+            while (k1 < m && cls[order[k1]] == kc) {
+                maxb = std::max<uint64_t>(maxb, bound[list[k1]]);
+                ++k1;
+            }
+            Launch ln;
+            ln.k0 = k0;
+            ln.k1 = k1;
+            ln.cap = class_caps(std::max<uint64_t>(maxb, 1), scale);
+            ln.stride = lyg_fill_work_bytes(ln.cap.ev, ln.cap.heap, ln.cap.act);
+            const size_t per_block = ln.stride * FILL_WAVES_PER_BLOCK;
+            size_t blocks = (k1 - k0 + FILL_WAVES_PER_BLOCK - 1) / FILL_WAVES_PER_BLOCK;
+            blocks = std::min<size_t>(blocks, (size_t)c->cu_count * 8);
+            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, WORK_BUDGET / per_block));
+            work_bytes = std::max(work_bytes, ln.blocks * per_block);
+            launches.push_back(ln);
+            k0 = k1;
+        }
+        if (!ensure(c, c->f_work, work_bytes)) return LYG_ERR_NO_SPACE;
+        if (c->timing && round == 0) (void)hipEventRecord(c->ev[2], s);
+        for (size_t li = 0; li < launches.size(); ++li) {
+            const Launch& ln = launches[li];
+            L.work = as<uint8_t>(c->f_work);
+            L.work_stride = ln.stride;
+            L.ev_cap = ln.cap.ev;
+            L.heap_cap = ln.cap.heap;
+            L.act_cap = ln.cap.act;
+            L.path_list = d_list + ln.k0;
+            L.n_list = (uint32_t)(ln.k1 - ln.k0);
+            L.stage_v_off = d_off + ln.k0;
+            L.stage_t_off = d_off + (m + 1) + ln.k0;
+            L.work_counter = c->d_counters + 8 + (li & 7);
+            if (!check(c, lyg_launch_fill_sweep(&L, (int)ln.blocks, s), "fill sweep"))
+                return LYG_ERR_DEVICE;
+        }
+        if (c->timing && round == 0) (void)hipEventRecord(c->ev[3], s);
+        R->round_len.push_back((uint32_t)m);
+        std::vector<uint32_t> st_h(n);
+        if (!check(c, hipMemcpyAsync(st_h.data(), L.path_status, 4ull * n, hipMemcpyDeviceToHost, s),
+                   "D2H") ||
+            !check(c, hipStreamSynchronize(s), "sync"))
+            return LYG_ERR_DEVICE;
+        std::vector<uint32_t> next;
+        for (uint32_t p : list) {
+            if (st_h[p] == ST_RETRY) next.push_back(p);
+            else R->status[p] = st_h[p];
+        }
+        todo.swap(next);
+    }
+    R->rounds = round;
+    for (uint32_t p : todo) R->status[p] = LYG_ERR_NO_SPACE;  // still out of room
+    // step 3: offsets
+    if (!check(c, lyg_launch_fill_u32(c->d_counters + 4, 1, 0, s), "reset") ||
+        !check(c, lyg_launch_scan(L.path_nv, n, vtx_off, c->d_totals + 0, c->d_counters + 4, s), "scan") ||
+        !check(c, lyg_launch_scan(L.path_ni, n, idx_off, c->d_totals + 1, c->d_counters + 4, s), "scan"))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    R->total_v = h[0];
+    R->total_i = h[1];
+    if ((uint32_t)(h[2] & 0xffffffffu)) return LYG_ERR_TOO_MANY_VERTICES;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
+    if (R->total_v > 0 && (uint64_t)J.vertex_offset + R->total_v - 1 > 0xfffffffeull)
+        return LYG_ERR_TOO_MANY_VERTICES;
+    return LYG_OK;
+}
+
+// Step 3: every round's staged output to the final arrays.
+int32_t fill_copy(lyg_ctx* c, const FillJob& J, const FillResult& R, const uint32_t* vtx_off,
+                  const uint32_t* idx_off, float* out_v, uint32_t* out_i) {
+    const uint32_t n = J.in.n_paths;
+    // round r swept the paths of f_list[r] into its own staging; a path's output is in the
+    // round recorded in f_round.
+    lyg::FillLaunch L{};
+    L.in = J.in;
+    L.path_status = as<uint32_t>(c->f_status);
+    L.path_round = as<uint32_t>(c->f_round);
+    L.vtx_off = vtx_off;
+    L.idx_off = idx_off;
+    L.out_vertices = out_v;
+    L.out_indices = out_i;
+    L.vertex_offset = J.vertex_offset;
+    (void)n;
+    if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
+    for (int r = 0; r < R.rounds; ++r) {
+        L.round = (uint32_t)r;
+        L.path_list = as<uint32_t>(c->f_list[r]);
+        L.stage_v = as<float>(c->f_stage_v[r]);
+        L.stage_t = as<uint32_t>(c->f_stage_t[r]);
+        L.n_list = R.round_len[r];
+        L.stage_v_off = as<uint64_t>(c->f_stage_off[r]);
+        L.stage_t_off = as<uint64_t>(c->f_stage_off[r]) + (R.round_len[r] + 1);
+        if (!check(c, lyg_launch_fill_copy(&L, c->stream), "fill copy")) return LYG_ERR_DEVICE;
+    }
+    if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
+    return LYG_OK;
+}
+
+int32_t first_error(const FillResult& R) {
+    for (uint32_t s : R.status)
+        if (s != 0) return (int32_t)s;
+    return LYG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// FillTessellator::tessellate_path (fill.rs:628-641) into caller-owned host arrays.
+int32_t lyg_fill_path(lyg_ctx* c, const lyg_path_view* path, const lyg_fill_options* opts,
+                      lyg_output* out) {
+    if (!c || !path || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
+    if (opts->fill_rule > 1 || opts->sweep_orientation > 1) return LYG_ERR_INVALID_ARGUMENT;
+    if ((path->n_verbs && !path->verbs) || (path->n_points && !path->points))
+        return LYG_ERR_INVALID_ARGUMENT;
+    out->n_vertices = 0;
+    out->n_indices = 0;
+    // tessellate_impl checks the tolerance before anything else is observable (fill.rs:755-759)
+    if (!(opts->tolerance > 0.0f)) return LYG_ERR_TOLERANCE_IS_NAN;
+    if (path->n_verbs == 0) return LYG_OK;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    if (!ensure(c, c->in_verbs, path->n_verbs) || !ensure(c, c->in_points, 8ull * path->n_points) ||
+        !ensure(c, c->in_offs, 16) || !ensure(c, c->in_opts, sizeof(lyg_fill_options)) ||
+        !ensure(c, c->offs_v, 8) || !ensure(c, c->offs_i, 8))
+        return LYG_ERR_DEVICE;
+    uint32_t* h = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
+    h[0] = 0;
+    h[1] = path->n_verbs;
+    h[2] = 0;
+    h[3] = path->n_points;
+    hipStream_t s = c->stream;
+    if (!check(c, hipMemcpyAsync(c->in_verbs.p, path->verbs, path->n_verbs, hipMemcpyHostToDevice, s), "H2D") ||
+        (path->n_points &&
+         !check(c, hipMemcpyAsync(c->in_points.p, path->points, 8ull * path->n_points, hipMemcpyHostToDevice, s), "H2D")) ||
+        !check(c, hipMemcpyAsync(c->in_offs.p, h, 16, hipMemcpyHostToDevice, s), "H2D") ||
+        !check(c, hipMemcpyAsync(c->in_opts.p, opts, sizeof(*opts), hipMemcpyHostToDevice, s), "H2D"))
+        return LYG_ERR_DEVICE;
+    FillJob J;
+    J.in.verbs = as<uint8_t>(c->in_verbs);
+    J.in.points = as<float>(c->in_points);
+    J.in.verb_off = as<uint32_t>(c->in_offs);
+    J.in.point_off = as<uint32_t>(c->in_offs) + 2;
+    J.in.n_paths = 1;
+    J.in.first_path = 0;
+    J.opts = as<lyg_fill_options>(c->in_opts);
+    J.opts_stride = 0;
+    J.vertex_offset = out->vertex_offset;
+    FillResult R;
+    int32_t st = fill_sweep(c, J, as<uint32_t>(c->offs_v), as<uint32_t>(c->offs_i), &R);
+    if (st != LYG_OK) return st;
+    st = first_error(R);
+    if (st != LYG_OK) return st;
+    const uint64_t tv = R.total_v, ti = R.total_i;
+    if (tv > out->vertex_capacity || ti > out->index_capacity) {
+        out->n_vertices = (uint32_t)tv;
+        out->n_indices = (uint32_t)ti;
+        return LYG_ERR_CAPACITY;
+    }
+    if (tv == 0 && ti == 0) return LYG_OK;
+    if (!ensure(c, c->out_v, 8 * tv) || !ensure(c, c->out_i, 4 * ti)) return LYG_ERR_DEVICE;
+    st = fill_copy(c, J, R, as<uint32_t>(c->offs_v), as<uint32_t>(c->offs_i), as<float>(c->out_v),
+                   as<uint32_t>(c->out_i));
+    if (st != LYG_OK) return st;
+    if (!check(c, hipMemcpyAsync(out->vertices, c->out_v.p, 8 * tv, hipMemcpyDeviceToHost, s), "D2H") ||
+        (ti && !check(c, hipMemcpyAsync(out->indices, c->out_i.p, 4 * ti, hipMemcpyDeviceToHost, s), "D2H")) ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    out->n_vertices = (uint32_t)tv;
+    out->n_indices = (uint32_t)ti;
+    return LYG_OK;
+}
+
+// Batched fill: every path as one tessellate_path call into a shared VertexBuffers; a
+// path that fails contributes nothing (its geometry is aborted, fill.rs:788-793) and the
+// first failing path's status is returned.
+int32_t lyg_fill_batch(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_options* opts,
+                       uint32_t n_opts, lyg_batch_out* out) {
+    if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
+    if (n_opts != 1 && n_opts != in->n_paths) return LYG_ERR_INVALID_ARGUMENT;
+    out->n_vertices = 0;
+    out->n_indices = 0;
+    if (in->n_paths == 0) return LYG_OK;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    FillJob J;
+    J.in.verbs = in->verbs;
+    J.in.points = in->points;
+    J.in.verb_off = in->verb_offsets;
+    J.in.point_off = in->point_offsets;
+    J.in.n_paths = in->n_paths;
+    J.in.first_path = 0;
+    J.opts = opts;
+    J.opts_stride = n_opts == 1 ? 0 : 1;
+    J.vertex_offset = out->vertex_offset;
+    FillResult R;
+    int32_t st = fill_sweep(c, J, out->path_vertex_offsets, out->path_index_offsets, &R);
+    if (st != LYG_OK) return st;
+    out->n_vertices = R.total_v;
+    out->n_indices = R.total_i;
+    if (out->path_status) {
+        if (!check(c, hipMemcpyAsync(out->path_status, R.status.data(), 4ull * in->n_paths,
+                                     hipMemcpyHostToDevice, c->stream), "H2D") ||
+            !check(c, hipStreamSynchronize(c->stream), "sync"))
+            return LYG_ERR_DEVICE;
+    }
+    if (R.total_v > out->vertex_capacity || R.total_i > out->index_capacity) return LYG_ERR_CAPACITY;
+    st = fill_copy(c, J, R, out->path_vertex_offsets, out->path_index_offsets, out->vertices,
+                   out->indices);
+    if (st != LYG_OK) return st;
+    if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
+    return first_error(R);
+}
+
+}  // extern "C"

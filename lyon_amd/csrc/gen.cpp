@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "../../include/lyg.h"
 
@@ -131,6 +132,114 @@ int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance,
         o.line_join = joins[j];
         o.start_cap = (uint8_t)c;
         o.end_cap = (uint8_t)c;
+        o._pad = 0;
+        out[i] = o;
+    }
+    return LYG_OK;
+}
+
+// The batched fill workload: closed paths in [0, 4096]^2, even paths are star polygons,
+// odd paths cubic loops.
+//  * star: n points (n log-uniform in [5, 512]) at angles 2*pi*k*j/n, k the smallest
+//    integer >= n/3 coprime with n (so the polygon {n/k} crosses itself), each at radius
+//    R * U[0.2, 1] with R uniform in [16, 512] around a centre in [R, 4096 - R]^2;
+//  * cubic loop: m cubics (m log-uniform in [4, 256]) through anchors at angles 2*pi*j/m
+//    and radius R * U[0.5, 1], control points at anchor +- R * U[0, 0.6] in random
+//    directions (loops and cusps appear), R uniform in [16, 512].
+int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, float* points,
+                           uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
+                           uint64_t* n_points) {
+    uint64_t nv = 0, np = 0;
+    const double two_pi = 6.283185307179586;
+    for (uint32_t i = 0; i < n_paths; ++i) {
+        Pcg32 rng(path_seed(seed ^ 0x5eedf111ull, i));
+        if (verb_offsets) verb_offsets[i] = (uint32_t)nv;
+        if (point_offsets) point_offsets[i] = (uint32_t)np;
+        auto put = [&](float x, float y) {
+            if (points) {
+                points[2 * np] = x;
+                points[2 * np + 1] = y;
+            }
+            np++;
+        };
+        auto vb = [&](uint8_t v) {
+            if (verbs) verbs[nv] = v;
+            nv++;
+        };
+        const float R = 16.0f + rng.uniform() * (512.0f - 16.0f);
+        const float cx = R + rng.uniform() * (4096.0f - 2.0f * R);
+        const float cy = R + rng.uniform() * (4096.0f - 2.0f * R);
+        const uint64_t first = np;
+        if ((i & 1u) == 0) {
+            uint32_t n = (uint32_t)std::lround(std::exp(std::log(5.0) + rng.uniform() * (std::log(512.0) - std::log(5.0))));
+            n = n < 5 ? 5 : (n > 512 ? 512 : n);
+            uint32_t k = (n + 2) / 3;
+            auto gcd = [](uint32_t a, uint32_t b) {
+                while (b) {
+                    uint32_t t = a % b;
+                    a = b;
+                    b = t;
+                }
+                return a;
+            };
+            while (gcd(n, k) != 1) ++k;
+            vb(LYG_VERB_BEGIN);
+            for (uint32_t j = 0; j < n; ++j) {
+                double a = two_pi * (double)((uint64_t)k * j % n) / (double)n;
+                float r = R * (0.2f + 0.8f * rng.uniform());
+                put(cx + r * (float)std::cos(a), cy + r * (float)std::sin(a));
+                if (j > 0) vb(LYG_VERB_LINE);
+            }
+        } else {
+            uint32_t m = (uint32_t)std::lround(std::exp(std::log(4.0) + rng.uniform() * (std::log(256.0) - std::log(4.0))));
+            m = m < 4 ? 4 : (m > 256 ? 256 : m);
+            std::vector<float> ax(m), ay(m);
+            for (uint32_t j = 0; j < m; ++j) {
+                double a = two_pi * (double)j / (double)m;
+                float r = R * (0.5f + 0.5f * rng.uniform());
+                ax[j] = cx + r * (float)std::cos(a);
+                ay[j] = cy + r * (float)std::sin(a);
+            }
+            vb(LYG_VERB_BEGIN);
+            put(ax[0], ay[0]);
+            for (uint32_t j = 0; j < m; ++j) {
+                uint32_t q = (j + 1) % m;
+                for (int c = 0; c < 2; ++c) {
+                    float bx = c == 0 ? ax[j] : ax[q], by = c == 0 ? ay[j] : ay[q];
+                    double d = two_pi * rng.uniform();
+                    float len = R * 0.6f * rng.uniform();
+                    float x = bx + len * (float)std::cos(d), y = by + len * (float)std::sin(d);
+                    x = x < 0.0f ? 0.0f : (x > 4096.0f ? 4096.0f : x);
+                    y = y < 0.0f ? 0.0f : (y > 4096.0f ? 4096.0f : y);
+                    put(x, y);
+                }
+                put(ax[q], ay[q]);
+                vb(LYG_VERB_CUBIC);
+            }
+        }
+        // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
+        vb(LYG_VERB_CLOSE);
+        if (points) put(points[2 * first], points[2 * first + 1]);
+        else np++;
+    }
+    if (verb_offsets) verb_offsets[n_paths] = (uint32_t)nv;
+    if (point_offsets) point_offsets[n_paths] = (uint32_t)np;
+    if (n_verbs) *n_verbs = nv;
+    if (n_points) *n_points = np;
+    return LYG_OK;
+}
+
+// Per-path FillOptions of the batched fill workload: EvenOdd / NonZero with equal
+// probability, vertical sweep, intersections handled.
+int32_t lyg_gen_fill_options(uint64_t seed, uint32_t n_paths, float tolerance,
+                             lyg_fill_options* out) {
+    for (uint32_t i = 0; i < n_paths; ++i) {
+        Pcg32 rng(path_seed(seed ^ 0x0f111f111ull, i));
+        lyg_fill_options o;
+        o.tolerance = tolerance;
+        o.fill_rule = (uint8_t)(rng.next() & 1u);
+        o.sweep_orientation = LYG_ORIENT_VERTICAL;
+        o.handle_intersections = 1;
         o._pad = 0;
         out[i] = o;
     }

@@ -29,78 +29,9 @@ hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 }
 
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-}  // namespace
-
-struct lyg_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int cu_count = 256;
-    std::string last_error;
-    // grow-only device scratch
-    DevBuf in_verbs, in_points, in_offs, in_opts, counts_v, counts_i, offs_v, offs_i, out_v,
-        out_i, misc, arcs, flat_kinds, flat_ctrl;
-    uint32_t* d_counters = nullptr;  // [0..3] work-queue heads, [4] overflow flag
-    uint64_t* d_totals = nullptr;    // [0] vertices, [1] indices
-    void* h_pinned = nullptr;        // small pinned staging area for totals
-    // optional kernel timing: events around the count and emit kernels on `stream`
-    bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-};
+#include "ctx.h"
 
 namespace {
-
-// LYG_TRACE=1 prints every host step to stderr (flushed), for diagnosing a run that
-// goes quiet.
-bool trace_on() {
-    static int on = -1;
-    if (on < 0) {
-        const char* v = getenv("LYG_TRACE");
-        on = (v && *v && *v != '0') ? 1 : 0;
-    }
-    return on == 1;
-}
-
-bool check(lyg_ctx* c, hipError_t e, const char* what) {
-    if (trace_on()) {
-        fprintf(stderr, "[lyg] %s -> %d\n", what, (int)e);
-        fflush(stderr);
-    }
-    if (e == hipSuccess) return true;
-    c->last_error = std::string(what) + ": " + hipGetErrorString(e);
-    return false;
-}
-
-bool ensure(lyg_ctx* c, DevBuf& b, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (b.cap >= bytes) return true;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-    size_t cap = std::max(bytes, (size_t)4096);
-    cap = (cap + 4095) & ~(size_t)4095;
-    if (!check(c, hipMalloc(&b.p, cap), "hipMalloc")) return false;
-    b.cap = cap;
-    return true;
-}
-
-template <class T>
-T* as(DevBuf& b) {
-    return reinterpret_cast<T*>(b.p);
-}
-
-int persistent_grid(lyg_ctx* c, uint32_t n_paths) {
-    // 4 waves per block, one path per wave at a time; 8 blocks per CU keep 32 waves/CU.
-    uint32_t blocks_needed = (n_paths + 3) / 4;
-    uint32_t cap = (uint32_t)c->cu_count * 8;
-    return (int)std::max(1u, std::min(blocks_needed, cap));
-}
 
 // Count pass + scan over the paths of `L`; fills per-path counts and offsets on the
 // device and the totals on the host.
@@ -200,8 +131,13 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     DevBuf* bufs[] = {&c->in_verbs, &c->in_points, &c->in_offs, &c->in_opts, &c->counts_v,
                       &c->counts_i, &c->offs_v,    &c->offs_i,  &c->out_v,   &c->out_i,
                       &c->misc,     &c->arcs,      &c->flat_kinds, &c->flat_ctrl};
-    for (DevBuf* b : bufs)
-        if (b->p) (void)hipFree(b->p);
+    for (DevBuf* b : bufs) free_buf(*b);
+    free_buf(c->f_bound);
+    free_buf(c->f_status);
+    free_buf(c->f_round);
+    free_buf(c->f_work);
+    for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
+        for (DevBuf& b : *v) free_buf(b);
     if (c->d_counters) (void)hipFree(c->d_counters);
     if (c->d_totals) (void)hipFree(c->d_totals);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -250,6 +186,46 @@ int32_t lyg_ctx_kernel_times(lyg_ctx* c, float* count_ms, float* emit_ms) {
     if (count_ms) *count_ms = a;
     if (emit_ms) *emit_ms = b;
     return LYG_OK;
+}
+
+// ---- device memory --------------------------------------------------------------------
+int32_t lyg_device_alloc(lyg_ctx* c, uint64_t bytes, void** ptr) {
+    if (!c || !ptr) return LYG_ERR_INVALID_ARGUMENT;
+    *ptr = nullptr;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    if (!check(c, hipMalloc(ptr, bytes ? bytes : 16), "hipMalloc")) return LYG_ERR_NO_SPACE;
+    return LYG_OK;
+}
+
+int32_t lyg_device_free(lyg_ctx* c, void* ptr) {
+    if (!c) return LYG_ERR_INVALID_ARGUMENT;
+    if (!ptr) return LYG_OK;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice") || !check(c, hipFree(ptr), "hipFree"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_copy_to_device(lyg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!bytes) return LYG_OK;
+    if (!check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "H2D") ||
+        !check(c, hipStreamSynchronize(c->stream), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_copy_to_host(lyg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!bytes) return LYG_OK;
+    if (!check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "D2H") ||
+        !check(c, hipStreamSynchronize(c->stream), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_ctx_synchronize(lyg_ctx* c) {
+    if (!c) return LYG_ERR_INVALID_ARGUMENT;
+    return check(c, hipStreamSynchronize(c->stream), "sync") ? LYG_OK : LYG_ERR_DEVICE;
 }
 
 // ---- plain per-path stroke -------------------------------------------------------------
@@ -385,6 +361,10 @@ int32_t lyg_stroke_batch(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_op
     L.out_indices = out->indices;
     st = stroke_emit(c, L);
     if (st != LYG_OK) return st;
+    if (out->path_status &&
+        !check(c, lyg_launch_fill_u32(reinterpret_cast<uint32_t*>(out->path_status), in->n_paths, 0,
+                                      c->stream), "status"))
+        return LYG_ERR_DEVICE;
     if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
     if (emit_failed(c)) return LYG_ERR_INTERNAL;
     return LYG_OK;
@@ -487,21 +467,6 @@ int32_t lyg_flatten_arcs(lyg_ctx* c, const float* arcs, uint32_t n, float tolera
 int32_t lyg_arc_to_quadratics(lyg_ctx* c, const float* arcs, uint32_t n, float* out_quads,
                               uint32_t quad_capacity, uint32_t* offsets, uint32_t* n_quads) {
     return arcs_common(c, arcs, n, 0.0f, true, out_quads, quad_capacity, offsets, n_quads);
-}
-
-}  // extern "C"
-
-extern "C" {
-
-// Fill entry points are provided by fill.hip when it is built; until then they report
-// LYG_ERR_UNSUPPORTED.
-__attribute__((weak)) int32_t lyg_fill_path(lyg_ctx*, const lyg_path_view*, const lyg_fill_options*,
-                                            lyg_output*) {
-    return LYG_ERR_UNSUPPORTED;
-}
-__attribute__((weak)) int32_t lyg_fill_batch(lyg_ctx*, const lyg_batch_in*, const lyg_fill_options*,
-                                             uint32_t, lyg_batch_out*) {
-    return LYG_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"

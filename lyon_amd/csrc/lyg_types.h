@@ -48,7 +48,9 @@ struct FillLaunch {
     uint32_t* path_nv;
     uint32_t* path_ni;
     uint32_t* path_status;
-    const uint32_t* path_list;     // paths to process (nullptr: first_path + 0..n_list)
+    uint32_t* path_round;          // the round whose staging holds the path's output
+    uint32_t round;
+    const uint32_t* path_list;     // paths to process (nullptr: 0..n_list)
     uint32_t n_list;
     uint32_t* work_counter;
     // copy pass

@@ -127,3 +127,70 @@ def batch_triangles(batch, opts, is_fill, n_threads=1):
                                      _ptr(batch.verb_offsets, U32P),
                                      _ptr(batch.point_offsets, U32P), _ptr(o, F32P), len(batch),
                                      int(is_fill), int(n_threads))
+
+
+def _box(points, pad_frac=0.01):
+    p = np.asarray(points, np.float64).reshape(-1, 2)
+    p = p[np.isfinite(p).all(axis=1)]
+    if len(p) == 0:
+        return 0.0, 0.0, 1.0, 1.0
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    ext = max(float((hi - lo).max()), 1e-6)
+    pad = ext * pad_frac
+    # a square box keeps the samples isotropic
+    cx, cy = (lo + hi) / 2
+    h = ext / 2 + pad
+    return cx - h, cy - h, cx + h, cy + h
+
+
+def coverage_compare(va, ia, vb, ib, box=None, n=4096, eps=1e-4):
+    """Per-sample coverage of two triangle sets (vertices (k,2) f32, local u32 indices) on
+    an n x n grid; samples within eps of a triangle edge are excluded.  Returns a dict:
+    mismatch, excluded, covered_a, covered_b, overlap_a, overlap_b, area_a, area_b."""
+    va = np.ascontiguousarray(va, np.float32).reshape(-1, 2)
+    vb = np.ascontiguousarray(vb, np.float32).reshape(-1, 2)
+    ia = np.ascontiguousarray(ia, np.uint32)
+    ib = np.ascontiguousarray(ib, np.uint32)
+    if box is None:
+        box = _box(np.concatenate([va, vb]))
+    L = lib()
+    L.orc_coverage_compare.argtypes = [F32P, U32P, C.c_uint64, F32P, U32P, C.c_uint64,
+                                       C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_uint32, C.c_double, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double)]
+    out = (C.c_uint64 * 8)()
+    area = (C.c_double * 2)()
+    L.orc_coverage_compare(_ptr(va, F32P), _ptr(ia, U32P), len(ia) // 3, _ptr(vb, F32P),
+                           _ptr(ib, U32P), len(ib) // 3, *box, n, eps, out, area)
+    keys = ("mismatch", "excluded", "covered_a", "covered_b", "overlap_a", "overlap_b")
+    r = {k: int(out[j]) for j, k in enumerate(keys)}
+    r["area_a"], r["area_b"] = float(area[0]), float(area[1])
+    return r
+
+
+def winding_compare(path, tolerance, fill_rule, v, i, box=None, n=4096, eps=1e-4):
+    """Triangles (v, i) against the textbook winding-number coverage of the path's
+    flattened edges (vertical sweep flattening).  Keys as coverage_compare, with a =
+    triangles, b = winding, plus n_edges."""
+    v = np.ascontiguousarray(v, np.float32).reshape(-1, 2)
+    i = np.ascontiguousarray(i, np.uint32)
+    pv = np.ascontiguousarray(path.verbs, np.uint8)
+    pp = np.ascontiguousarray(path.points, np.float32)
+    if box is None:
+        box = _box(pp)
+    L = lib()
+    L.orc_winding_coverage_compare.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float,
+                                               C.c_int, F32P, U32P, C.c_uint64, C.c_double,
+                                               C.c_double, C.c_double, C.c_double, C.c_uint32,
+                                               C.c_double, C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_double)]
+    out = (C.c_uint64 * 8)()
+    area = (C.c_double * 2)()
+    L.orc_winding_coverage_compare(_ptr(pv, U8P), len(pv), _ptr(pp, F32P), len(pp), tolerance,
+                                   fill_rule, _ptr(v, F32P), _ptr(i, U32P), len(i) // 3, *box,
+                                   n, eps, out, area)
+    keys = ("mismatch", "excluded", "covered_a", "covered_b", "overlap_a", "overlap_b",
+            "n_edges")
+    r = {k: int(out[j]) for j, k in enumerate(keys)}
+    r["area_a"] = float(area[0])
+    return r

@@ -539,6 +539,10 @@ struct Span {
     std::unique_ptr<AdvancedMono> tess;
 };
 
+// A Rust panic of the reference (an out-of-bounds index, or unreachable!() on a span that
+// has already ended): the oracle reports it as the PANIC status instead of a result.
+struct LyonPanic {};
+
 struct Tessellator {
     V2 current_position{-FLT_MAX, -FLT_MAX};
     uint32_t current_vertex = INVALID_VERTEX;
@@ -556,7 +560,18 @@ struct Tessellator {
     Output* out = nullptr;
 
     // Spans (fill.rs:223-300)
+    // Spans::begin_span / end_span (fill.rs:233-263): Vec::insert and indexing panic
+    // out of range, end_span on an ended span reaches unreachable!().
+    AdvancedMono& live(int32_t idx) {
+        if (idx < 0 || (size_t)idx >= spans.size() || !spans[idx].tess) throw LyonPanic();
+        return *spans[idx].tess;
+    }
+    ActiveEdge& act(size_t i) {
+        if (i >= active.size()) throw LyonPanic();
+        return active[i];
+    }
     void begin_span(int32_t idx, V2 pos, uint32_t vertex) {
+        if (idx < 0 || (size_t)idx > spans.size()) throw LyonPanic();
         std::unique_ptr<AdvancedMono> t;
         if (!pool.empty()) {
             t = std::move(pool.back());
@@ -568,6 +583,7 @@ struct Tessellator {
         spans.insert(spans.begin() + idx, Span{std::move(t)});
     }
     void end_span(int32_t idx, V2 pos, uint32_t id) {
+        live(idx);
         Span& s = spans[idx];
         if (s.tess) {
             s.tess->end(pos, id);
@@ -733,18 +749,18 @@ struct Tessellator {
     // fill.rs:1381-1452
     void process_edges_above() {
         for (auto& ve : scan.vertex_events)
-            spans[ve.first].tess->vertex(current_position, current_vertex, ve.second);
+            live(ve.first).vertex(current_position, current_vertex, ve.second);
         for (int32_t si : scan.spans_to_end) end_span(si, current_position, current_vertex);
         cleanup_spans();
         for (size_t ei : scan.edges_to_split) {
-            ActiveEdge& e = active[ei];
+            ActiveEdge& e = act(ei);
             V2 to = e.to;
             edges_below.push_back(
                 PendingEdge{to, slope(to - current_position), e.src_edge, e.winding, e.range_end});
             e.to = current_position;
         }
         if (scan.merge_event) {
-            ActiveEdge& e = active[scan.above_start];
+            ActiveEdge& e = act(scan.above_start);
             e.is_merge = true;
             e.from = e.to;
             e.winding = 0;
@@ -809,7 +825,7 @@ struct Tessellator {
     // fill.rs:1563-1608
     void split_event(size_t left_idx, int32_t left_span) {
         size_t right_idx = left_idx + 1;
-        V2 ul = active[left_idx].from, ur = active[right_idx].from;
+        V2 ul = act(left_idx).from, ur = act(right_idx).from;
         int32_t right_span = left_span + 1;
         V2 up;
         uint32_t uid;
@@ -824,8 +840,8 @@ struct Tessellator {
             new_span = right_span;
         }
         begin_span(new_span, up, uid);
-        spans[left_span].tess->vertex(current_position, current_vertex, RIGHT);
-        spans[right_span].tess->vertex(current_position, current_vertex, LEFT);
+        live(left_span).vertex(current_position, current_vertex, RIGHT);
+        live(right_span).vertex(current_position, current_vertex, LEFT);
     }
 
     // fill.rs:1455-1524
@@ -997,6 +1013,11 @@ struct Tessellator {
                 prev_x = x;
             }
         }
+        // keys.sort_by(|a, b| a.0.partial_cmp(&b.0).unwrap() ..): a NaN key (a merge
+        // vertex ahead of every edge) panics as soon as two keys are compared.
+        if (keys.size() >= 2)
+            for (auto& k : keys)
+                if (std::isnan(k.first)) throw LyonPanic();
         std::stable_sort(keys.begin(), keys.end(), [&](const std::pair<float, size_t>& a,
                                                         const std::pair<float, size_t>& b) {
             if (a.first < b.first) return true;
@@ -1070,6 +1091,31 @@ struct Tessellator {
     }
 };
 
+// The edges the event builder produces for `path` (vertical sweep), each as x0 y0 x1 y1
+// in the path's own direction: the polygon a fill covers.
+void flattened_edges(const refpath::PathView& path, float tolerance, std::vector<float>& out) {
+    EventQueueBuilder b;
+    b.tolerance = tolerance;
+    refpath::Iter it(path);
+    refpath::Event e;
+    while (it.next(e)) {
+        switch (e.kind) {
+            case refpath::EV_BEGIN: b.begin(e.to); break;
+            case refpath::EV_LINE: b.line_segment(e.to, 0.0f, 1.0f); break;
+            case refpath::EV_QUAD: b.quadratic_bezier_segment(e.ctrl1, e.to); break;
+            case refpath::EV_CUBIC: b.cubic_bezier_segment(e.ctrl1, e.ctrl2, e.to); break;
+            case refpath::EV_END: b.end(e.to); break;
+        }
+    }
+    for (size_t i = 0; i < b.queue.events.size(); ++i) {
+        const EdgeData& d = b.queue.edge_data[i];
+        if (!d.is_edge) continue;
+        V2 p = b.queue.events[i].position, q = d.to;
+        if (d.winding < 0) std::swap(p, q);
+        out.insert(out.end(), {p.x, p.y, q.x, q.y});
+    }
+}
+
 // FillTessellator::tessellate_path -> tessellate -> tessellate_impl (fill.rs:578-821)
 int fill_path(const refpath::PathView& path, const FillOptions& opts, Output& out) {
     Tessellator t;
@@ -1099,7 +1145,12 @@ int fill_path(const refpath::PathView& path, const FillOptions& opts, Output& ou
     t.tolerance = opts.tolerance * 0.5f;
     t.out = &out;
     out.begin_geometry();
-    int st = t.tessellator_loop();
+    int st;
+    try {
+        st = t.tessellator_loop();
+    } catch (const LyonPanic&) {
+        st = PANIC;
+    }
     if (st != OK) {
         out.abort_geometry();
         return st;

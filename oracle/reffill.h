@@ -25,6 +25,7 @@ enum Status {
     POSITION_IS_NAN = 3,
     TOLERANCE_IS_NAN = 4,
     INTERNAL = 5,
+    PANIC = 6,  // the reference panics (Rust index out of bounds / unreachable!())
 };
 
 struct Output {
@@ -55,5 +56,8 @@ struct Output {
 };
 
 int fill_path(const refpath::PathView& path, const FillOptions& opts, Output& out);
+
+// The flattened edges of the event builder (x0 y0 x1 y1 each, in path direction).
+void flattened_edges(const refpath::PathView& path, float tolerance, std::vector<float>& out);
 
 }  // namespace reffill

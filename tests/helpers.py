@@ -1,7 +1,12 @@
-"""Shared comparison helpers for the parity tests."""
+"""Shared comparison helpers and input corpora for the parity tests."""
+import json
+import os
+
 import numpy as np
 
-from lyon_amd.path import PathBuilder
+from lyon_amd.path import PathBuilder, polygon
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def bbox_diag(points):
@@ -58,3 +63,88 @@ def empty_caps_path():
     b.begin((3, 0)); b.line_to((3, 0)); b.line_to((3, 0)); b.line_to((3, 0)); b.end(True)
     b.begin((4, 0)); b.end(False)
     return b.build()
+
+
+def fixture_path(subs, earcut=False):
+    """A path from a fuzz_tests.rs / earcut_tests.rs fixture (tests/golden/*.json)."""
+    b = PathBuilder()
+    for s in subs:
+        pts = s if earcut else s["points"]
+        if not pts:
+            continue
+        if earcut:
+            b.begin((pts[0][0], pts[0][0]))  # earcut_tests.rs:20192 begins at (x0, x0)
+            for p in pts:
+                b.line_to(p)
+            b.end(True)
+        else:
+            b.begin(pts[0])
+            for p in pts[1:]:
+                b.line_to(p)
+            b.end(s["closed"])
+    return b.build()
+
+
+def fuzz_fixtures():
+    cases = json.load(open(os.path.join(HERE, "golden", "fuzz_polygons.json")))
+    return [(name, fixture_path(subs)) for name, subs in cases.items()]
+
+
+def earcut_fixtures():
+    cases = json.load(open(os.path.join(HERE, "golden", "earcut_polygons.json")))
+    return [("earcut_" + name, fixture_path(rings, earcut=True)) for name, rings in cases.items()]
+
+
+# fill_tests.rs KAT polygons: (name, points, rotation step, EvenOdd triangle count)
+FILL_KATS = [
+    ("simple_triangle", [(0, 0), (1, 1), (0, 1)], 0.01, 1),
+    ("simple_split", [(0, 0), (2, 1), (2, 3), (1, 2), (0, 3)], 0.01, 3),
+    ("simple_merge_split", [(0, 0), (1, 1), (2, 0), (2, 3), (1, 2), (0, 3)], 0.01, 4),
+    ("simple_aligned", [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1)], 0.01, 6),
+    ("simple_1", [(0, 0), (1, 1), (2, 0), (1, 3), (0.5, 4), (0, 3)], 0.01, 4),
+    ("simple_2", [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3), (2, 3), (1, 3), (0, 3),
+                  (0, 2), (0, 1)], 0.01, 10),
+]
+
+
+def hole_path():
+    """fill_tests.rs test_hole_1."""
+    b = PathBuilder()
+    b.begin((-11, 5)); b.line_to((0, -5)); b.line_to((10, 5)); b.end(True)
+    b.begin((-5, 2)); b.line_to((0, -2)); b.line_to((4, 2)); b.end(True)
+    return b.build()
+
+
+def overlapping_squares_path():
+    """fill_tests.rs test_simple_double_merge-like: two overlapping squares."""
+    b = PathBuilder()
+    b.begin((20, 20)); b.line_to((60, 20)); b.line_to((60, 60)); b.line_to((20, 60)); b.end(True)
+    b.begin((40, 10)); b.line_to((70, 40)); b.line_to((40, 70)); b.line_to((10, 40)); b.end(True)
+    return b.build()
+
+
+def star_paths(n_paths, seed=5, max_points=80):
+    """Self-intersecting random stars: points at unsorted random angles."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_paths):
+        n = int(rng.integers(5, max_points))
+        ang = rng.uniform(0, 2 * np.pi, n)
+        r = rng.uniform(0.2, 1.0, n) * 100
+        pts = np.stack([200 + r * np.cos(ang), 200 + r * np.sin(ang)], 1).astype(np.float32)
+        out.append((f"star{k}", polygon([tuple(p) for p in pts], True)))
+    return out
+
+
+def fill_corpus():
+    """Every fill input of the parity tests: KAT polygons, lyon's fuzz and earcut fixtures,
+    the plain workload path and random self-intersecting stars."""
+    from lyon_amd import workloads
+    out = [(name, polygon(pts, True)) for name, pts, _, _ in FILL_KATS]
+    out.append(("hole", hole_path()))
+    out.append(("overlapping_squares", overlapping_squares_path()))
+    out.append(("plain", workloads.plain_path(0)))
+    out += fuzz_fixtures()
+    out += earcut_fixtures()
+    out += star_paths(12)
+    return out

@@ -20,7 +20,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import empty_caps_path, square_paths
+from helpers import FILL_KATS, empty_caps_path, fixture_path, square_paths
 from lyon_amd.path import Path, PathBuilder, polygon
 from oracle import pyoracle as O
 
@@ -117,17 +117,6 @@ def rotations(path, step, expected):
     return n
 
 
-FILL_KATS = [
-    ("simple_triangle", [(0, 0), (1, 1), (0, 1)], 0.01, 1),
-    ("simple_split", [(0, 0), (2, 1), (2, 3), (1, 2), (0, 3)], 0.01, 3),
-    ("simple_merge_split", [(0, 0), (1, 1), (2, 0), (2, 3), (1, 2), (0, 3)], 0.01, 4),
-    ("simple_aligned", [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1)], 0.01, 6),
-    ("simple_1", [(0, 0), (1, 1), (2, 0), (1, 3), (0.5, 4), (0, 3)], 0.01, 4),
-    ("simple_2", [(0, 0), (1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3), (2, 3), (1, 3), (0, 3),
-                  (0, 2), (0, 1)], 0.01, 10),
-]
-
-
 @pytest.mark.parametrize("name,pts,step,expected", FILL_KATS, ids=[k[0] for k in FILL_KATS])
 def test_fill_kat_rotations(name, pts, step, expected):
     """fill_tests.rs: exact EvenOdd triangle counts at every rotation (the reference
@@ -165,25 +154,6 @@ def test_fill_tolerance_errors():
     for t in (0.0, -1.0, float("nan")):
         st, v, i = O.fill(p, tolerance=t)
         assert st == 4 and len(v) == 0
-
-
-def fixture_path(subs, earcut=False):
-    b = PathBuilder()
-    for s in subs:
-        pts = s if earcut else s["points"]
-        if not pts:
-            continue
-        if earcut:
-            b.begin((pts[0][0], pts[0][0]))  # earcut_tests.rs:20192 begins at (x0, x0)
-            for p in pts:
-                b.line_to(p)
-            b.end(True)
-        else:
-            b.begin(pts[0])
-            for p in pts[1:]:
-                b.line_to(p)
-            b.end(s["closed"])
-    return b.build()
 
 
 def test_fuzz_fixtures_no_error():

@@ -147,7 +147,7 @@ def test_batch_matches_sequential_plain_calls():
     batch = workloads.stroke_batch(300, 8, 256, 2048.0, seed=0)
     opts = workloads.stroke_options(300, tolerance=0.05)
     res = StrokeBatcher().run(DeviceBatch(batch, opts))
-    vo = res.vertex_offsets.cpu().numpy().view(np.uint32)
+    vo = res.vertex_offsets
     for i in range(len(batch)):
         o = opts[i]
         so = StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
