@@ -212,8 +212,9 @@ int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance,
                                float w_max, lyg_stroke_options* out);
 
 /* The batched fill workload: closed paths in [0, 4096]^2, even paths self-intersecting
- * star polygons {n/k} (n log-uniform in [5, 512], radii R*U[0.2,1], R in [16, 512]), odd
- * paths loops of m cubics (m log-uniform in [4, 256]).  Same two-call protocol. */
+ * stars (5-512 points at unsorted random angles, radii R*U[0.2,1], R in [16, 512]), odd
+ * paths loops of 4-256 cubics scaled to flatten to 16-2048 edges at tolerance 0.05.
+ * Same two-call protocol. */
 int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, float* points,
                            uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
                            uint64_t* n_points);

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/lyg.h"
+#include "lyg_math.h"
 
 namespace {
 
@@ -138,30 +139,25 @@ int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance,
     return LYG_OK;
 }
 
-// The batched fill workload: closed paths in [0, 4096]^2, even paths are star polygons,
-// odd paths cubic loops.
-//  * star: n points (n log-uniform in [5, 512]) at angles 2*pi*k*j/n, k the smallest
-//    integer >= n/3 coprime with n (so the polygon {n/k} crosses itself), each at radius
-//    R * U[0.2, 1] with R uniform in [16, 512] around a centre in [R, 4096 - R]^2;
-//  * cubic loop: m cubics (m log-uniform in [4, 256]) through anchors at angles 2*pi*j/m
-//    and radius R * U[0.5, 1], control points at anchor +- R * U[0, 0.6] in random
-//    directions (loops and cusps appear), R uniform in [16, 512].
+// The batched fill workload: closed paths in [0, 4096]^2, even paths are stars, odd
+// paths cubic loops.
+//  * star: n points, n uniform in [5, 512], at angles uniform in [0, 2*pi) in draw order
+//    (unsorted, so the polygon crosses itself), radius R * U[0.2, 1] with R uniform in
+//    [16, 512], around a centre in [R, 4096 - R]^2;
+//  * cubic loop: m cubics, m uniform in [4, 256], through anchors at angles 2*pi*j/m and
+//    radius R * U[0.5, 1], control points at anchor + R * U[0, 0.6] in a random direction
+//    (loops and cusps appear); the loop is scaled about its centre until it flattens to
+//    16-2048 edges at tolerance 0.05 (Wang's counts, as the fill's event builder).
 int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, float* points,
                            uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
                            uint64_t* n_points) {
     uint64_t nv = 0, np = 0;
     const double two_pi = 6.283185307179586;
+    std::vector<float> pts;
     for (uint32_t i = 0; i < n_paths; ++i) {
         Pcg32 rng(path_seed(seed ^ 0x5eedf111ull, i));
         if (verb_offsets) verb_offsets[i] = (uint32_t)nv;
         if (point_offsets) point_offsets[i] = (uint32_t)np;
-        auto put = [&](float x, float y) {
-            if (points) {
-                points[2 * np] = x;
-                points[2 * np + 1] = y;
-            }
-            np++;
-        };
         auto vb = [&](uint8_t v) {
             if (verbs) verbs[nv] = v;
             nv++;
@@ -169,30 +165,20 @@ int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, floa
         const float R = 16.0f + rng.uniform() * (512.0f - 16.0f);
         const float cx = R + rng.uniform() * (4096.0f - 2.0f * R);
         const float cy = R + rng.uniform() * (4096.0f - 2.0f * R);
-        const uint64_t first = np;
+        pts.clear();
         if ((i & 1u) == 0) {
-            uint32_t n = (uint32_t)std::lround(std::exp(std::log(5.0) + rng.uniform() * (std::log(512.0) - std::log(5.0))));
-            n = n < 5 ? 5 : (n > 512 ? 512 : n);
-            uint32_t k = (n + 2) / 3;
-            auto gcd = [](uint32_t a, uint32_t b) {
-                while (b) {
-                    uint32_t t = a % b;
-                    a = b;
-                    b = t;
-                }
-                return a;
-            };
-            while (gcd(n, k) != 1) ++k;
+            uint32_t n = 5u + rng.next() % 508u;
             vb(LYG_VERB_BEGIN);
             for (uint32_t j = 0; j < n; ++j) {
-                double a = two_pi * (double)((uint64_t)k * j % n) / (double)n;
+                double a = two_pi * rng.uniform();
                 float r = R * (0.2f + 0.8f * rng.uniform());
-                put(cx + r * (float)std::cos(a), cy + r * (float)std::sin(a));
+                pts.push_back(cx + r * (float)std::cos(a));
+                pts.push_back(cy + r * (float)std::sin(a));
                 if (j > 0) vb(LYG_VERB_LINE);
             }
         } else {
-            uint32_t m = (uint32_t)std::lround(std::exp(std::log(4.0) + rng.uniform() * (std::log(256.0) - std::log(4.0))));
-            m = m < 4 ? 4 : (m > 256 ? 256 : m);
+            uint32_t m = 4u + rng.next() % 253u;
+            vb(LYG_VERB_BEGIN);
             std::vector<float> ax(m), ay(m);
             for (uint32_t j = 0; j < m; ++j) {
                 double a = two_pi * (double)j / (double)m;
@@ -200,27 +186,47 @@ int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, floa
                 ax[j] = cx + r * (float)std::cos(a);
                 ay[j] = cy + r * (float)std::sin(a);
             }
-            vb(LYG_VERB_BEGIN);
-            put(ax[0], ay[0]);
+            pts.push_back(ax[0]);
+            pts.push_back(ay[0]);
             for (uint32_t j = 0; j < m; ++j) {
                 uint32_t q = (j + 1) % m;
                 for (int c = 0; c < 2; ++c) {
                     float bx = c == 0 ? ax[j] : ax[q], by = c == 0 ? ay[j] : ay[q];
                     double d = two_pi * rng.uniform();
                     float len = R * 0.6f * rng.uniform();
-                    float x = bx + len * (float)std::cos(d), y = by + len * (float)std::sin(d);
-                    x = x < 0.0f ? 0.0f : (x > 4096.0f ? 4096.0f : x);
-                    y = y < 0.0f ? 0.0f : (y > 4096.0f ? 4096.0f : y);
-                    put(x, y);
+                    pts.push_back(bx + len * (float)std::cos(d));
+                    pts.push_back(by + len * (float)std::sin(d));
                 }
-                put(ax[q], ay[q]);
+                pts.push_back(ax[q]);
+                pts.push_back(ay[q]);
                 vb(LYG_VERB_CUBIC);
             }
+            // scale about the centre until 16 <= flattened edges <= 2048
+            for (int it = 0; it < 32; ++it) {
+                uint64_t f = 0;
+                for (uint32_t j = 0; j < m; ++j) {
+                    const float* p = &pts[6 * j];
+                    lyg::vec2 a = lyg::mk(p[0], p[1]), b = lyg::mk(p[2], p[3]);
+                    lyg::vec2 c = lyg::mk(p[4], p[5]), d = lyg::mk(p[6], p[7]);
+                    f += lyg::seg_count(lyg::cubic_wang(a, b, c, d, 0.05f));
+                }
+                float k;
+                if (f > 2048) k = 0.8f * (float)(2048.0 / (double)f) * (float)(2048.0 / (double)f);
+                else if (f < 16) k = 1.25f * (float)(16.0 / (double)f) * (float)(16.0 / (double)f);
+                else break;
+                for (size_t e = 0; e < pts.size(); e += 2) {
+                    pts[e] = cx + (pts[e] - cx) * k;
+                    pts[e + 1] = cy + (pts[e + 1] - cy) * k;
+                }
+            }
+            for (size_t e = 0; e < pts.size(); ++e) pts[e] = std::fmin(std::fmax(pts[e], 0.0f), 4096.0f);
         }
         // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
         vb(LYG_VERB_CLOSE);
-        if (points) put(points[2 * first], points[2 * first + 1]);
-        else np++;
+        pts.push_back(pts[0]);
+        pts.push_back(pts[1]);
+        if (points) std::memcpy(points + 2 * np, pts.data(), 4 * pts.size());
+        np += pts.size() / 2;
     }
     if (verb_offsets) verb_offsets[n_paths] = (uint32_t)nv;
     if (point_offsets) point_offsets[n_paths] = (uint32_t)np;
