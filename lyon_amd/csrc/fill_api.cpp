@@ -18,7 +18,8 @@
 #include "lyg_types.h"
 
 extern "C" {
-size_t lyg_fill_work_bytes(uint32_t ev_cap, uint32_t heap_cap, uint32_t act_cap);
+size_t lyg_fill_work_bytes(const lyg::fillk::FillCaps* caps, size_t hot_bytes);
+int lyg_fill_occupancy(uint32_t waves_per_block, uint32_t hot_bytes);
 hipError_t lyg_launch_fill_bound(const lyg::FillLaunch* L, uint32_t n, uint32_t* bound,
                                  hipStream_t s);
 hipError_t lyg_launch_fill_sweep(const lyg::FillLaunch* L, int grid, hipStream_t s);
@@ -36,18 +37,21 @@ constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO
 constexpr size_t WORK_BUDGET = 4ull << 30;           // per-launch working memory
 constexpr size_t STAGE_BUDGET = 64ull << 30;         // staging memory of one round
 constexpr int MAX_CLASSES = 8;
-constexpr int FILL_WAVES_PER_BLOCK = 4;
 
-struct Caps {
-    uint32_t ev, heap, act;
-};
+using lyg::fillk::FillCaps;
 
-// Working-memory capacities of one wave for paths of at most `bound` events at `scale`.
-Caps class_caps(uint64_t bound, uint64_t scale) {
-    Caps k;
-    k.ev = (uint32_t)bound;
-    k.heap = (uint32_t)std::min<uint64_t>(scale * (2 * bound + 256), 0x7fffffffu);
-    k.act = (uint32_t)std::min<uint64_t>(scale * (bound + 64), 0x7fffffffu);
+// Working-memory capacities of one wave for paths of at most `bound` events at `scale`:
+// active edges, spans and pending edges are bounded by the edges (the events), the heap
+// and the node pool grow with intersections (retry rounds scale them).
+FillCaps class_caps(uint64_t bound, uint64_t scale) {
+    auto cap = [](uint64_t v) { return (uint32_t)std::min<uint64_t>(v, 0x7fffffffu); };
+    FillCaps k;
+    k.ev = cap(bound);
+    k.heap = cap(scale * (2 * bound + 256));
+    k.act = cap(scale * (bound + 64));
+    k.span = k.act;
+    k.below = k.act;
+    k.node = cap(4ull * k.act + 256);
     return k;
 }
 
@@ -170,8 +174,11 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         // one launch per size class; the classes share the working memory, sized once
         struct Launch {
             size_t k0, k1, blocks, stride;
-            Caps cap;
+            FillCaps cap;
         };
+        const uint32_t wpb = c->fill_waves_per_block;
+        const uint32_t hot = c->fill_hot_bytes;
+        const int occ = std::max(1, lyg_fill_occupancy(wpb, hot));
         std::vector<Launch> launches;
         size_t work_bytes = 0;
         for (size_t k0 = 0; k0 < m;) {
@@ -186,10 +193,10 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             ln.k0 = k0;
             ln.k1 = k1;
             ln.cap = class_caps(std::max<uint64_t>(maxb, 1), scale);
-            ln.stride = lyg_fill_work_bytes(ln.cap.ev, ln.cap.heap, ln.cap.act);
-            const size_t per_block = ln.stride * FILL_WAVES_PER_BLOCK;
-            size_t blocks = (k1 - k0 + FILL_WAVES_PER_BLOCK - 1) / FILL_WAVES_PER_BLOCK;
-            blocks = std::min<size_t>(blocks, (size_t)c->cu_count * 8);
+            ln.stride = lyg_fill_work_bytes(&ln.cap, hot);
+            const size_t per_block = ln.stride * wpb;
+            size_t blocks = (k1 - k0 + wpb - 1) / wpb;
+            blocks = std::min<size_t>(blocks, (size_t)c->cu_count * occ);
             ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, WORK_BUDGET / per_block));
             work_bytes = std::max(work_bytes, ln.blocks * per_block);
             launches.push_back(ln);
@@ -201,9 +208,9 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             const Launch& ln = launches[li];
             L.work = as<uint8_t>(c->f_work);
             L.work_stride = ln.stride;
-            L.ev_cap = ln.cap.ev;
-            L.heap_cap = ln.cap.heap;
-            L.act_cap = ln.cap.act;
+            L.caps = ln.cap;
+            L.hot_bytes = hot;
+            L.waves_per_block = wpb;
             L.path_list = d_list + ln.k0;
             L.n_list = (uint32_t)(ln.k1 - ln.k0);
             L.stage_v_off = d_off + ln.k0;

@@ -34,13 +34,23 @@ struct StrokeLaunch {
     uint32_t* err;           // emit pass: set when a write falls outside its path's range
 };
 
+namespace fillk {
+// Capacities of one wave's fill working arrays (events, heap, active edges, spans,
+// monotone nodes, pending edges).
+struct FillCaps {
+    uint32_t ev, heap, act, span, node, below;
+};
+}  // namespace fillk
+
 struct FillLaunch {
     PathBatchDev in;
     const lyg_fill_options* opts;  // per path (opts_stride 1) or shared (stride 0)
     uint32_t opts_stride;
     uint8_t* work;                 // per-wave working memory (wave slot * work_stride)
     uint64_t work_stride;
-    uint32_t ev_cap, heap_cap, act_cap;
+    fillk::FillCaps caps;          // capacities of the per-wave arrays
+    uint32_t hot_bytes;            // per-wave dynamic LDS for the hot arrays (0: none)
+    uint32_t waves_per_block;
     float* stage_v;                // per-path staging, path i at stage_v_off[i] (vertices)
     uint32_t* stage_t;             // and stage_t_off[i] (triangles)
     const uint64_t* stage_v_off;
