@@ -1,0 +1,34 @@
+// Internal to liblyg.so: the batched entry points behind the C ABI, with the output
+// allocation left to the caller (the C ABI checks the caller's capacity; the serving
+// loop allocates from its device pool once the totals are known).
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "ctx.h"
+#include "lyg_types.h"
+
+namespace lygi {
+
+// Called once the batch's totals are known; sets the output arrays or returns a status.
+using OutAlloc = std::function<int32_t(uint64_t n_vertices, uint64_t n_indices, float** vertices,
+                                       uint32_t** indices)>;
+
+struct BatchSpec {
+    lyg::PathBatchDev in;
+    const void* opts;        // device: lyg_stroke_options or lyg_fill_options records
+    uint32_t opts_stride;    // 0: one record for every path
+    uint32_t vertex_offset;
+    bool local_ids;          // ids count from vertex_offset in every path
+    uint32_t* vtx_off;       // device, n + 1
+    uint32_t* idx_off;       // device, n + 1
+};
+
+// Batched stroke; *tv / *ti get the totals (also on a capacity error).
+int32_t stroke_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
+                     uint64_t* ti);
+// Batched fill; `status` gets every path's status (a failed path produces nothing).
+int32_t fill_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
+                   uint64_t* ti, std::vector<uint32_t>* status);
+
+}  // namespace lygi

@@ -30,6 +30,7 @@ hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t 
 }
 
 #include "ctx.h"
+#include "internal.h"
 
 namespace {
 
@@ -90,6 +91,46 @@ bool emit_failed(lyg_ctx* c) {
     }
     return false;
 }
+
+}  // namespace
+
+int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
+                           uint64_t* tv_out, uint64_t* ti_out) {
+    const uint32_t n = b.in.n_paths;
+    if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->counts_i, 4ull * n))
+        return LYG_ERR_DEVICE;
+    lyg::StrokeLaunch L{};
+    L.in = b.in;
+    L.opts = reinterpret_cast<const lyg_stroke_options*>(b.opts);
+    L.opts_stride = b.opts_stride;
+    L.path_nv = as<uint32_t>(c->counts_v);
+    L.path_ni = as<uint32_t>(c->counts_i);
+    L.vertex_offset = b.vertex_offset;
+    L.local_ids = b.local_ids ? 1u : 0u;
+    uint64_t tv = 0, ti = 0;
+    int32_t st = stroke_count_and_scan(c, L, b.vtx_off, b.idx_off, &tv, &ti);
+    *tv_out = tv;
+    *ti_out = ti;
+    if (st != LYG_OK) return st;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
+    if (!b.local_ids && tv > 0 && (uint64_t)b.vertex_offset + tv - 1 > 0xfffffffeull)
+        return LYG_ERR_TOO_MANY_VERTICES;
+    float* ov = nullptr;
+    uint32_t* oi = nullptr;
+    st = alloc(tv, ti, &ov, &oi);
+    if (st != LYG_OK) return st;
+    L.vtx_off = b.vtx_off;
+    L.idx_off = b.idx_off;
+    L.out_vertices = ov;
+    L.out_indices = oi;
+    st = stroke_emit(c, L);
+    if (st != LYG_OK) return st;
+    if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
+    if (emit_failed(c)) return LYG_ERR_INTERNAL;
+    return LYG_OK;
+}
+
+namespace {
 
 bool valid_stroke_opts(const lyg_stroke_options& o) {
     return o.start_cap <= 2 && o.end_cap <= 2 && o.line_join <= 3;
@@ -335,38 +376,36 @@ int32_t lyg_stroke_batch(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_op
     out->n_indices = 0;
     if (in->n_paths == 0) return LYG_OK;
     if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
-    if (!ensure(c, c->counts_v, 4ull * in->n_paths) || !ensure(c, c->counts_i, 4ull * in->n_paths))
-        return LYG_ERR_DEVICE;
-    lyg::StrokeLaunch L{};
-    L.in.verbs = in->verbs;
-    L.in.points = in->points;
-    L.in.verb_off = in->verb_offsets;
-    L.in.point_off = in->point_offsets;
-    L.in.n_paths = in->n_paths;
-    L.opts = opts;
-    L.opts_stride = n_opts == 1 ? 0 : 1;
-    L.path_nv = as<uint32_t>(c->counts_v);
-    L.path_ni = as<uint32_t>(c->counts_i);
-    L.vertex_offset = out->vertex_offset;
-    uint64_t tv, ti;
-    int32_t st = stroke_count_and_scan(c, L, out->path_vertex_offsets, out->path_index_offsets, &tv, &ti);
-    if (st != LYG_OK) return st;
-    if (tv > 0 && (uint64_t)out->vertex_offset + tv - 1 > 0xfffffffeull) return LYG_ERR_TOO_MANY_VERTICES;
+    lygi::BatchSpec b{};
+    b.in.verbs = in->verbs;
+    b.in.points = in->points;
+    b.in.verb_off = in->verb_offsets;
+    b.in.point_off = in->point_offsets;
+    b.in.n_paths = in->n_paths;
+    b.opts = opts;
+    b.opts_stride = n_opts == 1 ? 0 : 1;
+    b.vertex_offset = out->vertex_offset;
+    b.local_ids = false;
+    b.vtx_off = out->path_vertex_offsets;
+    b.idx_off = out->path_index_offsets;
+    uint64_t tv = 0, ti = 0;
+    int32_t st = lygi::stroke_batch(
+        c, b,
+        [&](uint64_t nv, uint64_t ni, float** v, uint32_t** i) -> int32_t {
+            if (nv > out->vertex_capacity || ni > out->index_capacity) return LYG_ERR_CAPACITY;
+            *v = out->vertices;
+            *i = out->indices;
+            return LYG_OK;
+        },
+        &tv, &ti);
     out->n_vertices = tv;
     out->n_indices = ti;
-    if (tv > out->vertex_capacity || ti > out->index_capacity) return LYG_ERR_CAPACITY;
-    L.vtx_off = out->path_vertex_offsets;
-    L.idx_off = out->path_index_offsets;
-    L.out_vertices = out->vertices;
-    L.out_indices = out->indices;
-    st = stroke_emit(c, L);
     if (st != LYG_OK) return st;
     if (out->path_status &&
         !check(c, lyg_launch_fill_u32(reinterpret_cast<uint32_t*>(out->path_status), in->n_paths, 0,
                                       c->stream), "status"))
         return LYG_ERR_DEVICE;
     if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
-    if (emit_failed(c)) return LYG_ERR_INTERNAL;
     return LYG_OK;
 }
 

@@ -30,6 +30,7 @@ struct StrokeLaunch {
     float* out_vertices;
     uint32_t* out_indices;
     uint32_t vertex_offset;  // BuffersBuilder::with_vertex_offset
+    uint32_t local_ids;      // ids count from vertex_offset in every path (not the batch)
     uint32_t* work_counter;  // device work queue head (persistent kernels)
     uint32_t* err;           // emit pass: set when a write falls outside its path's range
 };
@@ -69,6 +70,7 @@ struct FillLaunch {
     float* out_vertices;
     uint32_t* out_indices;
     uint32_t vertex_offset;
+    uint32_t local_ids;            // ids count from vertex_offset in every path
 };
 
 struct FlattenLaunch {

@@ -501,7 +501,9 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             const uint32_t nt_incl = incl_scan(nt);
             const uint32_t my_vbase = c.run_v + nv_incl - nv;  // path-relative
             const uint32_t my_tbase = c.run_i + nt_incl - nt;
-            const uint32_t real_base = EMIT ? (L.vertex_offset + path_v0 + my_vbase) : my_vbase;
+            // ids: vertex_offset + position in the batch, or in the path with local_ids
+            const uint32_t id_v0 = L.local_ids ? 0u : path_v0;
+            const uint32_t real_base = EMIT ? (L.vertex_offset + id_v0 + my_vbase) : my_vbase;
             if (is_join) pub.base = real_base;
             prv = gather_pub(pub, p1, c.pub);
             {
@@ -511,7 +513,8 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             ein.pm1_base = prv.base;
 
             if (EMIT) {
-                Sink se{real_base, 0, 0, L.out_vertices, L.vertex_offset, nullptr, true,
+                // vertex id -> slot: id - (vertex_offset + id_v0 - path_v0), modulo 2^32
+                Sink se{real_base, 0, 0, L.out_vertices, L.vertex_offset + id_v0 - path_v0, nullptr, true,
                         path_v_end, 0, L.err};
                 const uint64_t islot = (uint64_t)path_i0 + 3ull * my_tbase;
                 se.iout = L.out_indices + islot;
