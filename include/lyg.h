@@ -59,7 +59,8 @@ enum lyg_status {
     LYG_ERR_INVALID_ARGUMENT = 17,
     LYG_ERR_DEVICE = 18,            /* HIP runtime failure */
     LYG_ERR_UNSUPPORTED = 19,       /* option combination not implemented */
-    LYG_ERR_NO_SPACE = 20           /* serving pool cannot hold the request */
+    LYG_ERR_NO_SPACE = 20,          /* device memory (working memory or pool) exhausted */
+    LYG_PENDING = 21                /* serving loop: the request is not finished */
 };
 
 /* StrokeOptions (crates/tessellation/src/lib.rs:311-433); defaults: width 1, miter limit 4,
@@ -197,6 +198,66 @@ int32_t lyg_stroke_batch_count(lyg_ctx* ctx, const lyg_batch_in* in,
                                const lyg_stroke_options* opts, uint32_t n_opts,
                                uint32_t* path_nv, uint32_t* path_ni, uint64_t* total_nv,
                                uint64_t* total_ni);
+
+/* ---- serving loop (no lyon counterpart) -------------------------------------------------
+ * Continuous batching of single-path requests on one GPU.  Requests (host paths, copied at
+ * submit) are admitted into batches while their worst-case output fits the unreserved part
+ * of a device pool; up to max_inflight batches run at once, each on its own context and
+ * stream, so requests arriving while others run are batched as soon as a worker and pool
+ * room free up.  Results stay in the pool (device pointers, vertex ids local to the
+ * request, as tessellate_path into an empty VertexBuffers) until released.  Every call is
+ * thread-safe. */
+typedef struct lyg_server lyg_server;
+typedef struct lyg_server_config {
+    uint64_t pool_bytes;       /* device pool holding the results */
+    uint32_t max_batch_paths;  /* requests per batch */
+    uint32_t max_inflight;     /* batches on the device at once */
+    uint32_t linger_us;        /* wait this long for a short queue to grow (0: none) */
+    int32_t device;
+} lyg_server_config;
+typedef struct lyg_server_result {
+    int32_t status;            /* the request's lyg_status */
+    const float* vertices;     /* device pointer (x, y pairs); valid until released */
+    const uint32_t* indices;   /* device pointer */
+    uint32_t n_vertices, n_indices;
+    double latency_us;         /* submit to completion */
+} lyg_server_result;
+typedef struct lyg_server_stats {
+    uint64_t completed, triangles, vertices, batches, pending, pool_high_water;
+    double mean_batch;         /* requests per batch */
+    double p50_us, p99_us, max_us;  /* submit-to-completion latency over completed requests */
+    double elapsed_s;          /* first submit to last completion */
+} lyg_server_stats;
+lyg_server* lyg_server_create(const lyg_server_config* cfg);
+void lyg_server_destroy(lyg_server* server);
+int32_t lyg_server_submit_fill(lyg_server* server, const lyg_path_view* path,
+                               const lyg_fill_options* opts, uint64_t* ticket);
+int32_t lyg_server_submit_stroke(lyg_server* server, const lyg_path_view* path,
+                                 const lyg_stroke_options* opts, uint64_t* ticket);
+/* LYG_PENDING until the request is done, then LYG_OK with the result filled in. */
+int32_t lyg_server_poll(lyg_server* server, uint64_t ticket, lyg_server_result* out);
+int32_t lyg_server_wait(lyg_server* server, uint64_t ticket, lyg_server_result* out);
+/* Copies a finished result to host arrays (n_vertices pairs, n_indices ids). */
+int32_t lyg_server_copy_result(lyg_server* server, uint64_t ticket, float* vertices,
+                               uint32_t* indices);
+/* Frees the request and, with the last request of its batch, the batch's pool space. */
+int32_t lyg_server_release(lyg_server* server, uint64_t ticket);
+int32_t lyg_server_get_stats(lyg_server* server, lyg_server_stats* out);
+/* Serving measurement: n_requests with Poisson arrivals at rate_per_s, each a stroke with
+ * probability stroke_frac (one path of log-uniform [min_segs, max_segs] segments, 25% line
+ * / 25% quadratic / 50% cubic, points in [0, 4096]^2, the batched stroke workload's
+ * options) or else a fill (a path of the batched fill workload, EvenOdd/NonZero 50/50);
+ * tolerance for both.  Requests draw from n_distinct pre-generated paths of each kind;
+ * every result is released when done.  Latency percentiles cover the server's lifetime. */
+typedef struct lyg_stream_stats {
+    uint64_t requests, triangles, batches, pool_high_water;
+    double offered_rate, submit_rate, completed_rate;  /* requests/s */
+    double triangles_per_s, p50_us, p99_us, max_us, mean_batch, seconds;
+} lyg_stream_stats;
+int32_t lyg_server_run_stream(lyg_server* server, uint64_t seed, uint32_t n_requests,
+                              double rate_per_s, float stroke_frac, uint32_t min_segs,
+                              uint32_t max_segs, float tolerance, uint32_t n_distinct,
+                              lyg_stream_stats* out);
 
 /* ---- synthetic inputs ----------------------------------------------------------------
  * Seeded generators for the benchmark workloads (SplitMix64 -> PCG32).  Host memory.

@@ -54,6 +54,33 @@ class BatchOutC(C.Structure):
                 ("n_indices", C.c_uint64), ("path_status", C.c_void_p)]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("pool_bytes", C.c_uint64), ("max_batch_paths", C.c_uint32),
+                ("max_inflight", C.c_uint32), ("linger_us", C.c_uint32), ("device", C.c_int32)]
+
+
+class ServerResultC(C.Structure):
+    _fields_ = [("status", C.c_int32), ("vertices", C.c_void_p), ("indices", C.c_void_p),
+                ("n_vertices", C.c_uint32), ("n_indices", C.c_uint32),
+                ("latency_us", C.c_double)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("triangles", C.c_uint64), ("vertices", C.c_uint64),
+                ("batches", C.c_uint64), ("pending", C.c_uint64),
+                ("pool_high_water", C.c_uint64), ("mean_batch", C.c_double),
+                ("p50_us", C.c_double), ("p99_us", C.c_double), ("max_us", C.c_double),
+                ("elapsed_s", C.c_double)]
+
+
+class StreamStatsC(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("triangles", C.c_uint64), ("batches", C.c_uint64),
+                ("pool_high_water", C.c_uint64), ("offered_rate", C.c_double),
+                ("submit_rate", C.c_double), ("completed_rate", C.c_double),
+                ("triangles_per_s", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double),
+                ("max_us", C.c_double), ("mean_batch", C.c_double), ("seconds", C.c_double)]
+
+
 # Every symbol include/lyg.h declares, with its ctypes signature.
 SIGNATURES = {
     "lyg_ctx_create": ([C.c_int], C.c_void_p),
@@ -85,6 +112,20 @@ SIGNATURES = {
                                 C.c_void_p, C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_gen_stroke_batch": ([C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
                               U8P, F32P, U32P, U32P, U64P, U64P], C.c_int32),
+    "lyg_server_create": ([C.POINTER(ServerConfigC)], C.c_void_p),
+    "lyg_server_destroy": ([C.c_void_p], None),
+    "lyg_server_submit_fill": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),
+                                C.POINTER(C.c_uint64)], C.c_int32),
+    "lyg_server_submit_stroke": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
+                                  C.POINTER(C.c_uint64)], C.c_int32),
+    "lyg_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(ServerResultC)], C.c_int32),
+    "lyg_server_wait": ([C.c_void_p, C.c_uint64, C.POINTER(ServerResultC)], C.c_int32),
+    "lyg_server_copy_result": ([C.c_void_p, C.c_uint64, F32P, U32P], C.c_int32),
+    "lyg_server_release": ([C.c_void_p, C.c_uint64], C.c_int32),
+    "lyg_server_get_stats": ([C.c_void_p, C.POINTER(ServerStatsC)], C.c_int32),
+    "lyg_server_run_stream": ([C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_float,
+                               C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                               C.POINTER(StreamStatsC)], C.c_int32),
     "lyg_gen_fill_batch": ([C.c_uint64, C.c_uint32, U8P, F32P, U32P, U32P, U64P, U64P],
                            C.c_int32),
     "lyg_gen_fill_options": ([C.c_uint64, C.c_uint32, C.c_float, C.POINTER(FillOptionsC)],

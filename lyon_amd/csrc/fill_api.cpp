@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "internal.h"
 #include "lyg_types.h"
 
 extern "C" {
@@ -81,6 +82,7 @@ struct FillJob {
     const lyg_fill_options* opts;
     uint32_t opts_stride;
     uint32_t vertex_offset;
+    bool local_ids = false;
 };
 
 struct FillResult {
@@ -249,7 +251,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     R->total_i = h[1];
     if ((uint32_t)(h[2] & 0xffffffffu)) return LYG_ERR_TOO_MANY_VERTICES;
     // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
-    if (R->total_v > 0 && (uint64_t)J.vertex_offset + R->total_v - 1 > 0xfffffffeull)
+    if (!J.local_ids && R->total_v > 0 && (uint64_t)J.vertex_offset + R->total_v - 1 > 0xfffffffeull)
         return LYG_ERR_TOO_MANY_VERTICES;
     return LYG_OK;
 }
@@ -269,6 +271,7 @@ int32_t fill_copy(lyg_ctx* c, const FillJob& J, const FillResult& R, const uint3
     L.out_vertices = out_v;
     L.out_indices = out_i;
     L.vertex_offset = J.vertex_offset;
+    L.local_ids = J.local_ids ? 1u : 0u;
     (void)n;
     if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
     for (int r = 0; r < R.rounds; ++r) {
@@ -292,6 +295,31 @@ int32_t first_error(const FillResult& R) {
 }
 
 }  // namespace
+
+int32_t lygi::fill_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
+                         uint64_t* tv, uint64_t* ti, std::vector<uint32_t>* status) {
+    FillJob J;
+    J.in = b.in;
+    J.in.first_path = 0;
+    J.opts = reinterpret_cast<const lyg_fill_options*>(b.opts);
+    J.opts_stride = b.opts_stride;
+    J.vertex_offset = b.vertex_offset;
+    J.local_ids = b.local_ids;
+    FillResult R;
+    int32_t st = fill_sweep(c, J, b.vtx_off, b.idx_off, &R);
+    *tv = R.total_v;
+    *ti = R.total_i;
+    if (status) *status = R.status;
+    if (st != LYG_OK) return st;
+    float* ov = nullptr;
+    uint32_t* oi = nullptr;
+    st = alloc(R.total_v, R.total_i, &ov, &oi);
+    if (st != LYG_OK) return st;
+    st = fill_copy(c, J, R, b.vtx_off, b.idx_off, ov, oi);
+    if (st != LYG_OK) return st;
+    if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
 
 extern "C" {
 
@@ -370,33 +398,40 @@ int32_t lyg_fill_batch(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_option
     out->n_indices = 0;
     if (in->n_paths == 0) return LYG_OK;
     if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
-    FillJob J;
-    J.in.verbs = in->verbs;
-    J.in.points = in->points;
-    J.in.verb_off = in->verb_offsets;
-    J.in.point_off = in->point_offsets;
-    J.in.n_paths = in->n_paths;
-    J.in.first_path = 0;
-    J.opts = opts;
-    J.opts_stride = n_opts == 1 ? 0 : 1;
-    J.vertex_offset = out->vertex_offset;
-    FillResult R;
-    int32_t st = fill_sweep(c, J, out->path_vertex_offsets, out->path_index_offsets, &R);
-    if (st != LYG_OK) return st;
-    out->n_vertices = R.total_v;
-    out->n_indices = R.total_i;
-    if (out->path_status) {
-        if (!check(c, hipMemcpyAsync(out->path_status, R.status.data(), 4ull * in->n_paths,
-                                     hipMemcpyHostToDevice, c->stream), "H2D") ||
-            !check(c, hipStreamSynchronize(c->stream), "sync"))
-            return LYG_ERR_DEVICE;
-    }
-    if (R.total_v > out->vertex_capacity || R.total_i > out->index_capacity) return LYG_ERR_CAPACITY;
-    st = fill_copy(c, J, R, out->path_vertex_offsets, out->path_index_offsets, out->vertices,
-                   out->indices);
-    if (st != LYG_OK) return st;
+    lygi::BatchSpec b{};
+    b.in.verbs = in->verbs;
+    b.in.points = in->points;
+    b.in.verb_off = in->verb_offsets;
+    b.in.point_off = in->point_offsets;
+    b.in.n_paths = in->n_paths;
+    b.opts = opts;
+    b.opts_stride = n_opts == 1 ? 0 : 1;
+    b.vertex_offset = out->vertex_offset;
+    b.local_ids = false;
+    b.vtx_off = out->path_vertex_offsets;
+    b.idx_off = out->path_index_offsets;
+    uint64_t tv = 0, ti = 0;
+    std::vector<uint32_t> status;
+    int32_t st = lygi::fill_batch(
+        c, b,
+        [&](uint64_t nv, uint64_t ni, float** v, uint32_t** i) -> int32_t {
+            if (out->path_status &&
+                !check(c, hipMemcpyAsync(out->path_status, status.data(), 4ull * in->n_paths,
+                                         hipMemcpyHostToDevice, c->stream), "H2D"))
+                return LYG_ERR_DEVICE;
+            if (nv > out->vertex_capacity || ni > out->index_capacity) return LYG_ERR_CAPACITY;
+            *v = out->vertices;
+            *i = out->indices;
+            return LYG_OK;
+        },
+        &tv, &ti, &status);
+    out->n_vertices = tv;
+    out->n_indices = ti;
     if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
-    return first_error(R);
+    if (st != LYG_OK) return st;
+    for (uint32_t x : status)
+        if (x != 0) return (int32_t)x;
+    return LYG_OK;
 }
 
 }  // extern "C"

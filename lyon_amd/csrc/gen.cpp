@@ -44,6 +44,127 @@ uint64_t path_seed(uint64_t seed, uint64_t i) { return splitmix64(seed ^ splitmi
 
 }  // namespace
 
+// One path of each workload, appended to `verbs` / `points` (x, y pairs).
+namespace lygen {
+
+void stroke_path(uint64_t seed, uint64_t i, uint32_t min_segs, uint32_t max_segs,
+                 float coord_max, bool closed, std::vector<uint8_t>& verbs,
+                 std::vector<float>& points) {
+    Pcg32 rng(path_seed(seed, i));
+    const double lmin = std::log((double)min_segs), lmax = std::log((double)max_segs);
+    double u = rng.uniform();
+    uint32_t segs = (uint32_t)std::round(std::exp(lmin + u * (lmax - lmin)));
+    segs = segs < min_segs ? min_segs : (segs > max_segs ? max_segs : segs);
+    const size_t first = points.size();
+    auto pt = [&]() {
+        float x = rng.uniform() * coord_max;
+        float y = rng.uniform() * coord_max;
+        points.push_back(x);
+        points.push_back(y);
+    };
+    verbs.push_back(LYG_VERB_BEGIN);
+    pt();
+    for (uint32_t s = 0; s < segs; ++s) {
+        float k = rng.uniform();
+        if (k < 0.25f) {
+            verbs.push_back(LYG_VERB_LINE);
+            pt();
+        } else if (k < 0.5f) {
+            verbs.push_back(LYG_VERB_QUADRATIC);
+            pt();
+            pt();
+        } else {
+            verbs.push_back(LYG_VERB_CUBIC);
+            pt();
+            pt();
+            pt();
+        }
+    }
+    if (closed) {
+        // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
+        verbs.push_back(LYG_VERB_CLOSE);
+        float fx = points[first], fy = points[first + 1];
+        points.push_back(fx);
+        points.push_back(fy);
+    } else {
+        verbs.push_back(LYG_VERB_END);
+    }
+}
+
+void fill_path(uint64_t seed, uint64_t i, std::vector<uint8_t>& verbs,
+               std::vector<float>& pts) {
+    const double two_pi = 6.283185307179586;
+    Pcg32 rng(path_seed(seed ^ 0x5eedf111ull, i));
+    auto vb = [&](uint8_t v) { verbs.push_back(v); };
+    const float R = 16.0f + rng.uniform() * (512.0f - 16.0f);
+    const float cx = R + rng.uniform() * (4096.0f - 2.0f * R);
+    const float cy = R + rng.uniform() * (4096.0f - 2.0f * R);
+    const size_t first = pts.size();
+    if ((i & 1u) == 0) {
+        uint32_t n = 5u + rng.next() % 508u;
+        vb(LYG_VERB_BEGIN);
+        for (uint32_t j = 0; j < n; ++j) {
+            double a = two_pi * rng.uniform();
+            float r = R * (0.2f + 0.8f * rng.uniform());
+            pts.push_back(cx + r * (float)std::cos(a));
+            pts.push_back(cy + r * (float)std::sin(a));
+            if (j > 0) vb(LYG_VERB_LINE);
+        }
+    } else {
+        uint32_t m = 4u + rng.next() % 253u;
+        vb(LYG_VERB_BEGIN);
+        std::vector<float> ax(m), ay(m);
+        for (uint32_t j = 0; j < m; ++j) {
+            double a = two_pi * (double)j / (double)m;
+            float r = R * (0.5f + 0.5f * rng.uniform());
+            ax[j] = cx + r * (float)std::cos(a);
+            ay[j] = cy + r * (float)std::sin(a);
+        }
+        pts.push_back(ax[0]);
+        pts.push_back(ay[0]);
+        for (uint32_t j = 0; j < m; ++j) {
+            uint32_t q = (j + 1) % m;
+            for (int c = 0; c < 2; ++c) {
+                float bx = c == 0 ? ax[j] : ax[q], by = c == 0 ? ay[j] : ay[q];
+                double d = two_pi * rng.uniform();
+                float len = R * 0.6f * rng.uniform();
+                pts.push_back(bx + len * (float)std::cos(d));
+                pts.push_back(by + len * (float)std::sin(d));
+            }
+            pts.push_back(ax[q]);
+            pts.push_back(ay[q]);
+            vb(LYG_VERB_CUBIC);
+        }
+        // scale about the centre until 16 <= flattened edges <= 2048
+        for (int it = 0; it < 32; ++it) {
+            uint64_t f = 0;
+            for (uint32_t j = 0; j < m; ++j) {
+                const float* p = &pts[first + 6 * j];
+                lyg::vec2 a = lyg::mk(p[0], p[1]), b = lyg::mk(p[2], p[3]);
+                lyg::vec2 c = lyg::mk(p[4], p[5]), d = lyg::mk(p[6], p[7]);
+                f += lyg::seg_count(lyg::cubic_wang(a, b, c, d, 0.05f));
+            }
+            float k;
+            if (f > 2048) k = 0.8f * (float)(2048.0 / (double)f) * (float)(2048.0 / (double)f);
+            else if (f < 16) k = 1.25f * (float)(16.0 / (double)f) * (float)(16.0 / (double)f);
+            else break;
+            for (size_t e = first; e < pts.size(); e += 2) {
+                pts[e] = cx + (pts[e] - cx) * k;
+                pts[e + 1] = cy + (pts[e + 1] - cy) * k;
+            }
+        }
+        for (size_t e = first; e < pts.size(); ++e)
+            pts[e] = std::fmin(std::fmax(pts[e], 0.0f), 4096.0f);
+    }
+    // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
+    vb(LYG_VERB_CLOSE);
+    float fx = pts[first], fy = pts[first + 1];
+    pts.push_back(fx);
+    pts.push_back(fy);
+}
+
+}  // namespace lygen
+
 extern "C" {
 
 // Open (or closed) paths with a log-uniform segment count in [min_segs, max_segs],
@@ -55,59 +176,18 @@ int32_t lyg_gen_stroke_batch(uint64_t seed, uint32_t n_paths, uint32_t min_segs,
                              uint64_t* n_verbs, uint64_t* n_points) {
     if (min_segs < 1 || max_segs < min_segs) return LYG_ERR_INVALID_ARGUMENT;
     uint64_t nv = 0, np = 0;
-    const double lmin = std::log((double)min_segs), lmax = std::log((double)max_segs);
+    std::vector<uint8_t> v;
+    std::vector<float> p;
     for (uint32_t i = 0; i < n_paths; ++i) {
-        Pcg32 rng(path_seed(seed, i));
-        double u = rng.uniform();
-        double segs_d = std::round(std::exp(lmin + u * (lmax - lmin)));
-        uint32_t segs = (uint32_t)segs_d;
-        if (segs < min_segs) segs = min_segs;
-        if (segs > max_segs) segs = max_segs;
+        v.clear();
+        p.clear();
+        lygen::stroke_path(seed, i, min_segs, max_segs, coord_max, closed != 0, v, p);
         if (verb_offsets) verb_offsets[i] = (uint32_t)nv;
         if (point_offsets) point_offsets[i] = (uint32_t)np;
-        auto pt = [&]() {
-            float x = rng.uniform() * coord_max;
-            float y = rng.uniform() * coord_max;
-            if (points) {
-                points[2 * np] = x;
-                points[2 * np + 1] = y;
-            }
-            np++;
-        };
-        auto vb = [&](uint8_t v) {
-            if (verbs) verbs[nv] = v;
-            nv++;
-        };
-        vb(LYG_VERB_BEGIN);
-        pt();
-        for (uint32_t s = 0; s < segs; ++s) {
-            float k = rng.uniform();
-            if (k < 0.25f) {
-                vb(LYG_VERB_LINE);
-                pt();
-            } else if (k < 0.5f) {
-                vb(LYG_VERB_QUADRATIC);
-                pt();
-                pt();
-            } else {
-                vb(LYG_VERB_CUBIC);
-                pt();
-                pt();
-                pt();
-            }
-        }
-        if (closed) {
-            // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
-            vb(LYG_VERB_CLOSE);
-            if (points) {
-                uint32_t first = point_offsets ? point_offsets[i] : 0;
-                points[2 * np] = points[2 * first];
-                points[2 * np + 1] = points[2 * first + 1];
-            }
-            np++;
-        } else {
-            vb(LYG_VERB_END);
-        }
+        if (verbs) std::memcpy(verbs + nv, v.data(), v.size());
+        if (points) std::memcpy(points + 2 * np, p.data(), 4 * p.size());
+        nv += v.size();
+        np += p.size() / 2;
     }
     if (verb_offsets) verb_offsets[n_paths] = (uint32_t)nv;
     if (point_offsets) point_offsets[n_paths] = (uint32_t)np;
@@ -152,81 +232,18 @@ int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, floa
                            uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
                            uint64_t* n_points) {
     uint64_t nv = 0, np = 0;
-    const double two_pi = 6.283185307179586;
-    std::vector<float> pts;
+    std::vector<uint8_t> v;
+    std::vector<float> p;
     for (uint32_t i = 0; i < n_paths; ++i) {
-        Pcg32 rng(path_seed(seed ^ 0x5eedf111ull, i));
+        v.clear();
+        p.clear();
+        lygen::fill_path(seed, i, v, p);
         if (verb_offsets) verb_offsets[i] = (uint32_t)nv;
         if (point_offsets) point_offsets[i] = (uint32_t)np;
-        auto vb = [&](uint8_t v) {
-            if (verbs) verbs[nv] = v;
-            nv++;
-        };
-        const float R = 16.0f + rng.uniform() * (512.0f - 16.0f);
-        const float cx = R + rng.uniform() * (4096.0f - 2.0f * R);
-        const float cy = R + rng.uniform() * (4096.0f - 2.0f * R);
-        pts.clear();
-        if ((i & 1u) == 0) {
-            uint32_t n = 5u + rng.next() % 508u;
-            vb(LYG_VERB_BEGIN);
-            for (uint32_t j = 0; j < n; ++j) {
-                double a = two_pi * rng.uniform();
-                float r = R * (0.2f + 0.8f * rng.uniform());
-                pts.push_back(cx + r * (float)std::cos(a));
-                pts.push_back(cy + r * (float)std::sin(a));
-                if (j > 0) vb(LYG_VERB_LINE);
-            }
-        } else {
-            uint32_t m = 4u + rng.next() % 253u;
-            vb(LYG_VERB_BEGIN);
-            std::vector<float> ax(m), ay(m);
-            for (uint32_t j = 0; j < m; ++j) {
-                double a = two_pi * (double)j / (double)m;
-                float r = R * (0.5f + 0.5f * rng.uniform());
-                ax[j] = cx + r * (float)std::cos(a);
-                ay[j] = cy + r * (float)std::sin(a);
-            }
-            pts.push_back(ax[0]);
-            pts.push_back(ay[0]);
-            for (uint32_t j = 0; j < m; ++j) {
-                uint32_t q = (j + 1) % m;
-                for (int c = 0; c < 2; ++c) {
-                    float bx = c == 0 ? ax[j] : ax[q], by = c == 0 ? ay[j] : ay[q];
-                    double d = two_pi * rng.uniform();
-                    float len = R * 0.6f * rng.uniform();
-                    pts.push_back(bx + len * (float)std::cos(d));
-                    pts.push_back(by + len * (float)std::sin(d));
-                }
-                pts.push_back(ax[q]);
-                pts.push_back(ay[q]);
-                vb(LYG_VERB_CUBIC);
-            }
-            // scale about the centre until 16 <= flattened edges <= 2048
-            for (int it = 0; it < 32; ++it) {
-                uint64_t f = 0;
-                for (uint32_t j = 0; j < m; ++j) {
-                    const float* p = &pts[6 * j];
-                    lyg::vec2 a = lyg::mk(p[0], p[1]), b = lyg::mk(p[2], p[3]);
-                    lyg::vec2 c = lyg::mk(p[4], p[5]), d = lyg::mk(p[6], p[7]);
-                    f += lyg::seg_count(lyg::cubic_wang(a, b, c, d, 0.05f));
-                }
-                float k;
-                if (f > 2048) k = 0.8f * (float)(2048.0 / (double)f) * (float)(2048.0 / (double)f);
-                else if (f < 16) k = 1.25f * (float)(16.0 / (double)f) * (float)(16.0 / (double)f);
-                else break;
-                for (size_t e = 0; e < pts.size(); e += 2) {
-                    pts[e] = cx + (pts[e] - cx) * k;
-                    pts[e + 1] = cy + (pts[e + 1] - cy) * k;
-                }
-            }
-            for (size_t e = 0; e < pts.size(); ++e) pts[e] = std::fmin(std::fmax(pts[e], 0.0f), 4096.0f);
-        }
-        // BuilderImpl::end(true) stores a copy of the first point (path.rs:537-542).
-        vb(LYG_VERB_CLOSE);
-        pts.push_back(pts[0]);
-        pts.push_back(pts[1]);
-        if (points) std::memcpy(points + 2 * np, pts.data(), 4 * pts.size());
-        np += pts.size() / 2;
+        if (verbs) std::memcpy(verbs + nv, v.data(), v.size());
+        if (points) std::memcpy(points + 2 * np, p.data(), 4 * p.size());
+        nv += v.size();
+        np += p.size() / 2;
     }
     if (verb_offsets) verb_offsets[n_paths] = (uint32_t)nv;
     if (point_offsets) point_offsets[n_paths] = (uint32_t)np;
