@@ -199,6 +199,34 @@ int32_t lyg_stroke_batch_count(lyg_ctx* ctx, const lyg_batch_in* in,
                                uint32_t* path_nv, uint32_t* path_ni, uint64_t* total_nv,
                                uint64_t* total_ni);
 
+/* ---- HIP graph form of the batched stroke (no lyon counterpart) -------------------------
+ * Captures one lyg_stroke_batch (counter reset, count kernel, offset scans, emit kernel)
+ * bound to these device buffers; lyg_graph_launch replays it with no host synchronisation
+ * inside (the emit kernel skips a path that does not fit the output capacity and reports
+ * it); lyg_graph_result waits and returns the totals and status. */
+typedef struct lyg_graph lyg_graph;
+int32_t lyg_stroke_batch_graph(lyg_ctx* ctx, const lyg_batch_in* in,
+                               const lyg_stroke_options* opts, uint32_t n_opts,
+                               const lyg_batch_out* out, lyg_graph** graph);
+int32_t lyg_graph_launch(lyg_graph* graph);
+int32_t lyg_graph_result(lyg_graph* graph, uint64_t* n_vertices, uint64_t* n_indices);
+void lyg_graph_destroy(lyg_graph* graph);
+
+/* ---- autotuner (no lyon counterpart) -----------------------------------------------------
+ * Runs the batch under each candidate launch shape (stroke: resident blocks per CU of the
+ * persistent grid; fill: waves per block and LDS bytes per wave for the sweep's hot
+ * arrays), keeps the fastest in the context for later calls and reports it. */
+typedef struct lyg_tuning {
+    uint32_t blocks_per_cu;
+    uint32_t waves_per_block;
+    uint32_t lds_bytes_per_wave;
+    float best_ms;
+} lyg_tuning;
+int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
+                            uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
+int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
+                          uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
+
 /* ---- serving loop (no lyon counterpart) -------------------------------------------------
  * Continuous batching of single-path requests on one GPU.  Requests (host paths, copied at
  * submit) are admitted into batches while their worst-case output fits the unreserved part

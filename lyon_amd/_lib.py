@@ -81,6 +81,11 @@ class StreamStatsC(C.Structure):
                 ("max_us", C.c_double), ("mean_batch", C.c_double), ("seconds", C.c_double)]
 
 
+class TuningC(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("waves_per_block", C.c_uint32),
+                ("lds_bytes_per_wave", C.c_uint32), ("best_ms", C.c_float)]
+
+
 # Every symbol include/lyg.h declares, with its ctypes signature.
 SIGNATURES = {
     "lyg_ctx_create": ([C.c_int], C.c_void_p),
@@ -112,6 +117,15 @@ SIGNATURES = {
                                 C.c_void_p, C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_gen_stroke_batch": ([C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
                               U8P, F32P, U32P, U32P, U64P, U64P], C.c_int32),
+    "lyg_stroke_batch_graph": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                                C.POINTER(BatchOutC), C.POINTER(C.c_void_p)], C.c_int32),
+    "lyg_graph_launch": ([C.c_void_p], C.c_int32),
+    "lyg_graph_result": ([C.c_void_p, U64P, U64P], C.c_int32),
+    "lyg_graph_destroy": ([C.c_void_p], None),
+    "lyg_autotune_stroke": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                             C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
+    "lyg_autotune_fill": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                           C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
     "lyg_server_create": ([C.POINTER(ServerConfigC)], C.c_void_p),
     "lyg_server_destroy": ([C.c_void_p], None),
     "lyg_server_submit_fill": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),

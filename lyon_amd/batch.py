@@ -200,3 +200,67 @@ class FillBatcher(_Batcher):
     """Batched fill; with check=False a path's failure is reported in ``status`` only."""
 
     _entry = "lyg_fill_batch"
+
+
+class StrokeGraph:
+    """One batched stroke captured into a HIP graph (lyg_stroke_batch_graph) on fixed
+    device buffers: ``launch()`` replays count -> scan -> emit with no host sync inside;
+    ``result()`` waits and returns a BatchResult (raises on an output overflow)."""
+
+    def __init__(self, db: DeviceBatch, vertex_capacity: int, index_capacity: int,
+                 vertex_offset: int = 0):
+        self.db = db
+        self.vertices = DeviceBuffer(8 * max(vertex_capacity, 1))
+        self.indices = DeviceBuffer(4 * max(index_capacity, 1))
+        self.voffs = DeviceBuffer(4 * (db.n + 1))
+        self.ioffs = DeviceBuffer(4 * (db.n + 1))
+        self.status = DeviceBuffer(4 * max(db.n, 1))
+        out = B.BatchOutC(self.vertices.ptr, self.indices.ptr, vertex_capacity, index_capacity,
+                          self.voffs.ptr, self.ioffs.ptr, vertex_offset, 0, 0, None)
+        cin = db.c_in()
+        h = C.c_void_p(0)
+        st = B.lib().lyg_stroke_batch_graph(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts,
+                                            C.byref(out), C.byref(h))
+        if st != 0:
+            raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+        self._h = h.value
+
+    def launch(self):
+        st = B.lib().lyg_graph_launch(self._h)
+        if st != 0:
+            raise TessellationError(st)
+
+    def result(self) -> BatchResult:
+        nv = C.c_uint64(0)
+        ni = C.c_uint64(0)
+        st = B.lib().lyg_graph_result(self._h, C.byref(nv), C.byref(ni))
+        if st != 0:
+            raise TessellationError(st)
+        return BatchResult(self.vertices, self.indices, self.voffs, self.ioffs, self.status,
+                           self.db.n, nv.value, ni.value)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                B.lib().lyg_graph_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def autotune(batcher: _Batcher, db: DeviceBatch) -> dict:
+    """Times db under each candidate launch shape (lyg_autotune_stroke / _fill) and keeps
+    the fastest in the context; returns the chosen shape."""
+    res = batcher.run(db, check=False)  # sizes the output arena
+    out = B.BatchOutC(batcher.vertices.ptr, batcher.indices.ptr, batcher.vertices.nbytes // 8,
+                      batcher.indices.nbytes // 4, batcher.voffs.ptr, batcher.ioffs.ptr, 0, 0, 0,
+                      batcher.status.ptr)
+    cin = db.c_in()
+    t = B.TuningC()
+    fn = B.lib().lyg_autotune_stroke if isinstance(batcher, StrokeBatcher) else \
+        B.lib().lyg_autotune_fill
+    st = fn(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts, C.byref(out), C.byref(t))
+    if st != 0:
+        raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+    del res
+    return {k: getattr(t, k) for k, _ in t._fields_}

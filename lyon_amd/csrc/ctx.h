@@ -29,6 +29,8 @@ struct lyg_ctx {
     // optional kernel timing: events around the count and emit kernels on `stream`
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // stroke launch shape (set by the autotuner): resident blocks per CU of the grid
+    uint32_t stroke_blocks_per_cu = 8;
     // fill launch shape (set by the autotuner): waves per block, LDS bytes per wave
     uint32_t fill_waves_per_block = 2;
     uint32_t fill_hot_bytes = 24 * 1024;
@@ -83,8 +85,9 @@ T* as(DevBuf& b) {
 }
 
 inline int persistent_grid(lyg_ctx* c, uint32_t n_paths) {
-    // 4 waves per block, one path per wave at a time; 8 blocks per CU keep 32 waves/CU.
+    // 4 waves per block, one path per wave at a time; stroke_blocks_per_cu blocks per CU
+    // (the autotuner's choice; 8 keeps 32 waves per CU).
     uint32_t blocks_needed = (n_paths + 3) / 4;
-    uint32_t cap = (uint32_t)c->cu_count * 8;
+    uint32_t cap = (uint32_t)c->cu_count * c->stroke_blocks_per_cu;
     return (int)std::max(1u, std::min(blocks_needed, cap));
 }

@@ -86,7 +86,8 @@ int32_t stroke_emit(lyg_ctx* c, lyg::StrokeLaunch& L) {
 bool emit_failed(lyg_ctx* c) {
     const uint32_t* h = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(c->h_pinned) + 128);
     if (*h) {
-        c->last_error = "emit pass wrote outside its path range (count/emit mismatch)";
+        c->last_error = (*h & 2u) ? "a path did not fit the output arrays"
+                                  : "emit pass wrote outside its path range (count/emit mismatch)";
         return true;
     }
     return false;
@@ -123,6 +124,8 @@ int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     L.idx_off = b.idx_off;
     L.out_vertices = ov;
     L.out_indices = oi;
+    L.vertex_capacity = tv;
+    L.index_capacity = ti;
     st = stroke_emit(c, L);
     if (st != LYG_OK) return st;
     if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
@@ -325,6 +328,8 @@ int32_t lyg_stroke_path(lyg_ctx* c, const lyg_path_view* path, const lyg_stroke_
     L.idx_off = as<uint32_t>(c->offs_i);
     L.out_vertices = as<float>(c->out_v);
     L.out_indices = as<uint32_t>(c->out_i);
+    L.vertex_capacity = tv;
+    L.index_capacity = ti;
     st = stroke_emit(c, L);
     if (st != LYG_OK) return st;
     if (!check(c, hipMemcpyAsync(out->vertices, c->out_v.p, 8 * tv, hipMemcpyDeviceToHost, s), "D2H") ||

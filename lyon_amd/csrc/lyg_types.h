@@ -32,7 +32,10 @@ struct StrokeLaunch {
     uint32_t vertex_offset;  // BuffersBuilder::with_vertex_offset
     uint32_t local_ids;      // ids count from vertex_offset in every path (not the batch)
     uint32_t* work_counter;  // device work queue head (persistent kernels)
-    uint32_t* err;           // emit pass: set when a write falls outside its path's range
+    uint32_t* err;           // emit pass: bit 0 a write outside its path's range, bit 1 a
+                             // path beyond the output capacity (the path is skipped)
+    uint64_t vertex_capacity;  // emit pass: output array sizes
+    uint64_t index_capacity;
 };
 
 namespace fillk {

@@ -227,6 +227,11 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
         path_i0 = L.idx_off[gpath];
         path_v_end = L.vtx_off[gpath + 1];
         path_i_end = L.idx_off[gpath + 1];
+        // a path that does not fit the output arrays is skipped and reported
+        if (path_v_end > L.vertex_capacity || path_i_end > L.index_capacity) {
+            if (lane_id() == 0 && L.err) atomicOr(L.err, 2u);
+            return;
+        }
     }
 
     Carry c;
