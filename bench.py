@@ -70,8 +70,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fill", action="store_true")
-    ap.add_argument("--traffic-json", default=None,
-                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary for the emit kernel")
+    ap.add_argument("--traffic-json",
+                    default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
+                    help="HBM traffic per emit launch measured with rocprofv3 counters")
+    ap.add_argument("--profile", action="store_true",
+                    help="only run the batched stroke steps and a 1 GiB device copy (the "
+                         "reference kernel for the traffic counters), for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharding of this "
@@ -89,6 +93,22 @@ def main():
     L = B.lib()
     ctx = _Ctx.get()
     L.lyg_ctx_enable_timing(ctx, 1)
+
+    if args.profile:
+        batch = workloads.stroke_batch(args.paths, 8, 256, 2048.0, seed=0)
+        opts = workloads.stroke_options(args.paths, tolerance=0.05, w_min=0.5, w_max=8.0)
+        db = DeviceBatch(batch, opts)
+        batcher = StrokeBatcher()
+        from lyon_amd.batch import DeviceBuffer
+        src = DeviceBuffer(1 << 30)
+        dst = DeviceBuffer(1 << 30)
+        for k in range(args.steps):
+            batcher.run(db)
+            L.lyg_device_copy(ctx, dst.ptr, src.ptr, 1 << 30)
+            synchronize()
+            log(f"profile step {k} done")
+        print(json.dumps({"profile": "stroke batch + 1 GiB copy", "steps": args.steps}))
+        return
 
     # ---- batched stroke (headline) ----------------------------------------------------
     log("generating stroke workload")

@@ -129,6 +129,9 @@ int32_t lyg_device_alloc(lyg_ctx* ctx, uint64_t bytes, void** ptr);
 int32_t lyg_device_free(lyg_ctx* ctx, void* ptr);
 int32_t lyg_copy_to_device(lyg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int32_t lyg_copy_to_host(lyg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy (16-byte aligned pointers) on the context's stream, asynchronous;
+ * a plain streaming kernel (also the reference kernel for measuring HBM traffic). */
+int32_t lyg_device_copy(lyg_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 /* Waits for all work queued on the context's stream. */
 int32_t lyg_ctx_synchronize(lyg_ctx* ctx);
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "lyg_copy_to_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
     "lyg_copy_to_host": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
     "lyg_ctx_synchronize": ([C.c_void_p], C.c_int32),
+    "lyg_device_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64], C.c_int32),
     "lyg_stroke_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
                          C.POINTER(OutputC)], C.c_int32),
     "lyg_fill_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),

@@ -220,9 +220,31 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Device-to-device copy, 16 bytes per thread per step (grid-stride); the tail is copied
+// byte by byte by the first threads.
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n16,
+                            const uint8_t* __restrict__ src_b, uint8_t* __restrict__ dst_b,
+                            uint64_t tail_from, uint64_t tail_n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < tail_n) dst_b[tail_from + t] = src_b[tail_from + t];
+}
+
 }  // namespace lyg
 
 extern "C" {
+
+hipError_t lyg_launch_copy(void* dst, const void* src, uint64_t bytes, int grid, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const uint64_t n16 = bytes / 16;
+    hipLaunchKernelGGL(lyg::copy_kernel, dim3(grid), dim3(256), 0, s,
+                       reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n16,
+                       reinterpret_cast<const uint8_t*>(src), reinterpret_cast<uint8_t*>(dst),
+                       16 * n16, bytes - 16 * n16);
+    return hipGetLastError();
+}
 
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s) {
     if (L->n == 0) return hipSuccess;

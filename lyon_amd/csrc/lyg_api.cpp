@@ -27,6 +27,7 @@ hipError_t lyg_launch_arc_to_quads(const float* arcs, uint32_t n, uint32_t* coun
 hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
                            uint32_t* overflow, hipStream_t s);
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+hipError_t lyg_launch_copy(void* dst, const void* src, uint64_t bytes, int grid, hipStream_t s);
 }
 
 #include "ctx.h"
@@ -263,6 +264,16 @@ int32_t lyg_copy_to_host(lyg_ctx* c, void* dst, const void* src, uint64_t bytes)
     if (!bytes) return LYG_OK;
     if (!check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "D2H") ||
         !check(c, hipStreamSynchronize(c->stream), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_device_copy(lyg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+    if (!c || (bytes && (!dst || !src))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!bytes) return LYG_OK;
+    if (((uintptr_t)dst | (uintptr_t)src) & 15u) return LYG_ERR_INVALID_ARGUMENT;
+    const int grid = c->cu_count * 8;
+    if (!check(c, lyg_launch_copy(dst, src, bytes, grid, c->stream), "copy"))
         return LYG_ERR_DEVICE;
     return LYG_OK;
 }
