@@ -35,7 +35,7 @@ struct lyg_ctx {
     uint32_t fill_waves_per_block = 2;
     uint32_t fill_hot_bytes = 24 * 1024;
     // fill: per-path arrays, per-wave working memory and per-round staging
-    DevBuf f_bound, f_status, f_round, f_work;
+    DevBuf f_bound, f_status, f_round, f_hint, f_work;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
 };
 

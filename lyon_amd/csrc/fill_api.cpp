@@ -98,8 +98,8 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     const uint32_t n = J.in.n_paths;
     hipStream_t s = c->stream;
     if (!ensure(c, c->f_bound, 4ull * n) || !ensure(c, c->f_status, 4ull * n) ||
-        !ensure(c, c->f_round, 4ull * n) || !ensure(c, c->counts_v, 4ull * n) ||
-        !ensure(c, c->counts_i, 4ull * n))
+        !ensure(c, c->f_round, 4ull * n) || !ensure(c, c->f_hint, 4ull * n) ||
+        !ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->counts_i, 4ull * n))
         return LYG_ERR_DEVICE;
     lyg::FillLaunch L{};
     L.in = J.in;
@@ -109,6 +109,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     L.path_ni = as<uint32_t>(c->counts_i);
     L.path_status = as<uint32_t>(c->f_status);
     L.path_round = as<uint32_t>(c->f_round);
+    L.path_hint = as<uint32_t>(c->f_hint);
     L.vertex_offset = J.vertex_offset;
     std::vector<uint32_t> bound(n);
     if (!check(c, lyg_launch_fill_bound(&L, n, as<uint32_t>(c->f_bound), s), "fill bound") ||
@@ -131,9 +132,9 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             !check(c, lyg_launch_fill_u32(L.path_round, n, 0xffffffffu, s), "zero"))
             return LYG_ERR_DEVICE;
     }
-    uint64_t scale = 1;
+    uint64_t scale = 1, next_scale = 4;
     int round = 0;
-    for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale *= 4) {
+    for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale = next_scale) {
         // order by size class, then by path
         std::vector<int> cls(todo.size());
         for (size_t k = 0; k < todo.size(); ++k) cls[k] = size_class(bound[todo[k]]);
@@ -223,17 +224,28 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         }
         if (c->timing && round == 0) (void)hipEventRecord(c->ev[3], s);
         R->round_len.push_back((uint32_t)m);
-        std::vector<uint32_t> st_h(n);
+        std::vector<uint32_t> st_h(n), hint_h(n);
         if (!check(c, hipMemcpyAsync(st_h.data(), L.path_status, 4ull * n, hipMemcpyDeviceToHost, s),
+                   "D2H") ||
+            !check(c, hipMemcpyAsync(hint_h.data(), L.path_hint, 4ull * n, hipMemcpyDeviceToHost, s),
                    "D2H") ||
             !check(c, hipStreamSynchronize(s), "sync"))
             return LYG_ERR_DEVICE;
         std::vector<uint32_t> next;
+        uint64_t grow = 4;  // next round's room: at least 4x, more if a path's hint asks
         for (uint32_t p : list) {
-            if (st_h[p] == ST_RETRY) next.push_back(p);
-            else R->status[p] = st_h[p];
+            if (st_h[p] == ST_RETRY) {
+                next.push_back(p);
+                uint64_t v, t;
+                stage_caps(bound[p], scale, &v, &t);
+                const uint64_t want = (uint64_t)hint_h[p] + hint_h[p] / 4;
+                while (v * grow < want && grow < 1024) grow *= 2;
+            } else {
+                R->status[p] = st_h[p];
+            }
         }
         todo.swap(next);
+        next_scale = scale * grow;
     }
     R->rounds = round;
     for (uint32_t p : todo) R->status[p] = LYG_ERR_NO_SPACE;  // still out of room

@@ -180,6 +180,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->f_bound);
     free_buf(c->f_status);
     free_buf(c->f_round);
+    free_buf(c->f_hint);
     free_buf(c->f_work);
     for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
         for (DevBuf& b : *v) free_buf(b);

@@ -63,6 +63,7 @@ struct FillLaunch {
     uint32_t* path_ni;
     uint32_t* path_status;
     uint32_t* path_round;          // the round whose staging holds the path's output
+    uint32_t* path_hint;           // retry status: estimated vertices the path needs
     uint32_t round;
     const uint32_t* path_list;     // paths to process (nullptr: 0..n_list)
     uint32_t n_list;
