@@ -225,6 +225,9 @@ typedef struct lyg_tuning {
     uint32_t lds_bytes_per_wave;
     float best_ms;
 } lyg_tuning;
+/* Sets the launch shapes explicitly (either pointer may be null), e.g. a tuning kept from an
+ * earlier run. */
+int32_t lyg_ctx_set_tuning(lyg_ctx* ctx, const lyg_tuning* stroke, const lyg_tuning* fill);
 int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,

@@ -264,3 +264,15 @@ def autotune(batcher: _Batcher, db: DeviceBatch) -> dict:
         raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
     del res
     return {k: getattr(t, k) for k, _ in t._fields_}
+
+
+def set_tuning(stroke_blocks_per_cu=None, fill_waves_per_block=None, fill_lds_bytes=None):
+    """Sets launch shapes explicitly (lyg_ctx_set_tuning)."""
+    s = f = None
+    if stroke_blocks_per_cu is not None:
+        s = B.TuningC(stroke_blocks_per_cu, 4, 0, 0.0)
+    if fill_waves_per_block is not None:
+        f = B.TuningC(0, fill_waves_per_block, fill_lds_bytes or 0, 0.0)
+    st = B.lib().lyg_ctx_set_tuning(_Ctx.get(), C.byref(s) if s else None, C.byref(f) if f else None)
+    if st != 0:
+        raise TessellationError(st)

@@ -33,7 +33,7 @@ struct lyg_ctx {
     uint32_t stroke_blocks_per_cu = 8;
     // fill launch shape (set by the autotuner): waves per block, LDS bytes per wave
     uint32_t fill_waves_per_block = 2;
-    uint32_t fill_hot_bytes = 24 * 1024;
+    uint32_t fill_hot_bytes = 16 * 1024;
     // fill: per-path arrays, per-wave working memory and per-round staging
     DevBuf f_bound, f_status, f_round, f_hint, f_work;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;

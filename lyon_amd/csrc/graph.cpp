@@ -142,6 +142,22 @@ void lyg_graph_destroy(lyg_graph* g) {
 }
 
 // ---- autotuner -----------------------------------------------------------------------
+int32_t lyg_ctx_set_tuning(lyg_ctx* c, const lyg_tuning* stroke, const lyg_tuning* fill) {
+    if (!c) return LYG_ERR_INVALID_ARGUMENT;
+    if (stroke) {
+        if (stroke->blocks_per_cu < 1 || stroke->blocks_per_cu > 64) return LYG_ERR_INVALID_ARGUMENT;
+        c->stroke_blocks_per_cu = stroke->blocks_per_cu;
+    }
+    if (fill) {
+        if (fill->waves_per_block < 1 || fill->waves_per_block > 4 ||
+            lyg_fill_occupancy(fill->waves_per_block, fill->lds_bytes_per_wave) < 1)
+            return LYG_ERR_INVALID_ARGUMENT;
+        c->fill_waves_per_block = fill->waves_per_block;
+        c->fill_hot_bytes = fill->lds_bytes_per_wave;
+    }
+    return LYG_OK;
+}
+
 int32_t lyg_autotune_stroke(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen) {
     if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
@@ -187,8 +203,8 @@ int32_t lyg_autotune_fill(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_opt
     struct Shape {
         uint32_t wpb, hot;
     };
-    static const Shape cand[] = {{1, 64 * 1024}, {1, 32 * 1024}, {2, 32 * 1024}, {2, 24 * 1024},
-                                 {2, 16 * 1024}, {4, 16 * 1024}, {4, 8 * 1024},  {4, 0}};
+    static const Shape cand[] = {{1, 64 * 1024}, {1, 40 * 1024}, {2, 32 * 1024}, {2, 24 * 1024},
+                                 {2, 16 * 1024}, {4, 16 * 1024}, {4, 8 * 1024},  {2, 0}};
     hipEvent_t e0, e1;
     if (!check(c, hipEventCreate(&e0), "event") || !check(c, hipEventCreate(&e1), "event"))
         return LYG_ERR_DEVICE;
