@@ -47,7 +47,7 @@ def kernel_times(L, ctx):
 
 def plain_rate(tess, paths, opts, make_buffers):
     """Triangles/s of one tessellate_path call per path (host buffers in and out)."""
-    for i in range(min(20, len(paths))):
+    for i in range(min(5, len(paths))):
         tess.tessellate_path(paths[i], opts[i], make_buffers()[1])
     tris = 0
     t0 = time.perf_counter()
@@ -55,6 +55,8 @@ def plain_rate(tess, paths, opts, make_buffers):
         vb, bb = make_buffers()
         tess.tessellate_path(paths[i], opts[i], bb)
         tris += len(vb.indices) // 3
+        if i % 16 == 15:
+            log(f"  plain call {i + 1}/{len(paths)}")
     return tris / (time.perf_counter() - t0)
 
 
@@ -67,6 +69,7 @@ def main():
     ap.add_argument("--fill-paths", type=int, default=4096)
     ap.add_argument("--fill-steps", type=int, default=5)
     ap.add_argument("--plain-sample", type=int, default=300)
+    ap.add_argument("--fill-plain-sample", type=int, default=48)
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fill", action="store_true")
@@ -170,14 +173,15 @@ def main():
         log(f"fill: timing {args.fill_steps} steps")
         sweep_ms, copy_ms = [], []
         t0 = time.perf_counter()
-        for _ in range(args.fill_steps):
+        for k in range(args.fill_steps):
             fb.run(fdb, check=False)
             a, b = kernel_times(L, ctx)
             copy_ms.append(a)
             sweep_ms.append(b)
+            log(f"  fill step {k} done")
         synchronize()
         fms = (time.perf_counter() - t0) * 1e3 / args.fill_steps
-        fns = min(args.plain_sample, args.fill_paths)
+        fns = min(args.fill_plain_sample, args.fill_paths)
         fos = [FillOptions(tolerance=0.05, fill_rule=FillRule(int(o["fill_rule"])))
                for o in fopts[:fns]]
         log("fill: plain per-path drop-in")
