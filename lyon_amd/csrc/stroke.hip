@@ -74,18 +74,39 @@ struct Carry {
     uint32_t run_v, run_i;  // vertices / triangles emitted so far in this path
 };
 
-template <class T>
-__device__ __forceinline__ T pick(T lane_val, int src, const T& c0, const T& c1, const T& c2);
+// Slot j of three values, selected field by field on values (a select between struct
+// lvalues would become a select of addresses and pin the carry in private memory).
+__device__ __forceinline__ uint32_t sel3(uint32_t a, uint32_t b, uint32_t c, int j) {
+    uint32_t r = j == 1 ? b : c;
+    return j == 0 ? a : r;
+}
+__device__ __forceinline__ float sel3f(float a, float b, float c, int j) {
+    float r = j == 1 ? b : c;
+    return j == 0 ? a : r;
+}
+__device__ __forceinline__ vec2 sel3(vec2 a, vec2 b, vec2 c, int j) {
+    return mk(sel3f(a.x, b.x, c.x, j), sel3f(a.y, b.y, c.y, j));
+}
+__device__ __forceinline__ Pub sel3(const Pub& a, const Pub& b, const Pub& c, int j) {
+    Pub r;
+    r.next[0] = sel3(a.next[0], b.next[0], c.next[0], j);
+    r.next[1] = sel3(a.next[1], b.next[1], c.next[1], j);
+    r.code = sel3(a.code, b.code, c.code, j);
+    r.base = sel3(a.base, b.base, c.base, j);
+    return r;
+}
 
+// Value of lane `src`, or of carry slot -src-1 when src < 0.  The carry slots are read
+// with constant indexes so the carry stays in registers.
 __device__ __forceinline__ vec2 gather_v(vec2 v, int src, const vec2 c[3]) {
     vec2 s = shfl_v(v, src < 0 ? 0 : src);
     if (src >= 0) return s;
-    return c[-src - 1];
+    return sel3(c[0], c[1], c[2], -src - 1);
 }
 __device__ __forceinline__ uint32_t gather_u(uint32_t v, int src, const uint32_t c[3]) {
     uint32_t s = shfl_u(v, src < 0 ? 0 : src);
     if (src >= 0) return s;
-    return c[-src - 1];
+    return sel3(c[0], c[1], c[2], -src - 1);
 }
 __device__ __forceinline__ Pub gather_pub(const Pub& v, int src, const Pub c[3]) {
     Pub r;
@@ -94,7 +115,7 @@ __device__ __forceinline__ Pub gather_pub(const Pub& v, int src, const Pub c[3])
     r.next[1] = shfl_v(v.next[1], s);
     r.code = shfl_u(v.code, s);
     r.base = shfl_u(v.base, s);
-    if (src < 0) r = c[-src - 1];
+    if (src < 0) r = sel3(c[0], c[1], c[2], -src - 1);
     return r;
 }
 
@@ -495,8 +516,12 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             uint32_t nv = 0, nt = 0;
             if (is_join) join_counts(J, prv.code, isp >= 3, &nv, &nt);
             if (item && isE) {
+                // the call takes addresses: it gets copies made here, so the
+                // originals never live in private memory
                 Sink sk{fake_base, 0, 0, nullptr, 0, nullptr, false, 0, 0, nullptr};
-                end_subpath(ein, P, sk);
+                const EndInput e2 = ein;
+                const SParams P2 = P;
+                end_subpath(e2, P2, sk);
                 nv = sk.nv;
                 nt = sk.ni;
             }
@@ -529,7 +554,13 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                     epd_from_pub(pj, PP, prv.next[0], prv.next[1], prv.code, prv.base, P.join);
                     join_emit(J, pj, isp >= 3, P, se);
                 }
-                if (item && isE) end_subpath(ein, P, se);
+                if (item && isE) {
+                    Sink s2 = se;
+                    const EndInput e2 = ein;
+                    const SParams P2 = P;
+                    end_subpath(e2, P2, s2);
+                    se = s2;
+                }
             }
 
             // ---- 5. carry to the next window ----
@@ -576,10 +607,12 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                         cisp[k] = nisp[k];
                         cpub[k] = npub[k];
                     } else {
-                        cpos[k] = c.pos[k - taken];
-                        cfl[k] = c.flags[k - taken];
-                        cisp[k] = c.isp[k - taken];
-                        cpub[k] = c.pub[k - taken];
+                        // constant-index selects keep the carry in registers
+                        const int j = k - taken;
+                        cpos[k] = sel3(c.pos[0], c.pos[1], c.pos[2], j);
+                        cfl[k] = sel3(c.flags[0], c.flags[1], c.flags[2], j);
+                        cisp[k] = sel3(c.isp[0], c.isp[1], c.isp[2], j);
+                        cpub[k] = sel3(c.pub[0], c.pub[1], c.pub[2], j);
                     }
                 }
                 for (int k = 0; k < 3; ++k) {
@@ -657,8 +690,13 @@ __device__ __forceinline__ uint32_t next_path(uint32_t* counter) {
     return readlane_u(mine, 0);
 }
 
+// Waves per SIMD: the walk is latency bound (one wave's window is a long dependent
+// chain of shuffles, LDS reads and scans), so occupancy beats spill-free code.  Measured
+// on the 10,000-path stroke workload: count pass 11.4 ms at 1 wave/SIMD, 7.2 at 2, 6.0 at
+// 4; emit pass 16.0 ms at 1, 11.1 at 2, 15.7 at 4 (its larger state spills too much).
 template <bool EMIT>
-__global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK) stroke_kernel(StrokeLaunch L) {
+__global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(EMIT ? 2 : 4))) stroke_kernel(StrokeLaunch L) {
     __shared__ VRec recs_all[WAVES_PER_BLOCK][WAVE];
     const int w = threadIdx.x / WAVE;
     VRec* recs = recs_all[w];

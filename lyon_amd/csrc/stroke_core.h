@@ -329,6 +329,19 @@ struct JInfo {
     RoundArc arc[2]; // round join arcs (valid when needs[side] and Round)
 };
 
+// One side's round join arc (stroke.rs:1957-1962); `side` is a constant so the join
+// state is never indexed at run time.
+template <int side>
+__device__ __forceinline__ void round_join_side(const JInfo& J, const SParams& p, Sink& s) {
+    if (!J.needs[side]) return;
+    const uint32_t pv = J.x.pv[side], nv = J.x.nv[side];
+    const bool swap = J.arc[side].swap;
+    Sink s2 = s;  // the call takes its address: keep the caller's sink in registers
+    tessellate_arc(J.arc[side].a0, J.arc[side].a1, swap ? nv : pv, swap ? pv : nv,
+                   J.arc[side].nsub, J.x.pos, p.hw, s2);
+    s = s2;
+}
+
 // tessellate_join (stroke.rs:1907-1965) with the arcs precomputed in `J`.
 __device__ __forceinline__ void tessellate_join(const JInfo& J, const SParams& p, Sink& s) {
     const EPD& j = J.x;
@@ -344,17 +357,8 @@ __device__ __forceinline__ void tessellate_join(const JInfo& J, const SParams& p
         }
     }
     if (j.line_join == LYG_JOIN_ROUND) {
-        for (int side = 0; side < 2; ++side) {
-            if (!J.needs[side]) continue;
-            uint32_t sv = j.pv[side], ev = j.nv[side];
-            if (J.arc[side].swap) {
-                uint32_t t = sv;
-                sv = ev;
-                ev = t;
-            }
-            tessellate_arc(J.arc[side].a0, J.arc[side].a1, sv, ev, J.arc[side].nsub, j.pos, p.hw,
-                           s);
-        }
+        round_join_side<SP>(J, p, s);
+        round_join_side<SN>(J, p, s);
     }
 }
 
@@ -377,16 +381,10 @@ __device__ __forceinline__ void join_side_positions(vec2 prevp, EPD& join, vec2 
     pt = divs(pt, pl);
     nt = divs(nt, nl);
     vec2 normal = compute_normal(pt, nt);
-    int front;
-    vec2 fnormal;
-    if (cross(pt, nt) >= 0.0f) {
-        front = SN;
-        fnormal = neg(normal);
-    } else {
-        front = SP;
-        fnormal = normal;
-    }
-    int back = 1 - front;
+    // front side: SN when turning left.  Slots are only ever indexed by constants (a
+    // runtime index would keep the join state in private memory).
+    const bool fsn = cross(pt, nt) >= 0.0f;
+    vec2 fnormal = fsn ? neg(normal) : normal;
     vec2 extruded = mul(fnormal, p.hw);
     bool unclipped_miter =
         (join.line_join == LYG_JOIN_MITER || join.line_join == LYG_JOIN_MITER_CLIP) &&
@@ -397,7 +395,8 @@ __device__ __forceinline__ void join_side_positions(vec2 prevp, EPD& join, vec2 
         float d_next = dot(extruded, neg(nt)) - nl;
         float d_prev = dot(extruded, pt) - pl;
         if (ffmin(d_next, d_prev) > 0.0f || sqlen(normal) < 1e-5f) {
-            join.fold[front] = 1;
+            if (fsn) join.fold[SN] = 1;
+            else join.fold[SP] = 1;
             fold = true;
         }
     }
@@ -408,19 +407,32 @@ __device__ __forceinline__ void join_side_positions(vec2 prevp, EPD& join, vec2 
     join.prev[SN] = sub(join.pos, n0);
     join.next[SN] = sub(join.pos, n1);
     if (!fold) {
-        vec2 mp[2] = {add(join.pos, mul(normal, p.hw)), sub(join.pos, mul(normal, p.hw))};
-        join.has_single[back] = 1;
-        join.single[back] = mp[back];
-        if (unclipped_miter) {
-            join.has_single[front] = 1;
-            join.single[front] = mp[front];
-        } else if (join.line_join == LYG_JOIN_MITER_CLIP) {
-            vec2 a = sub(join.prev[front], join.pos);
-            vec2 b = sub(join.next[front], join.pos);
+        const vec2 mp_p = add(join.pos, mul(normal, p.hw));
+        const vec2 mp_n = sub(join.pos, mul(normal, p.hw));
+        // the back side always gets its single vertex, the front one for a miter
+        if (fsn || unclipped_miter) {
+            join.has_single[SP] = 1;
+            join.single[SP] = mp_p;
+        }
+        if (!fsn || unclipped_miter) {
+            join.has_single[SN] = 1;
+            join.single[SN] = mp_n;
+        }
+        if (!unclipped_miter && join.line_join == LYG_JOIN_MITER_CLIP) {
+            const vec2 fprev = fsn ? join.prev[SN] : join.prev[SP];
+            const vec2 fnext = fsn ? join.next[SN] : join.next[SP];
+            vec2 a = sub(fprev, join.pos);
+            vec2 b = sub(fnext, join.pos);
             vec2 i1, i2;
             clip_intersections(a, b, fnormal, p.miter_limit * p.hw, &i1, &i2);
-            join.prev[front] = add(join.pos, i1);
-            join.next[front] = add(join.pos, i2);
+            const vec2 np = add(join.pos, i1), nn = add(join.pos, i2);
+            if (fsn) {
+                join.prev[SN] = np;
+                join.next[SN] = nn;
+            } else {
+                join.prev[SP] = np;
+                join.next[SP] = nn;
+            }
         }
     }
 }
@@ -517,36 +529,42 @@ __device__ __forceinline__ void join_counts(const JInfo& J, uint32_t pcode, bool
     *nt = t;
 }
 
+// One side's join vertices (add_join_base_vertices, stroke.rs:2024-2055): the ids of
+// its prev/next vertices (equal when the side has a single vertex).
+template <int side>
+__device__ __forceinline__ void join_base_vertices(const EPD& join, const SParams& p, Sink& s,
+                                                   uint32_t& a, uint32_t& b) {
+    if (join.has_single[side]) {
+        a = b = sink_sv(s, join.pos, divs(sub(join.single[side], join.pos), p.hw), p.hw);
+    } else {
+        a = sink_sv(s, join.pos, divs(sub(join.prev[side], join.pos), p.hw), p.hw);
+        b = sink_sv(s, join.pos, divs(sub(join.next[side], join.pos), p.hw), p.hw);
+    }
+}
+
 // The emitting half: join vertices (flattened_step's POS/NEG pair or
 // add_join_base_vertices NEG then POS, stroke.rs:2024-2055), the edge triangles from
 // `prev` when with_edge (stroke.rs:1616-1618) and tessellate_join.  Fills J.x's ids.
+// The ids are computed as values and stored once after the branches: stores to the same
+// slots in both branches would be merged into stores through a selected address, which
+// keeps the join state in private memory.
 __device__ __forceinline__ void join_emit(JInfo& J, const EPD& prev, bool with_edge,
                                           const SParams& p, Sink& s) {
     EPD& join = J.x;
+    uint32_t pp = INVALID_ID, np = INVALID_ID, pn = INVALID_ID, nn = INVALID_ID;
     if (J.fast) {
         if (!J.weird) {
-            uint32_t a = sink_sv(s, join.pos, J.normal, p.hw);
-            uint32_t b = sink_sv(s, join.pos, neg(J.normal), p.hw);
-            join.pv[SP] = join.nv[SP] = a;
-            join.pv[SN] = join.nv[SN] = b;
+            pp = np = sink_sv(s, join.pos, J.normal, p.hw);
+            pn = nn = sink_sv(s, join.pos, neg(J.normal), p.hw);
         }
     } else {
-        for (int k = 0; k < 2; ++k) {
-            int side = k == 0 ? SN : SP;
-            if (join.has_single[side]) {
-                uint32_t id =
-                    sink_sv(s, join.pos, divs(sub(join.single[side], join.pos), p.hw), p.hw);
-                join.pv[side] = join.nv[side] = id;
-            } else {
-                uint32_t a =
-                    sink_sv(s, join.pos, divs(sub(join.prev[side], join.pos), p.hw), p.hw);
-                uint32_t b =
-                    sink_sv(s, join.pos, divs(sub(join.next[side], join.pos), p.hw), p.hw);
-                join.pv[side] = a;
-                join.nv[side] = b;
-            }
-        }
+        join_base_vertices<SN>(join, p, s, pn, nn);
+        join_base_vertices<SP>(join, p, s, pp, np);
     }
+    join.pv[SP] = pp;
+    join.nv[SP] = np;
+    join.pv[SN] = pn;
+    join.nv[SN] = nn;
     if (with_edge) add_edge_triangles(prev, join, s);
     tessellate_join(J, p, s);
 }
