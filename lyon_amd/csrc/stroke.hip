@@ -581,14 +581,14 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                 int taken = 0;
                 for (int k = 0; k < 3; ++k) {
                     int s = lanes[k] < 0 ? 0 : lanes[k];
-                    vec2 pv_ = shfl_v(pos, s);
-                    uint32_t fv = shfl_u(flags, s);
-                    uint32_t iv = shfl_u(isp, s);
+                    vec2 pv_ = readlane_v(pos, s);
+                    uint32_t fv = readlane_u(flags, s);
+                    uint32_t iv = readlane_u(isp, s);
                     Pub pb_;
-                    pb_.next[0] = shfl_v(pub.next[0], s);
-                    pb_.next[1] = shfl_v(pub.next[1], s);
-                    pb_.code = shfl_u(pub.code, s);
-                    pb_.base = shfl_u(pub.base, s);
+                    pb_.next[0] = readlane_v(pub.next[0], s);
+                    pb_.next[1] = readlane_v(pub.next[1], s);
+                    pb_.code = readlane_u(pub.code, s);
+                    pb_.base = readlane_u(pub.base, s);
                     if (lanes[k] >= 0) {
                         npos[k] = pv_;
                         nflags[k] = fv;
@@ -623,7 +623,7 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                 }
                 // keep-filter state
                 int lk = highest_bit(keptm);
-                vec2 lkp = shfl_v(pos, lk < 0 ? 0 : lk);
+                vec2 lkp = readlane_v(pos, lk < 0 ? 0 : lk);
                 const int lastv = highest_bit(validm);
                 const bool last_is_end = lastv >= 0 && ((endm >> lastv) & 1ull);
                 // sub-path counters after the window
@@ -653,11 +653,11 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                     uint64_t m1 = ballot(kept && isp == 1);
                     uint64_t m2 = ballot(kept && isp == 2);
                     int s0 = highest_bit(m0), s1 = highest_bit(m1), s2 = highest_bit(m2);
-                    vec2 g0 = shfl_v(pos, s0 < 0 ? 0 : s0);
-                    vec2 g1 = shfl_v(pos, s1 < 0 ? 0 : s1);
-                    uint32_t gf1 = shfl_u(flags, s1 < 0 ? 0 : s1);
-                    vec2 g2 = shfl_v(pos, s2 < 0 ? 0 : s2);
-                    uint32_t gb2 = shfl_u(pub.base, s2 < 0 ? 0 : s2);
+                    vec2 g0 = readlane_v(pos, s0 < 0 ? 0 : s0);
+                    vec2 g1 = readlane_v(pos, s1 < 0 ? 0 : s1);
+                    uint32_t gf1 = readlane_u(flags, s1 < 0 ? 0 : s1);
+                    vec2 g2 = readlane_v(pos, s2 < 0 ? 0 : s2);
+                    uint32_t gb2 = readlane_u(pub.base, s2 < 0 ? 0 : s2);
                     if (s0 >= 0 && s0 >= rbl) c.f0 = g0;
                     if (s1 >= 0 && s1 >= rbl) {
                         c.f1 = g1;
@@ -696,7 +696,7 @@ __device__ __forceinline__ uint32_t next_path(uint32_t* counter) {
 // 4; emit pass 16.0 ms at 1, 11.1 at 2, 15.7 at 4 (its larger state spills too much).
 template <bool EMIT>
 __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(EMIT ? 2 : 4))) stroke_kernel(StrokeLaunch L) {
+    LYG_WAVES_PER_EU(EMIT ? 2 : 4) stroke_kernel(StrokeLaunch L) {
     __shared__ VRec recs_all[WAVES_PER_BLOCK][WAVE];
     const int w = threadIdx.x / WAVE;
     VRec* recs = recs_all[w];

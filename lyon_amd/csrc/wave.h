@@ -6,6 +6,12 @@
 
 #include "lyg_math.h"
 
+// Occupancy target of a kernel, in waves per SIMD (a host-side build of the kernels for
+// checking may predefine it empty).
+#ifndef LYG_WAVES_PER_EU
+#define LYG_WAVES_PER_EU(...) __attribute__((amdgpu_waves_per_eu(__VA_ARGS__)))
+#endif
+
 namespace lyg {
 
 static constexpr int WAVE = 64;
@@ -26,14 +32,25 @@ __device__ __forceinline__ uint32_t readlane_u(uint32_t v, int l) {
 __device__ __forceinline__ float readlane_f(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
+// Lane l's value broadcast to the wave (l must be wave-uniform): the result is a scalar,
+// so wave-uniform state built from it stays out of vector registers.
+__device__ __forceinline__ vec2 readlane_v(vec2 v, int l) {
+    return mk(readlane_f(v.x, l), readlane_f(v.y, l));
+}
+// Inclusive prefix sum over the wave with DPP row moves (no LDS round trips):
+// shifts by 1, 2, 4, 8 inside each row of 16 lanes, then row 1 and 3 add the last lane
+// of the row before (row_bcast:15) and rows 2-3 add lane 31 (row_bcast:31).  Lanes
+// whose source is out of the row read 0 (bound_ctrl); rows outside the row mask add
+// the `old` operand, 0.
 __device__ __forceinline__ uint32_t incl_scan(uint32_t v) {
-    int l = lane_id();
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t t = shfl_u(v, l >= o ? l - o : l);
-        if (l >= o) v += t;
-    }
-    return v;
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)x;
 }
 __device__ __forceinline__ int highest_bit(uint64_t m) { return m ? 63 - __builtin_clzll(m) : -1; }
 __device__ __forceinline__ uint64_t mask_lt(int l) { return l >= 64 ? ~0ull : ((1ull << l) - 1ull); }
