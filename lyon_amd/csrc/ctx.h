@@ -35,8 +35,11 @@ struct lyg_ctx {
     uint32_t fill_waves_per_block = 2;
     uint32_t fill_hot_bytes = 16 * 1024;
     // fill: per-path arrays, per-wave working memory and per-round staging
-    DevBuf f_bound, f_status, f_round, f_hint, f_work;
+    DevBuf f_bound, f_status, f_round, f_hint, f_work, f_inter, f_boxes;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
+    // fill: one stream per size class of a round, and the events that join them
+    hipStream_t class_stream[8] = {};
+    hipEvent_t class_ev[9] = {};
 };
 
 inline void free_buf(DevBuf& b) {

@@ -10,6 +10,7 @@
 //   3. an exclusive scan over the per-path vertex/index counts, then fill_copy_kernel
 //      moves every round's staged output to its final place, rebasing the indices.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -24,6 +25,9 @@ int lyg_fill_occupancy(uint32_t waves_per_block, uint32_t hot_bytes);
 hipError_t lyg_launch_fill_bound(const lyg::FillLaunch* L, uint32_t n, uint32_t* bound,
                                  hipStream_t s);
 hipError_t lyg_launch_fill_sweep(const lyg::FillLaunch* L, int grid, hipStream_t s);
+size_t lyg_fill_box_bytes();
+hipError_t lyg_launch_fill_inter(const lyg::FillLaunch* L, uint32_t n, void* boxes, uint32_t* inter,
+                                 hipStream_t s);
 hipError_t lyg_launch_fill_copy(const lyg::FillLaunch* L, hipStream_t s);
 hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
                            uint32_t* overflow, hipStream_t s);
@@ -35,30 +39,32 @@ namespace {
 constexpr uint32_t ST_RETRY = 100;
 constexpr int MAX_ROUNDS = 5;
 constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO_SPACE
-constexpr size_t WORK_BUDGET = 4ull << 30;           // per-launch working memory
+constexpr size_t WORK_BUDGET = 16ull << 30;          // working memory of one round
 constexpr size_t STAGE_BUDGET = 64ull << 30;         // staging memory of one round
 constexpr int MAX_CLASSES = 8;
 
 using lyg::fillk::FillCaps;
 
-// Working-memory capacities of one wave for paths of at most `bound` events at `scale`:
-// active edges, spans and pending edges are bounded by the edges (the events), the heap
-// and the node pool grow with intersections (retry rounds scale them).
-FillCaps class_caps(uint64_t bound, uint64_t scale) {
+// Working-memory capacities of one wave for paths of at most `bound` events and `inter`
+// estimated self-intersections at `scale` (1, then growing with each retry round).
+// Active edges, spans and pending edges are bounded by the edges; the event heap and the
+// monotone node pool grow with the intersections (each one adds a vertex and can leave
+// two truncated edge parts pending).
+FillCaps class_caps(uint64_t bound, uint64_t inter, uint64_t scale) {
     auto cap = [](uint64_t v) { return (uint32_t)std::min<uint64_t>(v, 0x7fffffffu); };
     FillCaps k;
     k.ev = cap(bound);
-    k.heap = cap(scale * (2 * bound + 256));
+    k.heap = cap(scale * (2 * bound + 2 * inter + 256));
     k.act = cap(scale * (bound + 64));
     k.span = k.act;
     k.below = k.act;
-    k.node = cap(4ull * k.act + 256);
+    k.node = cap(4ull * k.act + 256 + scale * 2 * inter);
     return k;
 }
 
-// Staging capacity (vertices, triangles) of one path.
-void stage_caps(uint64_t bound, uint64_t scale, uint64_t* v, uint64_t* t) {
-    *v = std::min<uint64_t>(scale * (2 * bound + 256), 0x7fffffffu);
+// Staging capacity (vertices, triangles) of one path: its events plus its intersections.
+void stage_caps(uint64_t bound, uint64_t inter, uint64_t scale, uint64_t* v, uint64_t* t) {
+    *v = std::min<uint64_t>(scale * (2 * bound + 2 * inter + 256), 0x7fffffffu);
     *t = std::min<uint64_t>(2 * *v, 0x7fffffffu);
 }
 
@@ -66,10 +72,26 @@ int size_class(uint64_t bound) {
     int k = 0;
     uint64_t lim = 1u << 12;
     while (bound > lim && k < MAX_CLASSES - 1) {
-        lim <<= 3;
+        lim <<= 2;
         ++k;
     }
     return k;
+}
+
+// Streams (default priority) and events for running a round's size classes at once.
+bool class_streams(lyg_ctx* c, size_t n) {
+    if (n > (size_t)MAX_CLASSES) return false;
+    for (size_t i = 0; i < n; ++i)
+        if (!c->class_stream[i] &&
+            !check(c, hipStreamCreateWithFlags(&c->class_stream[i], hipStreamNonBlocking),
+                   "hipStreamCreate"))
+            return false;
+    for (size_t i = 0; i <= n; ++i)
+        if (!c->class_ev[i] &&
+            !check(c, hipEventCreateWithFlags(&c->class_ev[i], hipEventDisableTiming),
+                   "hipEventCreate"))
+            return false;
+    return true;
 }
 
 bool ensure_at(lyg_ctx* c, std::vector<DevBuf>& v, size_t i, size_t bytes) {
@@ -111,12 +133,27 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     L.path_round = as<uint32_t>(c->f_round);
     L.path_hint = as<uint32_t>(c->f_hint);
     L.vertex_offset = J.vertex_offset;
-    std::vector<uint32_t> bound(n);
+    std::vector<uint32_t> bound(n), inter(n);
+    uint32_t n_verbs = 0;
     if (!check(c, lyg_launch_fill_bound(&L, n, as<uint32_t>(c->f_bound), s), "fill bound") ||
         !check(c, hipMemcpyAsync(bound.data(), c->f_bound.p, 4ull * n, hipMemcpyDeviceToHost, s),
                "D2H") ||
+        !check(c, hipMemcpyAsync(&n_verbs, J.in.verb_off + n, 4, hipMemcpyDeviceToHost, s), "D2H") ||
         !check(c, hipStreamSynchronize(s), "sync"))
         return LYG_ERR_DEVICE;
+    // intersection estimate: one box record per verb, then a pair count per path
+    if (!ensure(c, c->f_inter, 4ull * n) ||
+        !ensure(c, c->f_boxes, lyg_fill_box_bytes() * ((size_t)n_verbs + 1)))
+        return LYG_ERR_DEVICE;
+    if (!check(c, lyg_launch_fill_inter(&L, n, c->f_boxes.p, as<uint32_t>(c->f_inter), s),
+               "fill inter") ||
+        !check(c, hipMemcpyAsync(inter.data(), c->f_inter.p, 4ull * n, hipMemcpyDeviceToHost, s),
+               "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    // size classes go by the larger of the two room drivers
+    std::vector<uint64_t> need(n);
+    for (uint32_t i = 0; i < n; ++i) need[i] = (uint64_t)bound[i] + 2ull * inter[i];
     R->status.assign(n, 0);
     // paths still to sweep, with their scale (4^round)
     std::vector<uint32_t> todo;
@@ -137,7 +174,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale = next_scale) {
         // order by size class, then by path
         std::vector<int> cls(todo.size());
-        for (size_t k = 0; k < todo.size(); ++k) cls[k] = size_class(bound[todo[k]]);
+        for (size_t k = 0; k < todo.size(); ++k) cls[k] = size_class(need[todo[k]]);
         std::vector<size_t> order(todo.size());
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] < cls[b]; });
@@ -150,7 +187,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             uint32_t p = todo[order[k]];
             list[k] = p;
             uint64_t v, t;
-            stage_caps(bound[p], scale, &v, &t);
+            stage_caps(bound[p], inter[p], scale, &v, &t);
             sv[k + 1] = sv[k] + v;
             st[k + 1] = st[k] + t;
         }
@@ -174,42 +211,58 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         L.round = (uint32_t)round;
         L.stage_v = as<float>(c->f_stage_v[round]);
         L.stage_t = as<uint32_t>(c->f_stage_t[round]);
-        // one launch per size class; the classes share the working memory, sized once
+        // One launch per size class, all classes running at once on the context's class
+        // streams (largest class first), each with its own part of the working memory.
         struct Launch {
-            size_t k0, k1, blocks, stride;
+            size_t k0, k1, blocks, stride, work_off;
             FillCaps cap;
         };
         const uint32_t wpb = c->fill_waves_per_block;
         const uint32_t hot = c->fill_hot_bytes;
         const int occ = std::max(1, lyg_fill_occupancy(wpb, hot));
         std::vector<Launch> launches;
-        size_t work_bytes = 0;
+        size_t want_bytes = 0;
         for (size_t k0 = 0; k0 < m;) {
             int kc = cls[order[k0]];
             size_t k1 = k0;
-            uint64_t maxb = 0;
+            uint64_t maxb = 0, maxi = 0;
             while (k1 < m && cls[order[k1]] == kc) {
                 maxb = std::max<uint64_t>(maxb, bound[list[k1]]);
+                maxi = std::max<uint64_t>(maxi, inter[list[k1]]);
                 ++k1;
             }
             Launch ln;
             ln.k0 = k0;
             ln.k1 = k1;
-            ln.cap = class_caps(std::max<uint64_t>(maxb, 1), scale);
+            ln.cap = class_caps(std::max<uint64_t>(maxb, 1), maxi, scale);
             ln.stride = lyg_fill_work_bytes(&ln.cap, hot);
-            const size_t per_block = ln.stride * wpb;
             size_t blocks = (k1 - k0 + wpb - 1) / wpb;
-            blocks = std::min<size_t>(blocks, (size_t)c->cu_count * occ);
-            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, WORK_BUDGET / per_block));
-            work_bytes = std::max(work_bytes, ln.blocks * per_block);
+            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * occ));
+            ln.work_off = 0;
+            want_bytes += ln.blocks * ln.stride * wpb;
             launches.push_back(ln);
             k0 = k1;
         }
+        // over the budget: every class gives up blocks in proportion
+        size_t work_bytes = 0;
+        for (Launch& ln : launches) {
+            if (want_bytes > WORK_BUDGET)
+                ln.blocks = std::max<size_t>(
+                    1, (size_t)((double)ln.blocks * (double)WORK_BUDGET / (double)want_bytes));
+            ln.work_off = work_bytes;
+            work_bytes += (ln.blocks * ln.stride * wpb + 255) & ~(size_t)255;
+        }
         if (!ensure(c, c->f_work, work_bytes)) return LYG_ERR_NO_SPACE;
+        const auto t_round = std::chrono::steady_clock::now();
         if (c->timing && round == 0) (void)hipEventRecord(c->ev[2], s);
-        for (size_t li = 0; li < launches.size(); ++li) {
+        if (!class_streams(c, launches.size()) ||
+            !check(c, hipEventRecord(c->class_ev[0], s), "event"))
+            return LYG_ERR_DEVICE;
+        for (size_t r = 0; r < launches.size(); ++r) {
+            const size_t li = launches.size() - 1 - r;  // largest class first
             const Launch& ln = launches[li];
-            L.work = as<uint8_t>(c->f_work);
+            hipStream_t cs = c->class_stream[li];
+            L.work = as<uint8_t>(c->f_work) + ln.work_off;
             L.work_stride = ln.stride;
             L.caps = ln.cap;
             L.hot_bytes = hot;
@@ -219,8 +272,23 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             L.stage_v_off = d_off + ln.k0;
             L.stage_t_off = d_off + (m + 1) + ln.k0;
             L.work_counter = c->d_counters + 8 + (li & 7);
-            if (!check(c, lyg_launch_fill_sweep(&L, (int)ln.blocks, s), "fill sweep"))
+            const auto t_launch = std::chrono::steady_clock::now();
+            if (!check(c, hipStreamWaitEvent(cs, c->class_ev[0], 0), "wait") ||
+                !check(c, lyg_launch_fill_sweep(&L, (int)ln.blocks, cs), "fill sweep") ||
+                !check(c, hipEventRecord(c->class_ev[1 + li], cs), "event") ||
+                !check(c, hipStreamWaitEvent(s, c->class_ev[1 + li], 0), "wait"))
                 return LYG_ERR_DEVICE;
+            if (trace_on()) {  // serialises the launches: for diagnosis only
+                (void)hipStreamSynchronize(cs);
+                fprintf(stderr,
+                        "[lyg]   class launch %zu: %zu paths, %zu blocks, %.1f KiB per wave, "
+                        "heap %u node %u act %u, %.2f ms\n",
+                        li, ln.k1 - ln.k0, ln.blocks, ln.stride / 1024.0, ln.cap.heap, ln.cap.node,
+                        ln.cap.act,
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                  t_launch).count());
+                fflush(stderr);
+            }
         }
         if (c->timing && round == 0) (void)hipEventRecord(c->ev[3], s);
         R->round_len.push_back((uint32_t)m);
@@ -237,12 +305,19 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             if (st_h[p] == ST_RETRY) {
                 next.push_back(p);
                 uint64_t v, t;
-                stage_caps(bound[p], scale, &v, &t);
+                stage_caps(bound[p], inter[p], scale, &v, &t);
                 const uint64_t want = (uint64_t)hint_h[p] + hint_h[p] / 4;
                 while (v * grow < want && grow < 1024) grow *= 2;
             } else {
                 R->status[p] = st_h[p];
             }
+        }
+        if (trace_on()) {
+            const double ms = std::chrono::duration<double, std::milli>(
+                                  std::chrono::steady_clock::now() - t_round).count();
+            fprintf(stderr, "[lyg] fill round %d: %zu paths in %zu launches, %.2f ms, %zu to retry\n",
+                    round, m, launches.size(), ms, next.size());
+            fflush(stderr);
         }
         todo.swap(next);
         next_scale = scale * grow;

@@ -182,6 +182,8 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->f_round);
     free_buf(c->f_hint);
     free_buf(c->f_work);
+    free_buf(c->f_inter);
+    free_buf(c->f_boxes);
     for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
         for (DevBuf& b : *v) free_buf(b);
     if (c->d_counters) (void)hipFree(c->d_counters);
@@ -189,6 +191,10 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (int i = 0; i < 4; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (hipStream_t& cs : c->class_stream)
+        if (cs) (void)hipStreamDestroy(cs);
+    for (hipEvent_t& ce : c->class_ev)
+        if (ce) (void)hipEventDestroy(ce);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
