@@ -52,6 +52,26 @@ __device__ __forceinline__ uint32_t incl_scan(uint32_t v) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
     return (uint32_t)x;
 }
+// Minimum of v over the wave, returned to every lane: DPP min-scan (lanes whose source is
+// outside the row or the row mask take the `old` operand, 0xffffffff, which cannot win),
+// then lane 63's value.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    const int U = -1;
+    uint32_t x = v, t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x = x < t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x = x < t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x = x < t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x = x < t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x = x < t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(U, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    x = x < t ? x : t;
+    return readlane_u(x, WAVE - 1);
+}
 __device__ __forceinline__ int highest_bit(uint64_t m) { return m ? 63 - __builtin_clzll(m) : -1; }
 __device__ __forceinline__ uint64_t mask_lt(int l) { return l >= 64 ? ~0ull : ((1ull << l) - 1ull); }
 __device__ __forceinline__ uint64_t mask_le(int l) { return l >= 63 ? ~0ull : ((1ull << (l + 1)) - 1ull); }
