@@ -4,8 +4,9 @@ Headline workload (batched stroke): 10,000 open paths with log-uniform segment c
 [8, 256] (25% lines / 25% quadratics / 50% cubics), points uniform in [0, 2048]^2,
 tolerance 0.05, per-path width U[0.5, 8], join uniform over {Miter, Round, Bevel}, caps
 uniform over {Butt, Square, Round}; output f32 positions + u32 indices.  A step = one
-batched pass over the 10,000 paths (count kernel, offset scan, emit kernel); ``value`` =
-output triangles per second over the timed steps.  Inputs are synthetic (seeded
+batched pass over the 10,000 paths (staging-room kernel and scan, one emit kernel into
+per-path staging, offset scan, compaction copy into the final arrays); ``value`` = output
+triangles per second over the timed steps.  Inputs are synthetic (seeded
 generator) and resident in HBM before the timed region.
 
 The same JSON line carries:
@@ -264,7 +265,7 @@ def main():
                    "parallelism": "single GPU"},
         "plain_per_path": {"value": plain_value, "unit": "triangles/s",
                            "sample_paths": ns, "batched_over_plain": value / plain_value},
-        "kernel_ms": {"count_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
+        "kernel_ms": {"sizing_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},

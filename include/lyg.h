@@ -116,9 +116,10 @@ const char* lyg_status_string(int32_t status);
 const char* lyg_ctx_last_error(const lyg_ctx* ctx);
 /* Measurement support (no lyon counterpart): when enabled, HIP events are recorded on the
  * context's stream around two phases of every call; lyg_ctx_kernel_times returns their
- * durations for the last call in milliseconds: for a stroke, the count pass (first) and
- * the emit pass (second); for a fill, the compaction copy (first) and the first sweep
- * round (second). */
+ * durations for the last call in milliseconds: for a batched stroke, the sizing phase
+ * (first: the staging-room kernel and its scans, or the count pass when the batch runs
+ * as count + emit) and the emit pass (second); for a fill, the compaction copy (first)
+ * and the first sweep round (second). */
 int32_t lyg_ctx_enable_timing(lyg_ctx* ctx, int enable);
 int32_t lyg_ctx_kernel_times(lyg_ctx* ctx, float* count_ms, float* emit_ms);
 

@@ -37,6 +37,9 @@ struct lyg_ctx {
     // fill: per-path arrays, per-wave working memory and per-round staging
     DevBuf f_bound, f_status, f_round, f_hint, f_work, f_inter, f_boxes;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
+    // stroke: staged single pass (emit into per-path staging room, then compact)
+    bool stroke_staged = true;
+    DevBuf st_cap_v, st_cap_i, st_off_v, st_off_i, st_v, st_i, st_fallback;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
     hipEvent_t class_ev[9] = {};

@@ -18,6 +18,9 @@
 
 extern "C" {
 hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hipStream_t stream);
+hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, uint32_t* cap_v,
+                                        uint32_t* cap_i, hipStream_t s);
+hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s);
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
 hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
@@ -96,6 +99,146 @@ bool emit_failed(lyg_ctx* c) {
 
 }  // namespace
 
+namespace {
+
+constexpr size_t STROKE_STAGE_BUDGET = 32ull << 30;  // staging bytes of one batch
+
+// Staged single pass: every path is emitted once into its own staging room (sized by
+// stroke_stage_caps_kernel), with path-local ids, while the exact counts are stored; the
+// counts are scanned into the final offsets, the caller's arrays are allocated, and one
+// copy moves each path into place.  A path that outgrew its room is emitted again
+// directly into the final arrays.  Returns 1 when the batch does not fit the staging
+// budget (the caller then runs the count and emit passes).
+int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
+                            lyg::StrokeLaunch& L, uint64_t* tv_out, uint64_t* ti_out) {
+    const uint32_t n = b.in.n_paths;
+    hipStream_t s = c->stream;
+    if (!ensure(c, c->st_cap_v, 4ull * n) || !ensure(c, c->st_cap_i, 4ull * n) ||
+        !ensure(c, c->st_off_v, 4ull * (n + 1)) || !ensure(c, c->st_off_i, 4ull * (n + 1)) ||
+        !ensure(c, c->st_fallback, 4ull * n + 4))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    // 1. staging room and its offsets
+    if (c->timing) (void)hipEventRecord(c->ev[0], s);
+    if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
+        !check(c, lyg_launch_stroke_stage_caps(&L, n, as<uint32_t>(c->st_cap_v),
+                                               as<uint32_t>(c->st_cap_i), s),
+               "stroke stage caps") ||
+        !check(c, lyg_launch_scan(as<uint32_t>(c->st_cap_v), n, as<uint32_t>(c->st_off_v),
+                                  c->d_totals + 0, c->d_counters + 4, s),
+               "scan") ||
+        !check(c, lyg_launch_scan(as<uint32_t>(c->st_cap_i), n, as<uint32_t>(c->st_off_i),
+                                  c->d_totals + 1, c->d_counters + 4, s),
+               "scan"))
+        return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[1], s);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t sv = h[0], si = h[1];
+    if ((uint32_t)h[2] != 0 || 8 * sv + 4 * si > STROKE_STAGE_BUDGET) return 1;
+    if (!ensure(c, c->st_v, 8 * sv) || !ensure(c, c->st_i, 4 * si)) return 1;
+    // 2. one emit pass into the staging room, storing the exact counts
+    L.vtx_off = as<uint32_t>(c->st_off_v);
+    L.idx_off = as<uint32_t>(c->st_off_i);
+    L.out_vertices = as<float>(c->st_v);
+    L.out_indices = as<uint32_t>(c->st_i);
+    L.vertex_capacity = sv;
+    L.index_capacity = si;
+    L.vertex_offset = 0;
+    L.local_ids = 1;
+    L.store_counts = 1;
+    L.work_counter = c->d_counters + 1;
+    L.err = c->d_counters + 5;  // dropped writes are expected here: the counts decide
+    if (c->timing) (void)hipEventRecord(c->ev[2], s);
+    if (!check(c, lyg_launch_stroke(&L, 1, persistent_grid(c, n), s), "stroke emit (staged)"))
+        return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[3], s);
+    // 3. final offsets from the exact counts
+    if (!check(c, lyg_launch_fill_u32(c->d_counters + 4, 1, 0, s), "reset") ||
+        !check(c, lyg_launch_scan(L.path_nv, n, b.vtx_off, c->d_totals + 0, c->d_counters + 4, s),
+               "scan") ||
+        !check(c, lyg_launch_scan(L.path_ni, n, b.idx_off, c->d_totals + 1, c->d_counters + 4, s),
+               "scan") ||
+        !check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t tv = h[0], ti = h[1];
+    *tv_out = tv;
+    *ti_out = ti;
+    if ((uint32_t)h[2] != 0) return LYG_ERR_TOO_MANY_VERTICES;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
+    if (!b.local_ids && tv > 0 && (uint64_t)b.vertex_offset + tv - 1 > 0xfffffffeull)
+        return LYG_ERR_TOO_MANY_VERTICES;
+    float* ov = nullptr;
+    uint32_t* oi = nullptr;
+    int32_t st = alloc(tv, ti, &ov, &oi);
+    if (st != LYG_OK) return st;
+    // 4. compaction, and the list of paths that outgrew their room
+    uint32_t* n_fb = as<uint32_t>(c->st_fallback) + n;
+    lyg::StrokeCompact C{};
+    C.path_nv = L.path_nv;
+    C.path_ni = L.path_ni;
+    C.stage_voff = as<uint32_t>(c->st_off_v);
+    C.stage_ioff = as<uint32_t>(c->st_off_i);
+    C.stage_v = as<float>(c->st_v);
+    C.stage_i = as<uint32_t>(c->st_i);
+    C.vtx_off = b.vtx_off;
+    C.idx_off = b.idx_off;
+    C.out_vertices = ov;
+    C.out_indices = oi;
+    C.vertex_offset = b.vertex_offset;
+    C.local_ids = b.local_ids ? 1u : 0u;
+    C.n_paths = n;
+    C.fallback = as<uint32_t>(c->st_fallback);
+    C.n_fallback = n_fb;
+    uint32_t* hf = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
+    if (!check(c, lyg_launch_fill_u32(n_fb, 1, 0, s), "reset") ||
+        !check(c, lyg_launch_stroke_compact(&C, (int)std::min<uint32_t>(n, 8u * c->cu_count), s),
+               "stroke compact") ||
+        !check(c, hipMemcpyAsync(hf, n_fb, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    // 5. paths that outgrew their room: emitted again at their final offsets
+    if (*hf > 0) {
+        L.in.n_paths = *hf;
+        L.path_list = as<uint32_t>(c->st_fallback);
+        L.vtx_off = b.vtx_off;
+        L.idx_off = b.idx_off;
+        L.out_vertices = ov;
+        L.out_indices = oi;
+        L.vertex_capacity = tv;
+        L.index_capacity = ti;
+        L.vertex_offset = b.vertex_offset;
+        L.local_ids = b.local_ids ? 1u : 0u;
+        L.store_counts = 0;
+        L.work_counter = c->d_counters + 2;
+        L.err = c->d_counters + 6;
+        uint32_t* he = hf + 1;
+        if (!check(c, lyg_launch_fill_u32(c->d_counters + 2, 1, 0, s), "reset") ||
+            !check(c, lyg_launch_fill_u32(c->d_counters + 6, 1, 0, s), "reset") ||
+            !check(c, lyg_launch_stroke(&L, 1, persistent_grid(c, *hf), s), "stroke emit (again)") ||
+            !check(c, hipMemcpyAsync(he, c->d_counters + 6, 4, hipMemcpyDeviceToHost, s), "copy") ||
+            !check(c, hipStreamSynchronize(s), "sync"))
+            return LYG_ERR_DEVICE;
+        if (*he) {
+            c->last_error = "emit pass wrote outside its path range (staged/final mismatch)";
+            return LYG_ERR_INTERNAL;
+        }
+    }
+    if (trace_on()) {
+        fprintf(stderr, "[lyg] stroke staged: %u paths, staging %llu v / %llu i, out %llu v / %llu i, %u emitted again\n",
+                n, (unsigned long long)sv, (unsigned long long)si, (unsigned long long)tv,
+                (unsigned long long)ti, *hf);
+        fflush(stderr);
+    }
+    return LYG_OK;
+}
+
+}  // namespace
+
 int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
                            uint64_t* tv_out, uint64_t* ti_out) {
     const uint32_t n = b.in.n_paths;
@@ -109,6 +252,11 @@ int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     L.path_ni = as<uint32_t>(c->counts_i);
     L.vertex_offset = b.vertex_offset;
     L.local_ids = b.local_ids ? 1u : 0u;
+    if (c->stroke_staged && n > 0) {
+        lyg::StrokeLaunch Ls = L;
+        const int32_t r = stroke_batch_staged(c, b, alloc, Ls, tv_out, ti_out);
+        if (r != 1) return r;
+    }
     uint64_t tv = 0, ti = 0;
     int32_t st = stroke_count_and_scan(c, L, b.vtx_off, b.idx_off, &tv, &ti);
     *tv_out = tv;
@@ -183,6 +331,9 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->f_hint);
     free_buf(c->f_work);
     free_buf(c->f_inter);
+    for (DevBuf* b : {&c->st_cap_v, &c->st_cap_i, &c->st_off_v, &c->st_off_i, &c->st_v, &c->st_i,
+                      &c->st_fallback})
+        free_buf(*b);
     free_buf(c->f_boxes);
     for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
         for (DevBuf& b : *v) free_buf(b);

@@ -36,6 +36,27 @@ struct StrokeLaunch {
                              // path beyond the output capacity (the path is skipped)
     uint64_t vertex_capacity;  // emit pass: output array sizes
     uint64_t index_capacity;
+    const uint32_t* path_list;  // optional: the launch walks path_list[0 .. n_paths)
+    uint32_t store_counts;      // emit pass also stores per-path counts (staged emit)
+};
+
+// Staged single-pass stroke: copy of every path's staged output to its final place.
+struct StrokeCompact {
+    const uint32_t* path_nv;     // exact per-path counts (vertices, indices)
+    const uint32_t* path_ni;
+    const uint32_t* stage_voff;  // staging offsets (n + 1): the capacities' prefix sums
+    const uint32_t* stage_ioff;
+    const float* stage_v;
+    const uint32_t* stage_i;     // path-local vertex ids
+    const uint32_t* vtx_off;     // final offsets (n + 1)
+    const uint32_t* idx_off;
+    float* out_vertices;
+    uint32_t* out_indices;
+    uint32_t vertex_offset;
+    uint32_t local_ids;
+    uint32_t n_paths;
+    uint32_t* fallback;          // paths that outgrew their staging room
+    uint32_t* n_fallback;
 };
 
 namespace fillk {

@@ -703,10 +703,10 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     const int l = lane_id();
     uint32_t pid = next_path(L.work_counter);
     while (pid < L.in.n_paths) {
-        const uint32_t gp = L.in.first_path + pid;
+        const uint32_t gp = L.path_list ? L.path_list[pid] : L.in.first_path + pid;
         uint32_t nv = 0, ni = 0;
         stroke_one_path<EMIT>(L, gp, recs, &nv, &ni);
-        if (!EMIT) {
+        if (!EMIT || L.store_counts) {
             // every lane holds the totals; lane 0's store is the only one kept
             uint32_t* dst_v = L.path_nv + gp;
             uint32_t* dst_i = L.path_ni + gp;
@@ -719,11 +719,95 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     }
 }
 
+// ---- staged single pass -----------------------------------------------------------------
+// Room for one path's staged output: two vertices and two triangles per flattened point
+// (the fast path of every curve point), plus one full join per verb (four vertices and the
+// round-join arcs on both sides) and caps per sub-path.  A join inside a curve that is not
+// a fast-path join (a cusp) is not counted: such a path can outgrow its room, which the
+// compaction detects from the exact counts; it is then emitted again on its own.
+__global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, uint32_t* cap_v,
+                                         uint32_t* cap_i) {
+    const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
+    if (path >= n) return;
+    const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
+    const uint32_t vb = L.in.verb_off[path], ve = L.in.verb_off[path + 1];
+    const uint32_t pb = L.in.point_off[path], pe = L.in.point_off[path + 1];
+    const float* pts = L.in.points + 2ull * pb;
+    const uint32_t npts = pe - pb;
+    const float tol = o.tolerance;
+    uint64_t steps = 0, verbs = 0, subs = 0;
+    uint32_t p = 0;
+    vec2 cur = mk(0.0f, 0.0f);
+    for (uint32_t v = vb; v < ve; ++v) {
+        const uint8_t verb = L.in.verbs[v];
+        const uint32_t np = verb_points(verb);
+        vec2 q[3];
+        for (uint32_t k = 0; k < 3; ++k) q[k] = ldp(pts, npts, p + (k < np ? k : 0));
+        verbs += 1;
+        if (verb == LYG_VERB_BEGIN) {
+            subs += 1;
+            steps += 1;
+        } else if (verb == LYG_VERB_QUADRATIC) {
+            steps += (tol > 0.0f ? seg_count(quad_wang(cur, q[0], q[1], tol)) : 1u) + 2;
+        } else if (verb == LYG_VERB_CUBIC) {
+            steps += (tol > 0.0f ? seg_count(cubic_wang(cur, q[0], q[1], q[2], tol)) : 1u) + 1;
+        } else {
+            steps += 2;
+        }
+        if (np > 0) cur = q[np - 1];
+        p += np;
+    }
+    const bool round = o.line_join == LYG_JOIN_ROUND || o.start_cap == LYG_CAP_ROUND ||
+                       o.end_cap == LYG_CAP_ROUND;
+    const float hw = o.line_width * 0.5f;
+    uint64_t A = 0;
+    if (round && tol > 0.0f && hw > 0.0f)
+        A = arc_count(arc_subdivisions(2.0f * 3.14159265358979323846f, hw, tol));
+    const uint64_t cv = 2 * steps + verbs * (4 + 2 * A) + subs * (8 + 4 * A) + 64;
+    const uint64_t ct = 2 * steps + verbs * (6 + 2 * A) + subs * (10 + 4 * A) + 64;
+    cap_v[path] = (uint32_t)(cv < 0x7fffffffull ? cv : 0x7fffffffull);
+    cap_i[path] = (uint32_t)(3 * ct < 0x7fffffffull ? 3 * ct : 0x7fffffffull);
+}
+
+// One block per path (grid-stride over paths): copies the path's staged vertices and
+// indices (adding its id base) to the final offsets, or lists the path for a second emit
+// when its exact counts exceed its staging room.
+__global__ void __launch_bounds__(256) stroke_compact_kernel(StrokeCompact C) {
+    for (uint32_t p = blockIdx.x; p < C.n_paths; p += gridDim.x) {
+        const uint32_t nv = C.path_nv[p], ni = C.path_ni[p];
+        const uint32_t sv0 = C.stage_voff[p], si0 = C.stage_ioff[p];
+        if (nv > C.stage_voff[p + 1] - sv0 || ni > C.stage_ioff[p + 1] - si0) {
+            if (threadIdx.x == 0) C.fallback[atomicAdd(C.n_fallback, 1u)] = p;
+            continue;
+        }
+        const uint32_t v0 = C.vtx_off[p], i0 = C.idx_off[p];
+        const float2* sv = reinterpret_cast<const float2*>(C.stage_v) + sv0;
+        float2* dv = reinterpret_cast<float2*>(C.out_vertices) + v0;
+        for (uint32_t j = threadIdx.x; j < nv; j += blockDim.x) dv[j] = sv[j];
+        const uint32_t base = C.vertex_offset + (C.local_ids ? 0u : v0);
+        const uint32_t* si = C.stage_i + si0;
+        uint32_t* di = C.out_indices + i0;
+        for (uint32_t j = threadIdx.x; j < ni; j += blockDim.x) di[j] = si[j] + base;
+    }
+}
+
 }  // namespace lyg
 
 extern "C" {
 
 // Launch helpers used by the host runtime (lyg_api.cpp).
+hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, uint32_t* cap_v,
+                                        uint32_t* cap_i, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_stage_caps_kernel, dim3((n + 127) / 128), dim3(128), 0, s, *L, n,
+                       cap_v, cap_i);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s) {
+    if (C->n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_compact_kernel, dim3(grid), dim3(256), 0, s, *C);
+    return hipGetLastError();
+}
 hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hipStream_t stream) {
     dim3 block(lyg::WAVE * lyg::WAVES_PER_BLOCK);
     if (emit)
