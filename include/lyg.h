@@ -229,6 +229,11 @@ typedef struct lyg_tuning {
 /* Sets the launch shapes explicitly (either pointer may be null), e.g. a tuning kept from an
  * earlier run. */
 int32_t lyg_ctx_set_tuning(lyg_ctx* ctx, const lyg_tuning* stroke, const lyg_tuning* fill);
+/* How lyg_stroke_batch sizes its output (no lyon counterpart).  enabled = 1 (default): one
+ * emit pass into per-path staging room (room_scale times the computed room; a path that
+ * outgrows it is emitted again at its final place), then a compaction copy; enabled = 0:
+ * a count pass and an emit pass.  Both give the same output. */
+int32_t lyg_ctx_set_stroke_staging(lyg_ctx* ctx, int enabled, float room_scale);
 int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,

@@ -124,6 +124,7 @@ SIGNATURES = {
     "lyg_graph_result": ([C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_graph_destroy": ([C.c_void_p], None),
     "lyg_ctx_set_tuning": ([C.c_void_p, C.POINTER(TuningC), C.POINTER(TuningC)], C.c_int32),
+    "lyg_ctx_set_stroke_staging": ([C.c_void_p, C.c_int, C.c_float], C.c_int32),
     "lyg_autotune_stroke": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                              C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
     "lyg_autotune_fill": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,

@@ -276,3 +276,12 @@ def set_tuning(stroke_blocks_per_cu=None, fill_waves_per_block=None, fill_lds_by
     st = B.lib().lyg_ctx_set_tuning(_Ctx.get(), C.byref(s) if s else None, C.byref(f) if f else None)
     if st != 0:
         raise TessellationError(st)
+
+
+def set_stroke_staging(enabled: bool = True, room_scale: float = 1.0):
+    """How batched strokes size their output (lyg_ctx_set_stroke_staging): one emit pass
+    into per-path staging room then a compaction copy (default), or count + emit passes.
+    The output is the same either way."""
+    st = B.lib().lyg_ctx_set_stroke_staging(_Ctx.get(), 1 if enabled else 0, float(room_scale))
+    if st != 0:
+        raise TessellationError(st)

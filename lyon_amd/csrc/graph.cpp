@@ -158,6 +158,13 @@ int32_t lyg_ctx_set_tuning(lyg_ctx* c, const lyg_tuning* stroke, const lyg_tunin
     return LYG_OK;
 }
 
+int32_t lyg_ctx_set_stroke_staging(lyg_ctx* c, int enabled, float room_scale) {
+    if (!c || !(room_scale > 0.0f) || room_scale > 64.0f) return LYG_ERR_INVALID_ARGUMENT;
+    c->stroke_staged = enabled != 0;
+    c->stroke_room_scale = room_scale;
+    return LYG_OK;
+}
+
 int32_t lyg_autotune_stroke(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen) {
     if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;

@@ -18,8 +18,8 @@
 
 extern "C" {
 hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hipStream_t stream);
-hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, uint32_t* cap_v,
-                                        uint32_t* cap_i, hipStream_t s);
+hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, float room_scale,
+                                        uint32_t* cap_v, uint32_t* cap_i, hipStream_t s);
 hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s);
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
@@ -121,8 +121,8 @@ int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Ou
     // 1. staging room and its offsets
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
     if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
-        !check(c, lyg_launch_stroke_stage_caps(&L, n, as<uint32_t>(c->st_cap_v),
-                                               as<uint32_t>(c->st_cap_i), s),
+        !check(c, lyg_launch_stroke_stage_caps(&L, n, c->stroke_room_scale,
+                                               as<uint32_t>(c->st_cap_v), as<uint32_t>(c->st_cap_i), s),
                "stroke stage caps") ||
         !check(c, lyg_launch_scan(as<uint32_t>(c->st_cap_v), n, as<uint32_t>(c->st_off_v),
                                   c->d_totals + 0, c->d_counters + 4, s),

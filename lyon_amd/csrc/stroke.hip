@@ -725,8 +725,8 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
 // round-join arcs on both sides) and caps per sub-path.  A join inside a curve that is not
 // a fast-path join (a cusp) is not counted: such a path can outgrow its room, which the
 // compaction detects from the exact counts; it is then emitted again on its own.
-__global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, uint32_t* cap_v,
-                                         uint32_t* cap_i) {
+__global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, float room_scale,
+                                         uint32_t* cap_v, uint32_t* cap_i) {
     const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
     if (path >= n) return;
     const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
@@ -763,8 +763,11 @@ __global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, uint32_t* c
     uint64_t A = 0;
     if (round && tol > 0.0f && hw > 0.0f)
         A = arc_count(arc_subdivisions(2.0f * 3.14159265358979323846f, hw, tol));
-    const uint64_t cv = 2 * steps + verbs * (4 + 2 * A) + subs * (8 + 4 * A) + 64;
-    const uint64_t ct = 2 * steps + verbs * (6 + 2 * A) + subs * (10 + 4 * A) + 64;
+    const double f = room_scale > 0.0f ? (double)room_scale : 1.0;
+    const uint64_t cv =
+        (uint64_t)(f * (double)(2 * steps + verbs * (4 + 2 * A) + subs * (8 + 4 * A) + 64));
+    const uint64_t ct =
+        (uint64_t)(f * (double)(2 * steps + verbs * (6 + 2 * A) + subs * (10 + 4 * A) + 64));
     cap_v[path] = (uint32_t)(cv < 0x7fffffffull ? cv : 0x7fffffffull);
     cap_i[path] = (uint32_t)(3 * ct < 0x7fffffffull ? 3 * ct : 0x7fffffffull);
 }
@@ -796,11 +799,11 @@ __global__ void __launch_bounds__(256) stroke_compact_kernel(StrokeCompact C) {
 extern "C" {
 
 // Launch helpers used by the host runtime (lyg_api.cpp).
-hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, uint32_t* cap_v,
-                                        uint32_t* cap_i, hipStream_t s) {
+hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, float room_scale,
+                                        uint32_t* cap_v, uint32_t* cap_i, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(lyg::stroke_stage_caps_kernel, dim3((n + 127) / 128), dim3(128), 0, s, *L, n,
-                       cap_v, cap_i);
+                       room_scale, cap_v, cap_i);
     return hipGetLastError();
 }
 hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s) {

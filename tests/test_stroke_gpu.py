@@ -158,3 +158,44 @@ def test_batch_matches_sequential_plain_calls():
         g = res.path(i)
         assert_same_geometry(g, r, max(bbox_diag(p.points), so.line_width),
                              what=f"batch path {i}")
+
+
+def test_batch_sizing_modes_agree():
+    """The batched stroke's three sizing modes give the same bytes: one emit pass into
+    per-path staging (default), the same with staging room too small for most paths (they
+    are emitted again at their final place), and separate count and emit passes."""
+    from lyon_amd.batch import DeviceBatch, StrokeBatcher, set_stroke_staging
+    batch = workloads.stroke_batch(400, 4, 128, 1024.0, seed=5)
+    opts = workloads.stroke_options(400, tolerance=0.05, seed=5)
+    db = DeviceBatch(batch, opts)
+
+    def run(enabled, room):
+        set_stroke_staging(enabled, room)
+        r = StrokeBatcher().run(db, vertex_offset=7)
+        return (r.vertex_offsets, r.index_offsets, r.all_vertices().view(np.uint32),
+                r.all_indices())
+
+    try:
+        staged = run(True, 1.0)
+        squeezed = run(True, 0.05)
+        two_pass = run(False, 1.0)
+    finally:
+        set_stroke_staging(True, 1.0)
+    for name, other in (("room 0.05", squeezed), ("count + emit", two_pass)):
+        for a, b, what in zip(staged, other, ("vertex offsets", "index offsets", "vertices",
+                                               "indices")):
+            assert np.array_equal(a, b), f"{name}: {what} differ from the staged run"
+    # and the squeezed run (every path emitted twice) against the oracle on a sample
+    vo = squeezed[0]
+    for i in range(0, len(batch), 37):
+        o = opts[i]
+        so = StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
+                           tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
+                           end_cap=int(o["end_cap"]), line_join=int(o["line_join"]))
+        p = batch.path(i)
+        r = ref_stroke(p, so, vertex_offset=7 + int(vo[i]))
+        v = squeezed[2].view(np.float32).reshape(-1, 2)[vo[i]:vo[i + 1]]
+        io = squeezed[1]
+        idx = squeezed[3][io[i]:io[i + 1]]
+        assert_same_geometry((v, idx), r, max(bbox_diag(p.points), so.line_width),
+                             what=f"squeezed batch path {i}")
