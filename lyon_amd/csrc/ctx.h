@@ -40,6 +40,7 @@ struct lyg_ctx {
     // stroke: staged single pass (emit into per-path staging room, then compact)
     bool stroke_staged = true;
     float stroke_room_scale = 1.0f;  // multiplies every path's staging room
+    DevBuf nsub_thr;                 // round-join subdivision thresholds (31 floats)
     DevBuf st_cap_v, st_cap_i, st_off_v, st_off_i, st_v, st_i, st_fallback;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};

@@ -42,6 +42,7 @@ bool capture_stroke(lyg_ctx* c, lyg_graph* g, const lyg_batch_in* in,
     const uint32_t n = in->n_paths;
     hipStream_t s = c->stream;
     lyg::StrokeLaunch L{};
+    L.nsub_thr = as<float>(c->nsub_thr);
     L.in.verbs = in->verbs;
     L.in.points = in->points;
     L.in.verb_off = in->verb_offsets;

@@ -21,6 +21,7 @@ hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hip
 hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, float room_scale,
                                         uint32_t* cap_v, uint32_t* cap_i, hipStream_t s);
 hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s);
+hipError_t lyg_launch_nsub_thresholds(float* thr, hipStream_t s);
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
 hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
@@ -245,6 +246,7 @@ int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->counts_i, 4ull * n))
         return LYG_ERR_DEVICE;
     lyg::StrokeLaunch L{};
+    L.nsub_thr = as<float>(c->nsub_thr);
     L.in = b.in;
     L.opts = reinterpret_cast<const lyg_stroke_options*>(b.opts);
     L.opts_stride = b.opts_stride;
@@ -314,6 +316,13 @@ lyg_ctx* lyg_ctx_create(int device) {
         lyg_ctx_destroy(c);
         return nullptr;
     }
+    // round-join subdivision thresholds, once per context
+    if (!ensure(c, c->nsub_thr, 32 * sizeof(float)) ||
+        lyg_launch_nsub_thresholds(as<float>(c->nsub_thr), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        lyg_ctx_destroy(c);
+        return nullptr;
+    }
     return c;
 }
 
@@ -331,6 +340,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->f_hint);
     free_buf(c->f_work);
     free_buf(c->f_inter);
+    free_buf(c->nsub_thr);
     for (DevBuf* b : {&c->st_cap_v, &c->st_cap_i, &c->st_off_v, &c->st_off_i, &c->st_v, &c->st_i,
                       &c->st_fallback})
         free_buf(*b);
@@ -470,6 +480,7 @@ int32_t lyg_stroke_path(lyg_ctx* c, const lyg_path_view* path, const lyg_stroke_
         !check(c, hipMemcpyAsync(c->in_opts.p, opts, sizeof(*opts), hipMemcpyHostToDevice, s), "H2D"))
         return LYG_ERR_DEVICE;
     lyg::StrokeLaunch L{};
+    L.nsub_thr = as<float>(c->nsub_thr);
     L.in.verbs = as<uint8_t>(c->in_verbs);
     L.in.points = as<float>(c->in_points);
     L.in.verb_off = as<uint32_t>(c->in_offs);
@@ -524,6 +535,7 @@ int32_t lyg_stroke_batch_count(lyg_ctx* c, const lyg_batch_in* in, const lyg_str
     }
     if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
     lyg::StrokeLaunch L{};
+    L.nsub_thr = as<float>(c->nsub_thr);
     L.in.verbs = in->verbs;
     L.in.points = in->points;
     L.in.verb_off = in->verb_offsets;

@@ -137,22 +137,27 @@ LYG_HD uint64_t tchain_round(uint64_t S) {
     if (r > half || (r == half && (A & 1ull))) A += 1;
     return A << b;
 }
-LYG_HD uint64_t tchain_units(uint32_t m, uint32_t j) {
-    uint64_t T = 0;
-    uint32_t k = 0;
+// T after j steps, continuing from a known state: T0 = T after k0 steps (0, 0 from the
+// start).  The loop only reads (T, k), so any earlier state of the same chain is a valid
+// start.
+LYG_HD uint64_t tchain_units_from(uint32_t m, uint32_t j, uint32_t k0, uint64_t T0) {
+    uint64_t T = T0;
+    uint32_t k = k0;
     while (k < j) {
         uint64_t S = T + m;
         int b = bitlen64(S) - 24;
         if (b <= 0) {
-            // Exact region: all sums below 2^24 units are representable.
-            uint64_t lim = (1ull << 24) - 1;  // largest exact sum
+            // Exact region: all sums below 2^24 units are representable.  Here S < 2^24,
+            // so the step count is a 32-bit division (64-bit division is a long
+            // software routine on the GPU).
+            const uint32_t lim = (1u << 24) - 1;  // largest exact sum
             // number of further exact steps: T + c*m <= lim
-            uint64_t c = (lim - T) / m;
+            uint32_t c = (lim - (uint32_t)T) / m;
             if (c == 0) c = 1;
-            uint64_t left = j - k;
+            const uint32_t left = j - k;
             if (c > left) c = left;
-            T += c * m;
-            k += (uint32_t)c;
+            T += (uint64_t)c * m;
+            k += c;
             continue;
         }
         uint64_t g = 1ull << b;
@@ -175,20 +180,22 @@ LYG_HD uint64_t tchain_units(uint32_t m, uint32_t j) {
         // Steps whose sum stays below the binade end 2^(24+b): T + i*inc + m < lim_end.
         uint64_t lim_end = 1ull << (24 + b);
         uint64_t room = lim_end - 1 - S;  // S = T + m < lim_end
-        uint64_t c = room / inc + 1;
-        uint64_t left = j - k;
+        // room / inc = (room >> b) / inc_units (floor of floors), with room < 2^(23+b)
+        // (S >= 2^(23+b)) and inc_units < 2^24: a 32-bit division.
+        uint32_t c = (uint32_t)(room >> b) / (uint32_t)inc_units + 1;
+        const uint32_t left = j - k;
         if (c > left) c = left;
-        T += c * inc;
-        k += (uint32_t)c;
+        T += (uint64_t)c * inc;
+        k += c;
     }
     return T;
 }
-LYG_HD float tchain(float s, uint32_t j) {
-    if (j == 0) return 0.0f;
-    uint32_t bits = f2u(s);
-    uint32_t e = (bits >> 23) & 0xffu;
-    uint32_t m = (bits & 0x7fffffu) | 0x800000u;
-    uint64_t T = tchain_units(m, j);
+LYG_HD uint64_t tchain_units(uint32_t m, uint32_t j) { return tchain_units_from(m, j, 0, 0); }
+// The 24-bit significand m of s (s = m * ulp(s)).
+LYG_HD uint32_t tchain_mant(float s) { return (f2u(s) & 0x7fffffu) | 0x800000u; }
+// The f32 value T * ulp(s).
+LYG_HD float tchain_value(float s, uint64_t T) {
+    uint32_t e = (f2u(s) >> 23) & 0xffu;
     // value = T * 2^(e - 150); T < 2^(24 + b) is a multiple of 2^b, so T >> b fits in 24
     // bits and the conversion below is exact.
     int b = bitlen64(T) - 24;
@@ -202,6 +209,10 @@ LYG_HD float tchain(float s, uint32_t j) {
     float s1 = u2f((uint32_t)(e1 + 127) << 23);
     float s2 = u2f((uint32_t)(e2 + 127) << 23);
     return (fa * s1) * s2;
+}
+LYG_HD float tchain(float s, uint32_t j) {
+    if (j == 0) return 0.0f;
+    return tchain_value(s, tchain_units(tchain_mant(s), j));
 }
 
 }  // namespace lyg

@@ -38,6 +38,7 @@ struct StrokeLaunch {
     uint64_t index_capacity;
     const uint32_t* path_list;  // optional: the launch walks path_list[0 .. n_paths)
     uint32_t store_counts;      // emit pass also stores per-path counts (staged emit)
+    const float* nsub_thr;      // round-join subdivision thresholds (31 floats) or null
 };
 
 // Staged single-pass stroke: copy of every path's staged output to its final place.

@@ -38,6 +38,7 @@ enum : uint32_t {
     K_END = 5u,
 };
 enum : uint32_t { F_BEGIN = 1u, F_END = 2u, F_CLOSE = 4u, F_FLAT = 8u };
+static constexpr uint32_t NO_CHAIN = 0xffffffffu;
 
 struct VRec {
     vec2 a0, a1, a2, a3;  // curve A polynomial form (quadratic: a3 unused)
@@ -240,7 +241,7 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
     const float* pts = L.in.points + 2ull * pb;
     const uint32_t nverbs = ve - vb, npts = pe - pb;
     const lyg_stroke_options o = L.opts[L.opts_stride ? gpath : 0];
-    const SParams P = make_params(o);
+    const SParams P = make_params(o, L.nsub_thr);
 
     uint32_t path_v0 = 0, path_i0 = 0, path_v_end = 0, path_i_end = 0;
     if (EMIT) {
@@ -302,11 +303,16 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
         wave_sync();
 
         // ---- 2. windows of 64 steps ----------------------------------------------------
+        // t chain of the curve half the previous window ended in (key, steps, units)
+        uint32_t ch_key = NO_CHAIN, ch_j = 0;
+        uint64_t ch_T = 0;
         for (uint32_t w0 = 0; w0 < nsteps; w0 += WAVE) {
             const uint32_t g = w0 + (uint32_t)l;
             const bool valid = g < nsteps;
             vec2 pos = mk(nan, nan);
             uint32_t flags = 0;
+            uint32_t my_key = NO_CHAIN, my_j = 0;  // the t chain this lane evaluated
+            uint64_t my_T = 0;
             if (valid) {
                 // last verb whose offset <= g
                 int lo = 0, hi = (int)nch - 1;
@@ -326,22 +332,25 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                 } else if (kind == K_END) {
                     flags = F_END | ((vr.kind >> 8) ? F_CLOSE : 0u);
                 } else {
-                    bool cubic = kind == K_CUBIC;
-                    if (k < vr.nA) {
-                        if (k + 1 == vr.nA) {
-                            pos = vr.toA;
-                        } else {
-                            pos = horner(vr, false, cubic, tchain(vr.sA, k + 1));
-                            flags = F_FLAT;
-                        }
+                    const bool cubic = kind == K_CUBIC;
+                    const bool inA = k < vr.nA;
+                    const uint32_t kk = inA ? k : k - vr.nA;
+                    if (kk + 1 == (inA ? vr.nA : vr.nB)) {
+                        pos = inA ? vr.toA : vr.toB;
                     } else {
-                        uint32_t k2 = k - vr.nA;
-                        if (k2 + 1 == vr.nB) {
-                            pos = vr.toB;
-                        } else {
-                            pos = horner(vr, true, false, tchain(vr.sB, k2 + 1));
-                            flags = F_FLAT;
-                        }
+                        // t after kk + 1 steps, continuing the previous window's chain
+                        // when this curve half carries on from it
+                        const float sstep = inA ? vr.sA : vr.sB;
+                        const uint32_t key = ((uint32_t)lo << 1) | (inA ? 0u : 1u);
+                        const uint32_t j = kk + 1;
+                        const bool cont = key == ch_key && ch_j < j;
+                        const uint64_t T = tchain_units_from(tchain_mant(sstep), j,
+                                                             cont ? ch_j : 0u, cont ? ch_T : 0ull);
+                        pos = horner(vr, !inA, inA && cubic, tchain_value(sstep, T));
+                        flags = F_FLAT;
+                        my_key = key;
+                        my_j = j;
+                        my_T = T;
                     }
                 }
             }
@@ -644,6 +653,14 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                     newcnt = c.sp_cnt + __builtin_popcountll(keptm);
                     newm = last_mnec;
                 }
+                // the t chain of the last step, for the next window
+                {
+                    const int src = lastv < 0 ? 0 : lastv;
+                    ch_key = lastv < 0 ? NO_CHAIN : readlane_u(my_key, src);
+                    ch_j = readlane_u(my_j, src);
+                    ch_T = (uint64_t)readlane_u((uint32_t)my_T, src) |
+                           ((uint64_t)readlane_u((uint32_t)(my_T >> 32), src) << 32);
+                }
                 if (lk >= 0) c.last_kept = lkp;
                 if (last_is_end) c.last_kept = mk(nan, nan);
                 // firsts of the (possibly new) current sub-path
@@ -717,6 +734,21 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
         }
         pid = next_path(L.work_counter);
     }
+}
+
+// thr[k], k = 0..30: the smallest float x in [2^k, 2^(k+1)) whose round(log2(x)) is k + 1
+// (2^(k+1) if none), by bisection over the binade's bit patterns with the log2 the
+// subdivision count uses; the count of x in that binade is then k + (x >= thr[k]).
+__global__ void nsub_thresholds_kernel(float* thr) {
+    const int k = (int)threadIdx.x;
+    if (k > 30) return;
+    uint32_t lo = (uint32_t)(127 + k) << 23, hi = (uint32_t)(128 + k) << 23;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (__builtin_roundf(t_log2(u2f(mid))) >= (float)(k + 1)) hi = mid;
+        else lo = mid + 1;
+    }
+    thr[k] = u2f(lo);
 }
 
 // ---- staged single pass -----------------------------------------------------------------
@@ -804,6 +836,10 @@ hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, 
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(lyg::stroke_stage_caps_kernel, dim3((n + 127) / 128), dim3(128), 0, s, *L, n,
                        room_scale, cap_v, cap_i);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_nsub_thresholds(float* thr, hipStream_t s) {
+    hipLaunchKernelGGL(lyg::nsub_thresholds_kernel, dim3(1), dim3(64), 0, s, thr);
     return hipGetLastError();
 }
 hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s) {

@@ -36,13 +36,21 @@ __device__ __forceinline__ float fast_atan2(float y, float x) {
     return r;
 }
 __device__ __forceinline__ float angle_of(vec2 v) { return fast_atan2(v.y, v.x); }
+// fmodf(x, y) for y > 0: exact results without the general reduction when |x| < 2y
+// (|x| < y: x itself; y < |x| < 2y: x - y with x's sign, exact by Sterbenz's lemma).
+__device__ __forceinline__ float fmod_near(float x, float y) {
+    const float ax = ffabs(x);
+    if (ax < y) return x;
+    if (ax > y && ax < 2.0f * y) return x - __builtin_copysignf(y, x);
+    return fmodf(x, y);
+}
 // euclid Angle::angle_to
 __device__ __forceinline__ float angle_to(float from, float to) {
     const float PI = 3.14159265358979323846f;
     float two = 1.0f + 1.0f;
     float mx = PI * two;
-    float d = fmodf(to - from, mx);
-    return fmodf(two * d, mx) - d;
+    float d = fmod_near(to - from, mx);
+    return fmod_near(two * d, mx) - d;
 }
 // euclid Rotation2D::transform_vector
 __device__ __forceinline__ vec2 rotate_v(float angle, vec2 p) {
@@ -116,8 +124,12 @@ __device__ __forceinline__ float find_sharp_turn(vec2 from, vec2 ctrl, vec2 to) 
 struct SParams {
     float hw, tol, miter_limit, thr;
     int start_cap, end_cap, join;
+    float arc_step;          // circle flattening step of every round join and cap of the path
+    const float* nsub_thr;   // subdivision thresholds (nsub_thresholds_kernel) or null
 };
-__device__ __forceinline__ SParams make_params(const lyg_stroke_options& o) {
+__device__ __forceinline__ float circle_step(float radius, float tol);
+__device__ __forceinline__ SParams make_params(const lyg_stroke_options& o,
+                                               const float* nsub_thr = nullptr) {
     SParams p;
     p.hw = o.line_width * 0.5f;
     p.tol = o.tolerance;
@@ -128,6 +140,11 @@ __device__ __forceinline__ SParams make_params(const lyg_stroke_options& o) {
     p.start_cap = o.start_cap;
     p.end_cap = o.end_cap;
     p.join = o.line_join;
+    // every arc of a path has radius half_width: its flattening step is a per-path constant
+    const bool round = o.line_join == LYG_JOIN_ROUND || o.start_cap == LYG_CAP_ROUND ||
+                       o.end_cap == LYG_CAP_ROUND;
+    p.arc_step = round ? circle_step(p.hw, p.tol) : 0.0f;
+    p.nsub_thr = nsub_thr;
     return p;
 }
 
@@ -255,15 +272,35 @@ __device__ __forceinline__ void add_edge_triangles(const EPD& p0, const EPD& p1,
     if (p0_neg != p1_neg && p1_pos != p1_neg) sink_tri(s, p0_neg, p1_pos, p1_neg);
 }
 
-// stroke.rs:2749-2753 and the subdivision count of stroke.rs:2002-2004 / 2488-2490.
-__device__ __forceinline__ uint32_t arc_subdivisions(float diff, float radius, float tol) {
+// circle_flattening_step (stroke.rs:2749-2753).
+__device__ __forceinline__ float circle_step(float radius, float tol) {
     float t = ffmin(tol, radius);
-    float step = 2.0f * t_acos((radius - t) / radius);
-    float num_segments = __builtin_ceilf(ffabs(diff) / step);
-    float r = __builtin_roundf(t_log2(num_segments));
+    return 2.0f * t_acos((radius - t) / radius);
+}
+// round(log2(x)) as the subdivision count of stroke.rs:2002-2004 / 2488-2490 (0 when not
+// positive, saturating).
+__device__ __forceinline__ uint32_t nsub_log2(float x) {
+    float r = __builtin_roundf(t_log2(x));
     if (!(r > 0.0f)) return 0u;
     if (r >= 4294967296.0f) return 0xffffffffu;
     return (uint32_t)r;
+}
+// Subdivision count of an arc of `diff` radians with radius and tolerance as given.
+__device__ __forceinline__ uint32_t arc_subdivisions(float diff, float radius, float tol) {
+    float num_segments = __builtin_ceilf(ffabs(diff) / circle_step(radius, tol));
+    return nsub_log2(num_segments);
+}
+// The same for an arc of the path's own radius (half_width), from the per-path step and,
+// when present, the threshold table: round(log2(x)) for x in [2^k, 2^(k+1)) is k or k + 1
+// and grows with x, so it is k + (x >= thr[k]), where thr[k] was found by bisection with
+// the very same log2 (nsub_thresholds_kernel).
+__device__ __forceinline__ uint32_t path_arc_subdivisions(float diff, const SParams& p) {
+    const float x = __builtin_ceilf(ffabs(diff) / p.arc_step);
+    if (p.nsub_thr && x >= 1.0f && x < 2147483648.0f) {
+        const int k = (int)((f2u(x) >> 23) & 0xffu) - 127;
+        return (uint32_t)k + (x >= p.nsub_thr[k] ? 1u : 0u);
+    }
+    return nsub_log2(x);
 }
 
 // tessellate_arc (stroke.rs:2612-2653): pre-order recursion made iterative.
@@ -316,7 +353,7 @@ __device__ __forceinline__ RoundArc round_join_arc(const EPD& j, int side, const
     r.swap = side == SN;
     r.a0 = r.swap ? end_angle : start_angle;
     r.a1 = r.swap ? start_angle : end_angle;
-    r.nsub = arc_subdivisions(diff, p.hw, p.tol);
+    r.nsub = path_arc_subdivisions(diff, p);
     return r;
 }
 
@@ -613,7 +650,8 @@ __device__ __forceinline__ void round_cap(vec2 center, float radius, vec2 start_
     float diff = angle_to(start_angle, angle_of(edge_normal));
     float mid_angle = start_angle + diff;
     float end_angle = mid_angle + diff;
-    uint32_t nsub = arc_subdivisions(diff, radius, p.tol);
+    // radius is half_width at every call (caps of the path): the per-path step applies
+    uint32_t nsub = radius == p.hw ? path_arc_subdivisions(diff, p) : arc_subdivisions(diff, radius, p.tol);
     uint32_t mid = sink_sv(s, center, normalize(edge_normal), radius);
     sink_tri(s, start_v, mid, end_v);
     tessellate_arc(start_angle, mid_angle, start_v, mid, nsub, center, radius, s);

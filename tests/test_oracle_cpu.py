@@ -242,7 +242,7 @@ def test_arc_to_quadratic_counts():
 
 def test_tchain_matches_serial_accumulation():
     """The closed-form t chain used by the kernels equals the serial f32 sum (exhaustive
-    over j for 2000 segment counts)."""
+    over j for 2000 segment counts), also when resumed from an intermediate state."""
     import subprocess
     import sys
     src = os.path.join(os.path.dirname(HERE), "lyon_amd", "csrc", "lyg_math.h")
@@ -256,6 +256,15 @@ int main() {
   for (int k = 0; k < 2000; ++k) {
     float n = k < 300 ? (float)(k + 1) : u(rng); float s = 1.0f / n; float t = 0;
     for (uint32_t j = 1; j <= seg_count(n) + 2; ++j) { t = t + s; tot++; if (f2u(tchain(s, j)) != f2u(t)) bad++; }
+    // continuing a chain from any earlier state (the kernels resume where the previous
+    // window of steps stopped) gives the same units
+    const uint32_t m = tchain_mant(s), nj = seg_count(n) + 2;
+    for (uint32_t k0 = 0; k0 <= nj; k0 += 1 + k0 / 3) {
+      const uint64_t T0 = tchain_units(m, k0);
+      for (uint32_t j = k0; j <= nj; j += 1 + (j - k0) / 5) {
+        tot++; if (tchain_units_from(m, j, k0, T0) != tchain_units(m, j)) bad++;
+      }
+    }
   }
   printf("%%ld %%ld\n", tot, bad); return bad != 0;
 }''' % src
