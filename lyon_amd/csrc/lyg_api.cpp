@@ -197,7 +197,7 @@ int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Ou
     C.n_fallback = n_fb;
     uint32_t* hf = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
     if (!check(c, lyg_launch_fill_u32(n_fb, 1, 0, s), "reset") ||
-        !check(c, lyg_launch_stroke_compact(&C, (int)std::min<uint32_t>(n, 8u * c->cu_count), s),
+        !check(c, lyg_launch_stroke_compact(&C, (int)std::min<uint32_t>(n, 1u << 20), s),
                "stroke compact") ||
         !check(c, hipMemcpyAsync(hf, n_fb, 4, hipMemcpyDeviceToHost, s), "copy") ||
         !check(c, hipStreamSynchronize(s), "sync"))

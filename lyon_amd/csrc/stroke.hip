@@ -757,38 +757,49 @@ __global__ void nsub_thresholds_kernel(float* thr) {
 // round-join arcs on both sides) and caps per sub-path.  A join inside a curve that is not
 // a fast-path join (a cusp) is not counted: such a path can outgrow its room, which the
 // compaction detects from the exact counts; it is then emitted again on its own.
-__global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, float room_scale,
-                                         uint32_t* cap_v, uint32_t* cap_i) {
-    const uint32_t path = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(WAVE) stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n,
+                                                                 float room_scale,
+                                                                 uint32_t* cap_v, uint32_t* cap_i) {
+    // one wave per path, one lane per verb
+    const uint32_t path = blockIdx.x;
     if (path >= n) return;
+    const int l = lane_id();
     const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
     const uint32_t vb = L.in.verb_off[path], ve = L.in.verb_off[path + 1];
     const uint32_t pb = L.in.point_off[path], pe = L.in.point_off[path + 1];
     const float* pts = L.in.points + 2ull * pb;
     const uint32_t npts = pe - pb;
     const float tol = o.tolerance;
-    uint64_t steps = 0, verbs = 0, subs = 0;
-    uint32_t p = 0;
-    vec2 cur = mk(0.0f, 0.0f);
-    for (uint32_t v = vb; v < ve; ++v) {
-        const uint8_t verb = L.in.verbs[v];
-        const uint32_t np = verb_points(verb);
-        vec2 q[3];
-        for (uint32_t k = 0; k < 3; ++k) q[k] = ldp(pts, npts, p + (k < np ? k : 0));
-        verbs += 1;
-        if (verb == LYG_VERB_BEGIN) {
-            subs += 1;
-            steps += 1;
-        } else if (verb == LYG_VERB_QUADRATIC) {
-            steps += (tol > 0.0f ? seg_count(quad_wang(cur, q[0], q[1], tol)) : 1u) + 2;
-        } else if (verb == LYG_VERB_CUBIC) {
-            steps += (tol > 0.0f ? seg_count(cubic_wang(cur, q[0], q[1], q[2], tol)) : 1u) + 1;
-        } else {
-            steps += 2;
+    uint32_t steps = 0, subs = 0, pt_carry = 0;
+    for (uint32_t v0 = vb; v0 < ve; v0 += WAVE) {
+        const uint32_t v = v0 + (uint32_t)l;
+        const uint8_t verb = v < ve ? L.in.verbs[v] : (uint8_t)LYG_VERB_END;
+        const uint32_t np = v < ve ? verb_points(verb) : 0u;
+        const uint32_t incl = incl_scan(np);
+        const uint32_t first = pt_carry + incl - np;  // this verb's first point
+        pt_carry += readlane_u(incl, WAVE - 1);
+        if (v < ve) {
+            // the current point is the point stored before the verb's own points
+            const vec2 cur = ldp(pts, npts, first > 0 ? first - 1 : 0);
+            const vec2 a = ldp(pts, npts, first), b = ldp(pts, npts, first + 1),
+                       c = ldp(pts, npts, first + 2);
+            uint32_t add = 2;
+            if (verb == LYG_VERB_BEGIN) {
+                subs += 1;
+                add = 1;
+            } else if (verb == LYG_VERB_QUADRATIC) {
+                add = (tol > 0.0f ? min(seg_count(quad_wang(cur, a, b, tol)), 1u << 24) : 1u) + 2;
+            } else if (verb == LYG_VERB_CUBIC) {
+                add = (tol > 0.0f ? min(seg_count(cubic_wang(cur, a, b, c, tol)), 1u << 24) : 1u) + 1;
+            }
+            steps = min(steps + add, 1u << 25);  // 64 lanes' sum stays below 2^32
         }
-        if (np > 0) cur = q[np - 1];
-        p += np;
     }
+    // wave sums
+    const uint64_t verbs = ve - vb;
+    const uint64_t steps_all = readlane_u(incl_scan(steps), WAVE - 1);
+    const uint64_t subs_all = readlane_u(incl_scan(subs), WAVE - 1);
+    if (l != 0) return;
     const bool round = o.line_join == LYG_JOIN_ROUND || o.start_cap == LYG_CAP_ROUND ||
                        o.end_cap == LYG_CAP_ROUND;
     const float hw = o.line_width * 0.5f;
@@ -796,15 +807,15 @@ __global__ void stroke_stage_caps_kernel(StrokeLaunch L, uint32_t n, float room_
     if (round && tol > 0.0f && hw > 0.0f)
         A = arc_count(arc_subdivisions(2.0f * 3.14159265358979323846f, hw, tol));
     const double f = room_scale > 0.0f ? (double)room_scale : 1.0;
-    const uint64_t cv =
-        (uint64_t)(f * (double)(2 * steps + verbs * (4 + 2 * A) + subs * (8 + 4 * A) + 64));
-    const uint64_t ct =
-        (uint64_t)(f * (double)(2 * steps + verbs * (6 + 2 * A) + subs * (10 + 4 * A) + 64));
+    const uint64_t cv = (uint64_t)(f * (double)(2 * steps_all + verbs * (4 + 2 * A) +
+                                                subs_all * (8 + 4 * A) + 64));
+    const uint64_t ct = (uint64_t)(f * (double)(2 * steps_all + verbs * (6 + 2 * A) +
+                                                subs_all * (10 + 4 * A) + 64));
     cap_v[path] = (uint32_t)(cv < 0x7fffffffull ? cv : 0x7fffffffull);
     cap_i[path] = (uint32_t)(3 * ct < 0x7fffffffull ? 3 * ct : 0x7fffffffull);
 }
 
-// One block per path (grid-stride over paths): copies the path's staged vertices and
+// One block of 256 threads per path (grid-stride over paths): copies the path's staged vertices and
 // indices (adding its id base) to the final offsets, or lists the path for a second emit
 // when its exact counts exceed its staging room.
 __global__ void __launch_bounds__(256) stroke_compact_kernel(StrokeCompact C) {
@@ -816,13 +827,30 @@ __global__ void __launch_bounds__(256) stroke_compact_kernel(StrokeCompact C) {
             continue;
         }
         const uint32_t v0 = C.vtx_off[p], i0 = C.idx_off[p];
-        const float2* sv = reinterpret_cast<const float2*>(C.stage_v) + sv0;
-        float2* dv = reinterpret_cast<float2*>(C.out_vertices) + v0;
-        for (uint32_t j = threadIdx.x; j < nv; j += blockDim.x) dv[j] = sv[j];
+        const uint2* __restrict__ sv = reinterpret_cast<const uint2*>(C.stage_v) + sv0;
+        uint2* __restrict__ dv = reinterpret_cast<uint2*>(C.out_vertices) + v0;
+        // four loads in flight per thread, then the four stores
+        uint32_t j = threadIdx.x;
+        for (; j + 768 < nv; j += 1024) {
+            const uint2 a = sv[j], b = sv[j + 256], c = sv[j + 512], d = sv[j + 768];
+            dv[j] = a;
+            dv[j + 256] = b;
+            dv[j + 512] = c;
+            dv[j + 768] = d;
+        }
+        for (; j < nv; j += 256) dv[j] = sv[j];
         const uint32_t base = C.vertex_offset + (C.local_ids ? 0u : v0);
-        const uint32_t* si = C.stage_i + si0;
-        uint32_t* di = C.out_indices + i0;
-        for (uint32_t j = threadIdx.x; j < ni; j += blockDim.x) di[j] = si[j] + base;
+        const uint32_t* __restrict__ si = C.stage_i + si0;
+        uint32_t* __restrict__ di = C.out_indices + i0;
+        j = threadIdx.x;
+        for (; j + 768 < ni; j += 1024) {
+            const uint32_t a = si[j], b = si[j + 256], c = si[j + 512], d = si[j + 768];
+            di[j] = a + base;
+            di[j + 256] = b + base;
+            di[j + 512] = c + base;
+            di[j + 768] = d + base;
+        }
+        for (; j < ni; j += 256) di[j] = si[j] + base;
     }
 }
 
@@ -834,7 +862,7 @@ extern "C" {
 hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, float room_scale,
                                         uint32_t* cap_v, uint32_t* cap_i, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(lyg::stroke_stage_caps_kernel, dim3((n + 127) / 128), dim3(128), 0, s, *L, n,
+    hipLaunchKernelGGL(lyg::stroke_stage_caps_kernel, dim3(n), dim3(lyg::WAVE), 0, s, *L, n,
                        room_scale, cap_v, cap_i);
     return hipGetLastError();
 }
