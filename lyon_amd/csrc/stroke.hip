@@ -232,6 +232,7 @@ __device__ __forceinline__ uint32_t verb_points(uint8_t v) {
 // Walks one path with the whole wave.  Returns (vertices, triangles) in the count pass.
 template <bool EMIT>
 __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs,
+                                ArcJob* arc_jobs, uint32_t* arc_first, uint32_t* arc_c0,
                                 uint32_t* out_nv, uint32_t* out_ni) {
     const int l = lane_id();
     const uint32_t gpath = path;
@@ -558,10 +559,11 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                 const uint64_t islot = (uint64_t)path_i0 + 3ull * my_tbase;
                 se.iout = L.out_indices + islot;
                 se.ileft = islot < path_i_end ? (uint32_t)(path_i_end - islot) : 0u;
+                ArcJobs aj;  // round-join arcs, emitted by the whole wave below
                 if (is_join) {
                     EPD pj;
                     epd_from_pub(pj, PP, prv.next[0], prv.next[1], prv.code, prv.base, P.join);
-                    join_emit(J, pj, isp >= 3, P, se);
+                    join_emit(J, pj, isp >= 3, P, se, &aj);
                 }
                 if (item && isE) {
                     Sink s2 = se;
@@ -570,6 +572,7 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                     end_subpath(e2, P2, s2);
                     se = s2;
                 }
+                emit_arcs(aj, se, P.hw, arc_jobs, arc_first, arc_c0);
             }
 
             // ---- 5. carry to the next window ----
@@ -715,14 +718,21 @@ template <bool EMIT>
 __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     LYG_WAVES_PER_EU(EMIT ? 2 : 4) stroke_kernel(StrokeLaunch L) {
     __shared__ VRec recs_all[WAVES_PER_BLOCK][WAVE];
+    // the emit pass's cooperative arcs: two jobs per lane, first node and side split
+    constexpr int AW = EMIT ? WAVES_PER_BLOCK : 1;
+    __shared__ ArcJob arcs_all[AW][EMIT ? 2 * WAVE : 1];
+    __shared__ uint32_t arc_first_all[AW][EMIT ? WAVE : 1], arc_c0_all[AW][EMIT ? WAVE : 1];
     const int w = threadIdx.x / WAVE;
     VRec* recs = recs_all[w];
+    ArcJob* arc_jobs = arcs_all[EMIT ? w : 0];
+    uint32_t* arc_first = arc_first_all[EMIT ? w : 0];
+    uint32_t* arc_c0 = arc_c0_all[EMIT ? w : 0];
     const int l = lane_id();
     uint32_t pid = next_path(L.work_counter);
     while (pid < L.in.n_paths) {
         const uint32_t gp = L.path_list ? L.path_list[pid] : L.in.first_path + pid;
         uint32_t nv = 0, ni = 0;
-        stroke_one_path<EMIT>(L, gp, recs, &nv, &ni);
+        stroke_one_path<EMIT>(L, gp, recs, arc_jobs, arc_first, arc_c0, &nv, &ni);
         if (!EMIT || L.store_counts) {
             // every lane holds the totals; lane 0's store is the only one kept
             uint32_t* dst_v = L.path_nv + gp;

@@ -10,6 +10,7 @@
 #pragma once
 #include "lyg_math.h"
 #include "lyg_types.h"
+#include "wave.h"
 
 namespace lyg {
 
@@ -366,13 +367,59 @@ struct JInfo {
     RoundArc arc[2]; // round join arcs (valid when needs[side] and Round)
 };
 
+// A round-join arc left for the wave to emit cooperatively (emit_arcs): the same
+// vertices and triangles tessellate_arc would write, at the ids and triangle slots the
+// sink had reached when the arc came up.
+struct ArcJob {
+    float a0, a1, cx, cy;
+    uint32_t va, vb, n, first_id;
+    uint32_t* tri;   // index slots of the arc's first triangle
+    uint32_t ileft;  // index slots left in the path from `tri`
+    uint32_t pad;
+};
+struct ArcJobs {  // at most one arc per side of a lane's join
+    ArcJob sp, sn;
+    bool has_sp = false, has_sn = false;
+};
+
 // One side's round join arc (stroke.rs:1957-1962); `side` is a constant so the join
-// state is never indexed at run time.
+// state is never indexed at run time.  With `jobs` the arc is recorded and the sink
+// advanced past it; without, it is emitted here.
 template <int side>
-__device__ __forceinline__ void round_join_side(const JInfo& J, const SParams& p, Sink& s) {
+__device__ __forceinline__ void round_join_side(const JInfo& J, const SParams& p, Sink& s,
+                                                ArcJobs* jobs) {
     if (!J.needs[side]) return;
     const uint32_t pv = J.x.pv[side], nv = J.x.nv[side];
     const bool swap = J.arc[side].swap;
+    if (jobs) {
+        uint32_t n = J.arc[side].nsub;
+        if (n == 0) return;  // tessellate_arc emits nothing
+        if (n > 30) n = 30;
+        const uint32_t cnt = (1u << n) - 1u;
+        ArcJob a;
+        a.a0 = J.arc[side].a0;
+        a.a1 = J.arc[side].a1;
+        a.cx = J.x.pos.x;
+        a.cy = J.x.pos.y;
+        a.va = swap ? nv : pv;
+        a.vb = swap ? pv : nv;
+        a.n = n;
+        a.first_id = s.vid;
+        a.tri = s.iout + 3u * s.ni;
+        a.ileft = s.ileft > 3u * s.ni ? s.ileft - 3u * s.ni : 0u;
+        a.pad = 0;
+        if (side == SP) {
+            jobs->sp = a;
+            jobs->has_sp = true;
+        } else {
+            jobs->sn = a;
+            jobs->has_sn = true;
+        }
+        s.vid += cnt;
+        s.nv += cnt;
+        s.ni += cnt;
+        return;
+    }
     Sink s2 = s;  // the call takes its address: keep the caller's sink in registers
     tessellate_arc(J.arc[side].a0, J.arc[side].a1, swap ? nv : pv, swap ? pv : nv,
                    J.arc[side].nsub, J.x.pos, p.hw, s2);
@@ -380,7 +427,8 @@ __device__ __forceinline__ void round_join_side(const JInfo& J, const SParams& p
 }
 
 // tessellate_join (stroke.rs:1907-1965) with the arcs precomputed in `J`.
-__device__ __forceinline__ void tessellate_join(const JInfo& J, const SParams& p, Sink& s) {
+__device__ __forceinline__ void tessellate_join(const JInfo& J, const SParams& p, Sink& s,
+                                                ArcJobs* jobs = nullptr) {
     const EPD& j = J.x;
     bool needs_p = J.needs[SP], needs_n = J.needs[SN];
     if (!j.fold[SP] && !j.fold[SN]) {
@@ -394,8 +442,8 @@ __device__ __forceinline__ void tessellate_join(const JInfo& J, const SParams& p
         }
     }
     if (j.line_join == LYG_JOIN_ROUND) {
-        round_join_side<SP>(J, p, s);
-        round_join_side<SN>(J, p, s);
+        round_join_side<SP>(J, p, s, jobs);
+        round_join_side<SN>(J, p, s, jobs);
     }
 }
 
@@ -586,7 +634,7 @@ __device__ __forceinline__ void join_base_vertices(const EPD& join, const SParam
 // slots in both branches would be merged into stores through a selected address, which
 // keeps the join state in private memory.
 __device__ __forceinline__ void join_emit(JInfo& J, const EPD& prev, bool with_edge,
-                                          const SParams& p, Sink& s) {
+                                          const SParams& p, Sink& s, ArcJobs* jobs = nullptr) {
     EPD& join = J.x;
     uint32_t pp = INVALID_ID, np = INVALID_ID, pn = INVALID_ID, nn = INVALID_ID;
     if (J.fast) {
@@ -603,7 +651,76 @@ __device__ __forceinline__ void join_emit(JInfo& J, const EPD& prev, bool with_e
     join.pv[SN] = pn;
     join.nv[SN] = nn;
     if (with_edge) add_edge_triangles(prev, join, s);
-    tessellate_join(J, p, s);
+    tessellate_join(J, p, s, jobs);
+}
+
+// Emits the arcs recorded by the wave's lanes (ArcJobs) with all 64 lanes: node g of the
+// concatenated arcs goes to lane g mod 64.  Node p of an arc of depth n is found by
+// walking down from the root in tessellate_arc's pre-order (the node, its lower half,
+// its upper half): the same midpoints, ids first_id + p and triangles (va, id, vb) as
+// the serial recursion.  `jl` (128 jobs) and `xl`, `cl` (64 each) are the wave's LDS;
+// `s` is the calling lane's sink (every lane of the wave writes into the same path).
+__device__ __forceinline__ void emit_arcs(const ArcJobs& aj, const Sink& s, float hw, ArcJob* jl,
+                                          uint32_t* xl, uint32_t* cl) {
+    const int l = lane_id();
+    const uint32_t c0 = aj.has_sp ? (1u << aj.sp.n) - 1u : 0u;
+    const uint32_t c1 = aj.has_sn ? (1u << aj.sn.n) - 1u : 0u;
+    const uint32_t mine = c0 + c1;
+    const uint32_t incl = incl_scan(mine);
+    const uint32_t total = readlane_u(incl, WAVE - 1);
+    if (total == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();  // earlier readers of the LDS slots are done
+    if (c0) jl[2 * l] = aj.sp;
+    if (c1) jl[2 * l + 1] = aj.sn;
+    xl[l] = incl - mine;
+    cl[l] = c0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t base = 0; base < total; base += WAVE) {
+        const uint32_t g = base + (uint32_t)l;
+        if (g >= total) continue;
+        // owner lane: the last lane whose first node is <= g
+        int lo = 0, hi = WAVE - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (xl[mid] <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        uint32_t r = g - xl[lo];
+        const uint32_t oc0 = cl[lo];
+        const ArcJob a = r < oc0 ? jl[2 * lo] : jl[2 * lo + 1];
+        const uint32_t p = r < oc0 ? r : r - oc0;
+        float a0 = a.a0, a1 = a.a1;
+        uint32_t va = a.va, vb = a.vb, idx = 0, size = (1u << a.n) - 1u;
+        float mid = (a0 + a1) * 0.5f;
+        while (idx != p) {
+            const uint32_t half = (size - 1u) >> 1;
+            const uint32_t cur = a.first_id + idx;
+            if (p <= idx + half) {
+                a1 = mid;
+                vb = cur;
+                idx += 1;
+            } else {
+                a0 = mid;
+                va = cur;
+                idx += 1 + half;
+            }
+            size = half;
+            mid = (a0 + a1) * 0.5f;
+        }
+        Sink w = s;  // this lane's copy of the path's sink, placed at the node
+        w.vid = a.first_id + p;
+        const uint32_t id = sink_sv(w, mk(a.cx, a.cy), mk(t_cos(mid), t_sin(mid)), hw);
+        if (3u * p + 3u <= a.ileft) {
+            uint32_t* o = a.tri + 3u * p;
+            o[0] = va;
+            o[1] = id;
+            o[2] = vb;
+        } else if (s.err) {
+            atomicOr(s.err, 2u);
+        }
+    }
 }
 
 // The whole `count > 1` branch of fixed_width_step_impl (stroke.rs:1558-1625) for the
