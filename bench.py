@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="only run the batched stroke steps and a 1 GiB device copy (the "
                          "reference kernel for the traffic counters), for rocprofv3")
+    ap.add_argument("--profile-fill", action="store_true",
+                    help="only run the batched fill steps and a 1 GiB device copy, for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the sharding of this "
@@ -112,6 +114,22 @@ def main():
             synchronize()
             log(f"profile step {k} done")
         print(json.dumps({"profile": "stroke batch + 1 GiB copy", "steps": args.steps}))
+        return
+
+    if args.profile_fill:
+        fbatch = workloads.fill_batch(args.fill_paths, seed=0)
+        fopts = workloads.fill_options(args.fill_paths, tolerance=0.05)
+        fdb = DeviceBatch(fbatch, fopts)
+        fb = FillBatcher()
+        from lyon_amd.batch import DeviceBuffer
+        src = DeviceBuffer(1 << 30)
+        dst = DeviceBuffer(1 << 30)
+        for k in range(args.steps):
+            fb.run(fdb, check=False)
+            L.lyg_device_copy(ctx, dst.ptr, src.ptr, 1 << 30)
+            synchronize()
+            log(f"profile fill step {k} done")
+        print(json.dumps({"profile": "fill batch + 1 GiB copy", "steps": args.steps}))
         return
 
     # ---- batched stroke (headline) ----------------------------------------------------
