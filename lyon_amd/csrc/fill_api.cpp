@@ -42,6 +42,7 @@ constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO
 constexpr size_t WORK_BUDGET = 16ull << 30;          // working memory of one round
 constexpr size_t STAGE_BUDGET = 64ull << 30;         // staging memory of one round
 constexpr int MAX_CLASSES = 8;
+constexpr size_t CLASS_STREAMS = 3;
 
 using lyg::fillk::FillCaps;
 
@@ -78,15 +79,16 @@ int size_class(uint64_t bound) {
     return k;
 }
 
-// Streams (default priority) and events for running a round's size classes at once.
-bool class_streams(lyg_ctx* c, size_t n) {
-    if (n > (size_t)MAX_CLASSES) return false;
+// Streams (default priority) for running a round's size classes at once, and one event
+// per launch plus the start event.
+bool class_streams(lyg_ctx* c, size_t n, size_t n_launches) {
+    if (n > (size_t)MAX_CLASSES || n_launches > (size_t)MAX_CLASSES) return false;
     for (size_t i = 0; i < n; ++i)
         if (!c->class_stream[i] &&
             !check(c, hipStreamCreateWithFlags(&c->class_stream[i], hipStreamNonBlocking),
                    "hipStreamCreate"))
             return false;
-    for (size_t i = 0; i <= n; ++i)
+    for (size_t i = 0; i <= n_launches; ++i)
         if (!c->class_ev[i] &&
             !check(c, hipEventCreateWithFlags(&c->class_ev[i], hipEventDisableTiming),
                    "hipEventCreate"))
@@ -211,10 +213,14 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         L.round = (uint32_t)round;
         L.stage_v = as<float>(c->f_stage_v[round]);
         L.stage_t = as<uint32_t>(c->f_stage_t[round]);
-        // One launch per size class, all classes running at once on the context's class
-        // streams (largest class first), each with its own part of the working memory.
+        // One launch per size class, each with its own part of the working memory, on
+        // CLASS_STREAMS streams that run at once (a process gets four hardware queues;
+        // the context's own stream holds the fourth).  Classes go largest first to the
+        // stream with the least work queued, the work of a class being its largest path
+        // (its events plus twice its estimated intersections).
         struct Launch {
             size_t k0, k1, blocks, stride, work_off;
+            uint64_t est;
             FillCaps cap;
         };
         const uint32_t wpb = c->fill_waves_per_block;
@@ -234,6 +240,8 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             Launch ln;
             ln.k0 = k0;
             ln.k1 = k1;
+            ln.est = 0;
+            for (size_t k = k0; k < k1; ++k) ln.est = std::max<uint64_t>(ln.est, need[list[k]]);
             ln.cap = class_caps(std::max<uint64_t>(maxb, 1), maxi, scale);
             ln.stride = lyg_fill_work_bytes(&ln.cap, hot);
             size_t blocks = (k1 - k0 + wpb - 1) / wpb;
@@ -255,13 +263,19 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         if (!ensure(c, c->f_work, work_bytes)) return LYG_ERR_NO_SPACE;
         const auto t_round = std::chrono::steady_clock::now();
         if (c->timing && round == 0) (void)hipEventRecord(c->ev[2], s);
-        if (!class_streams(c, launches.size()) ||
+        const size_t n_streams = std::min<size_t>(launches.size(), CLASS_STREAMS);
+        if (!class_streams(c, n_streams, launches.size()) ||
             !check(c, hipEventRecord(c->class_ev[0], s), "event"))
             return LYG_ERR_DEVICE;
+        uint64_t load[CLASS_STREAMS] = {};
         for (size_t r = 0; r < launches.size(); ++r) {
             const size_t li = launches.size() - 1 - r;  // largest class first
             const Launch& ln = launches[li];
-            hipStream_t cs = c->class_stream[li];
+            size_t si = 0;
+            for (size_t q = 1; q < n_streams; ++q)
+                if (load[q] < load[si]) si = q;
+            load[si] += ln.est;
+            hipStream_t cs = c->class_stream[si];
             L.work = as<uint8_t>(c->f_work) + ln.work_off;
             L.work_stride = ln.stride;
             L.caps = ln.cap;
