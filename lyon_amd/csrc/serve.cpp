@@ -1,12 +1,13 @@
 // Serving loop: continuous batching of tessellation requests on one GPU.
 //
 // Requests (one path + options, copied at submit) wait in a FIFO.  A dispatcher thread
-// admits them into batches while their worst-case output fits the unreserved part of the
+// admits them into batches of one kind and one size class (factor 4 in flattened points)
+// while their worst-case output fits the unreserved part of the
 // device pool, and hands each batch to an idle worker.  Every worker owns a context (its
 // own HIP stream), so up to `max_inflight` batches are on the device at once: requests
 // that arrive while others run are batched as soon as a worker and pool room free up.
-// A batch runs the batched kernels (stroke: count, scan, emit; fill: sweep rounds, scan,
-// copy); once its totals are known, its output is allocated from the pool at its exact
+// A batch runs the batched kernels (stroke: staged emit, scan, compaction; fill: sweep
+// rounds, scan, copy); once its totals are known, its output is allocated from the pool at its exact
 // size and the reservation is returned.  Results stay in the pool (device pointers,
 // request-local vertex ids) until the client releases them.
 #include <hip/hip_runtime.h>
@@ -105,6 +106,7 @@ struct Request {
     lyg_fill_options fopts{};
     lyg_stroke_options sopts{};
     uint64_t reserve = 0;        // worst-case output bytes (admission)
+    uint32_t size_class = 0;     // log4 of the flattened-point bound: batches mix one class
     Clock::time_point t_submit, t_done;
     State state = S_QUEUED;
     int32_t status = LYG_OK;
@@ -239,12 +241,15 @@ void lyg_server::run_dispatcher() {
             lk.lock();
             if (stop) return;
         }
-        // admit FIFO requests of the oldest request's kind while they fit the pool
+        // admit FIFO requests of the oldest request's kind and size class while they fit
+        // the pool (a batch completes with its longest path: mixing sizes would make the
+        // short requests wait for the long ones)
         const Kind kind = pending.front()->kind;
+        const uint32_t cls = pending.front()->size_class;
         std::vector<Request*> batch;
         for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch_paths;) {
             Request* r = *it;
-            if (r->kind != kind) {
+            if (r->kind != kind || r->size_class != cls) {
                 ++it;
                 continue;
             }
@@ -404,6 +409,11 @@ int32_t submit(lyg_server* s, Kind kind, const lyg_path_view* path, const void* 
     if (kind == K_FILL) r->fopts = *static_cast<const lyg_fill_options*>(opts);
     else r->sopts = *static_cast<const lyg_stroke_options*>(opts);
     r->reserve = worst_case_bytes(*r);
+    {
+        const float tol = kind == K_FILL ? r->fopts.tolerance : r->sopts.tolerance;
+        const uint64_t f = flattened_bound(r->verbs, r->points, tol > 0 ? tol : 0.1f);
+        r->size_class = (uint32_t)(f > 1 ? (63 - __builtin_clzll(f)) / 2 : 0);
+    }
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return LYG_ERR_INVALID_ARGUMENT;
