@@ -19,6 +19,8 @@ The same JSON line carries:
   * roofline: algorithmic bytes of the stroke emit kernel (1 B/verb + 8 B/point +
     8 B/vertex + 4 B/index) over its average duration (HIP events on the library's
     stream); traffic from rocprofv3 counters when given (--traffic-json);
+  * serving: the serving loop (continuous batching, 8 GiB device pool, 4 in-flight
+    batches) under seeded Poisson request streams, with p50/p99/max request latency;
   * cpu_baseline: the oracle (C++ restatement of lyon's tessellators) on the box's host
     cores over a bounded sample of the same workloads, single thread and all cores.
 """
@@ -74,6 +76,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fill", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
                     help="HBM traffic per emit launch measured with rocprofv3 counters")
@@ -220,6 +223,28 @@ def main():
                                    "sample_paths": fns},
                 "batched_over_plain": (ftris / (fms / 1e3)) / fplain}
 
+    # ---- serving loop: Poisson request streams ---------------------------------------------
+    serving = None
+    if not args.no_serve:
+        from lyon_amd.serve import Server
+        serving = []
+        streams = [
+            ("stroke, 4-64 segments", 100000.0, 20000, 1.0, 4, 64),
+            ("stroke, 4-256 segments", 100000.0, 20000, 1.0, 4, 256),
+            ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256),
+        ]
+        for name, rate, nreq, sfrac, smin, smax in streams:
+            log(f"serving stream: {name} at {rate:.0f} req/s")
+            with Server(pool_bytes=8 << 30, max_batch_paths=1024, max_inflight=4) as srv:
+                r = srv.run_stream(n_requests=nreq, rate_per_s=rate, stroke_frac=sfrac,
+                                   min_segs=smin, max_segs=smax, tolerance=0.05,
+                                   n_distinct=256, seed=1)
+            serving.append({"stream": name, "requests": nreq,
+                            "offered_rate": rate, "completed_rate": r["completed_rate"],
+                            "p50_us": r["p50_us"], "p99_us": r["p99_us"], "max_us": r["max_us"],
+                            "triangles_per_s": r["triangles_per_s"],
+                            "mean_batch": r["mean_batch"]})
+
     # ---- CPU baseline ---------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -288,6 +313,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
         "fill": fill,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
