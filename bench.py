@@ -12,6 +12,8 @@ generator) and resident in HBM before the timed region.
 The same JSON line carries:
   * plain_per_path: the plain per-path drop-in (host buffers in/out, one call per path) on
     a sample of the paths, and the batched/plain ratio;
+  * flatten: lyon_geom flattening of the stroke workload's segments with device-resident
+    inputs and outputs (points/s and the emit kernel's roofline);
   * fill: the batched fill workload (4,096 closed paths, half self-intersecting star
     polygons with 5-512 points, half loops of 4-256 cubics, EvenOdd/NonZero 50/50,
     tolerance 0.05, coordinates in [0, 4096]^2) timed the same way, with its plain
@@ -77,6 +79,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fill", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-flatten", action="store_true")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
                     help="HBM traffic per emit launch measured with rocprofv3 counters")
@@ -177,6 +180,45 @@ def main():
 
     plain_value = plain_rate(StrokeTessellator(), [batch.path(i) for i in range(ns)], sos,
                              buffers)
+
+    # ---- batched flattening (lyon_geom segments) -----------------------------------------
+    flat = None
+    if not args.no_flatten:
+        from lyon_amd.batch import DeviceBuffer, flatten_segments_device
+        log("flatten: the segments of the stroke workload")
+        kinds, ctrl = workloads.segments_of(batch)
+        nseg = len(kinds)
+        dk = DeviceBuffer.from_numpy(kinds)
+        dc = DeviceBuffer.from_numpy(ctrl.reshape(-1))
+        doff = DeviceBuffer(4 * (nseg + 1))
+        try:
+            flatten_segments_device(dk, dc, nseg, 0.05, None, doff)
+            fl_total = 0
+        except Exception as e:
+            fl_total = getattr(e, "needed", 0)
+        dout = DeviceBuffer(8 * max(fl_total, 1))
+        for _ in range(max(args.warmup, 1)):
+            flatten_segments_device(dk, dc, nseg, 0.05, dout, doff)
+        fl_emit, fl_count = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            flatten_segments_device(dk, dc, nseg, 0.05, dout, doff)
+            a, b = kernel_times(L, ctx)
+            fl_count.append(a)
+            fl_emit.append(b)
+        fl_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        fl_bytes = 40 * nseg + 8 * fl_total  # kind + 8 floats + offset per segment, 8 B/point
+        fl_emit_ms = float(np.mean(fl_emit))
+        fl_ach = fl_bytes / (fl_emit_ms / 1e3) / 1e9
+        flat = {"workload": f"the {nseg} line/quadratic/cubic segments of the stroke workload, "
+                            "tol 0.05 (for_each_flattened_with_t, device memory in and out)",
+                "segments": nseg, "points_per_step": fl_total,
+                "value": fl_total / (fl_ms / 1e3), "unit": "points/s", "ms_per_step": fl_ms,
+                "kernel_ms": {"count_scan_avg": float(np.mean(fl_count)), "emit_avg": fl_emit_ms},
+                "roofline": {"bound": "hbm", "achieved": fl_ach, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": fl_ach / HBM_PEAK_GBS,
+                             "kernel": "flatten_emit_kernel", "algorithmic_bytes": fl_bytes}}
+        del dk, dc, doff, dout
 
     # ---- batched fill -------------------------------------------------------------------
     fill = None
@@ -312,6 +354,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
+        "flatten": flat,
         "fill": fill,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -154,6 +154,13 @@ int32_t lyg_fill_path(lyg_ctx* ctx, const lyg_path_view* path, const lyg_fill_op
 int32_t lyg_flatten_segments(lyg_ctx* ctx, const int32_t* kinds, const float* ctrl, uint32_t n,
                              float tolerance, float* out_points, uint32_t point_capacity,
                              uint32_t* offsets, uint32_t* n_points);
+/* The same with device pointers (inputs, points and offsets stay in device memory; the
+ * count is returned in *n_points).  Status LYG_ERR_CAPACITY (and *n_points) when
+ * out_points is null or point_capacity is too small. */
+int32_t lyg_flatten_segments_device(lyg_ctx* ctx, const int32_t* kinds, const float* ctrl,
+                                    uint32_t n, float tolerance, float* out_points,
+                                    uint32_t point_capacity, uint32_t* offsets,
+                                    uint32_t* n_points);
 /* Arc::for_each_flattened_with_t (arc.rs:339-368) for n arcs of 7 floats each
  * (center x, y, radii x, y, start angle, sweep angle, x rotation).  Host pointers. */
 int32_t lyg_flatten_arcs(lyg_ctx* ctx, const float* arcs, uint32_t n, float tolerance,

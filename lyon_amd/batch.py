@@ -285,3 +285,20 @@ def set_stroke_staging(enabled: bool = True, room_scale: float = 1.0):
     st = B.lib().lyg_ctx_set_stroke_staging(_Ctx.get(), 1 if enabled else 0, float(room_scale))
     if st != 0:
         raise TessellationError(st)
+
+
+def flatten_segments_device(kinds: DeviceBuffer, ctrl: DeviceBuffer, n: int, tolerance: float,
+                            out: DeviceBuffer, offsets: DeviceBuffer) -> int:
+    """lyg_flatten_segments_device: flattens n segments held in device memory into `out`
+    (f32 x, y pairs) with per-segment offsets (n + 1, u32); returns the point count.
+    Raises TessellationError(16) with the count in .needed when `out` is too small."""
+    cnt = C.c_uint32(0)
+    st = B.lib().lyg_flatten_segments_device(_Ctx.get(), kinds.ptr, ctrl.ptr, n, tolerance,
+                                             out.ptr if out is not None else None,
+                                             (out.nbytes // 8) if out is not None else 0,
+                                             offsets.ptr, C.byref(cnt))
+    if st != 0:
+        e = TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+        e.needed = cnt.value
+        raise e
+    return cnt.value

@@ -3,10 +3,11 @@
 // Segments (LineSegment / QuadraticBezierSegment / CubicBezierSegment): a count kernel
 // evaluates Wang's formula per segment (one lane each), the host-side runtime scans the
 // counts, and an emit kernel gives every output point its own lane: the lane finds its
-// segment by binary search over the offsets, recovers t with the closed-form serial
-// chain (lyg_math.h: tchain) and evaluates the polynomial form with Horner's rule in
-// lyon's operation order (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).
-// Output is fully coalesced: consecutive lanes write consecutive points.
+// segment in a window of segment offsets, recovers t with the closed-form serial chain
+// (lyg_math.h: tchain, resumed from the lane's previous point of the same segment) and
+// evaluates the polynomial form with Horner's rule in lyon's operation order
+// (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).  Output is fully coalesced:
+// consecutive lanes write consecutive points.
 //
 // Arcs (Arc::for_each_flattened_with_t, arc.rs:339-368) re-split the remaining arc at
 // every step, a serial recurrence, so one lane walks one arc.
@@ -64,35 +65,115 @@ __global__ void flatten_count_kernel(FlattenLaunch L) {
     L.counts[i] = s.n;
 }
 
-__global__ void flatten_emit_kernel(FlattenLaunch L, uint32_t total) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    // last segment whose offset <= g
-    uint32_t lo = 0, hi = L.n - 1;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi + 1) >> 1;
-        if (L.offsets[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    SegPoly s = seg_poly(L.kinds[lo], L.ctrl + 8ull * lo, L.tolerance);
-    uint32_t k = g - L.offsets[lo];
-    vec2 p;
-    if (k + 1 >= s.n) {
-        p = s.to;
-    } else {
-        float t = tchain(s.step, k + 1);
-        vec2 v = s.a0;
-        float tt = t;
-        v = add(v, mul(s.a1, tt));
-        tt = tt * t;
-        v = add(v, mul(s.a2, tt));
-        if (s.cubic) {
-            tt = tt * t;
-            v = add(v, mul(s.a3, tt));
+// Emit: each wave writes FL_ITERS * 64 consecutive output points, 64 per iteration (lane l
+// the point base + 64 i + l: coalesced stores).  The segments of an iteration's 64 points
+// are found in an LDS window of the next 64 segment offsets (binary search in LDS); a lane
+// keeps the polynomial of the segment it is in and its last t, so its next point of the
+// same segment continues lyon's serial sum 64 steps on (a point near its segment's start
+// sums from 0); only a wave's first points, deep inside a segment, use the closed form.
+static constexpr uint32_t FL_ITERS = 16;
+static constexpr uint32_t FL_WAVES = 4;  // waves per block
+
+__global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_kernel(FlattenLaunch L,
+                                                                      uint32_t total) {
+    __shared__ uint32_t win_all[FL_WAVES][WAVE];
+    const int l = lane_id();
+    const uint32_t w = threadIdx.x / WAVE;
+    uint32_t* win = win_all[w];
+    const uint64_t wave = (uint64_t)blockIdx.x * FL_WAVES + w;
+    const uint64_t base = wave * WAVE * FL_ITERS;
+    if (base >= total) return;
+    // segment of the wave's first point: last segment whose offset <= base
+    uint32_t cur = 0;
+    {
+        uint32_t lo = 0, hi = L.n - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (L.offsets[mid] <= base) lo = mid;
+            else hi = mid - 1;
         }
-        p = v;
+        cur = lo;
     }
-    reinterpret_cast<float2*>(L.out_points)[g] = make_float2(p.x, p.y);
+    uint32_t seg = 0xffffffffu, seg_begin = 0, k_prev = 0;
+    float t_prev = 0.0f;
+    SegPoly sp{};
+    for (uint32_t it = 0; it < FL_ITERS; ++it) {
+        const uint64_t first = base + (uint64_t)it * WAVE;
+        if (first >= total) break;  // uniform
+        const uint32_t g = (uint32_t)(first + (uint32_t)l);
+        const bool valid = (uint64_t)g < total;
+        // window of segment starts cur .. cur + 63 (segments past the last start "never")
+        const uint32_t wi = cur + (uint32_t)l;
+        const uint32_t wv = wi < L.n ? L.offsets[wi] : 0xffffffffu;
+        __builtin_amdgcn_wave_barrier();
+        win[l] = wv;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t my = cur;
+        if (valid) {
+            if (win[WAVE - 1] <= g) {
+                // more than 64 segments start before this point: search all of them
+                uint32_t lo = cur, hi = L.n - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (L.offsets[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                my = lo;
+            } else {
+                int lo = 0, hi = WAVE - 1;  // win[0] <= g < win[63]
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (win[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                my = cur + (uint32_t)lo;
+            }
+            if (my != seg) {
+                seg = my;
+                sp = seg_poly(L.kinds[seg], L.ctrl + 8ull * seg, L.tolerance);
+                seg_begin = L.offsets[seg];
+                k_prev = 0;
+                t_prev = 0.0f;
+            }
+            const uint32_t k = g - seg_begin;
+            vec2 p;
+            if (k + 1 >= sp.n) {
+                p = sp.to;
+            } else {
+                // t after k + 1 steps: lyon's own serial sum continued from this lane's
+                // previous point of the segment (64 steps back, or from 0 near the segment's
+                // start); the closed form only for a wave's first points deep in a segment
+                float t = t_prev;
+                const uint32_t d = k + 1 - k_prev;
+                if (d == WAVE) {
+#pragma unroll
+                    for (int q = 0; q < WAVE; ++q) t = t + sp.step;
+                } else if (d < WAVE) {
+                    for (uint32_t q = 0; q < d; ++q) t = t + sp.step;
+                } else {
+                    t = tchain(sp.step, k + 1);
+                }
+                k_prev = k + 1;
+                t_prev = t;
+                vec2 v = sp.a0;
+                float tt = t;
+                v = add(v, mul(sp.a1, tt));
+                tt = tt * t;
+                v = add(v, mul(sp.a2, tt));
+                if (sp.cubic) {
+                    tt = tt * t;
+                    v = add(v, mul(sp.a3, tt));
+                }
+                p = v;
+            }
+            reinterpret_cast<float2*>(L.out_points)[g] = make_float2(p.x, p.y);
+        }
+        // the next iteration starts in the segment of this iteration's last point
+        const uint64_t left = total - first;
+        const int last = left >= (uint64_t)WAVE ? WAVE - 1 : (int)left - 1;
+        cur = readlane_u(my, last);
+    }
 }
 
 // ---- arcs -------------------------------------------------------------------------------
@@ -215,6 +296,94 @@ __global__ void __launch_bounds__(1024) scan_offsets_kernel(const uint32_t* coun
     if (ovf) atomicOr(overflow, 1u);
 }
 
+// Large scans (n > SCAN_SMALL, lyg_launch_scan_tiles): three launches.  Tiles of SCAN_TILE counts (256 threads x
+// 16 consecutive counts each): per-tile sums, one block scanning the tile sums, then every
+// tile scanned again with its prefix and written.
+static constexpr uint32_t SCAN_SMALL = 32768;
+static constexpr uint32_t SCAN_TILE = 4096;
+
+__device__ __forceinline__ uint64_t block_incl_scan_u64(uint64_t v, uint64_t* sh) {
+    // 256 threads: wave-level inclusive scan, then the wave totals
+    const int l = lane_id();
+    const int w = threadIdx.x / WAVE;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+        const int src = l >= o ? l - o : l;
+        const uint64_t t = (uint64_t)shfl_u(lo, src) | ((uint64_t)shfl_u(hi, src) << 32);
+        if (l >= o) v += t;
+    }
+    if (l == WAVE - 1) sh[w] = v;
+    __syncthreads();
+    uint64_t before = 0;
+    for (int k = 0; k < w; ++k) before += sh[k];
+    __syncthreads();
+    return v + before;
+}
+
+__global__ void __launch_bounds__(256) scan_tile_sums_kernel(const uint32_t* counts, uint32_t n,
+                                                             uint64_t* tile_sums) {
+    __shared__ uint64_t sh[4];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    uint64_t v = 0;
+    for (uint32_t i = 0; i < SCAN_TILE / 256; ++i) {
+        const uint64_t k = base + (uint64_t)i * 256 + threadIdx.x;
+        if (k < n) v += counts[k];
+    }
+    v = block_incl_scan_u64(v, sh);
+    if (threadIdx.x == 255) tile_sums[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(256) scan_tile_prefix_kernel(uint64_t* tile_sums, uint32_t ntiles,
+                                                               uint64_t* total) {
+    __shared__ uint64_t sh[4];
+    uint64_t carry = 0;
+    for (uint32_t b = 0; b < ntiles; b += 256) {
+        const uint32_t k = b + threadIdx.x;
+        const uint64_t v = k < ntiles ? tile_sums[k] : 0;
+        const uint64_t incl = block_incl_scan_u64(v, sh);
+        if (k < ntiles) tile_sums[k] = carry + incl - v;  // exclusive prefix
+        __syncthreads();
+        if (threadIdx.x == 255) sh[0] = incl;
+        __syncthreads();
+        carry += sh[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) scan_tile_write_kernel(const uint32_t* counts, uint32_t n,
+                                                              const uint64_t* tile_prefix,
+                                                              uint32_t* offsets, uint32_t* overflow) {
+    __shared__ uint64_t sh[4];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * 16;
+    uint32_t c[16];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        c[i] = base + i < n ? counts[base + i] : 0u;
+        sum += c[i];
+    }
+    const uint64_t incl = block_incl_scan_u64(sum, sh);
+    uint64_t run = tile_prefix[blockIdx.x] + incl - sum;
+    bool ovf = false;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        if (base + i < n) {
+            if (run > 0xffffffffull) ovf = true;
+            offsets[base + i] = (uint32_t)run;
+        }
+        run += c[i];
+    }
+    if (ovf) atomicOr(overflow, 1u);
+}
+
+__global__ void scan_finish_kernel(const uint64_t* total, uint32_t* last, uint32_t* overflow) {
+    if (threadIdx.x != 0) return;
+    const uint64_t t = *total;
+    *last = (uint32_t)t;
+    if (t > 0xffffffffull) atomicOr(overflow, 1u);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -253,8 +422,9 @@ hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s) 
 }
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s) {
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(lyg::flatten_emit_kernel, dim3((total + 255) / 256), dim3(256), 0, s, *L,
-                       total);
+    const uint64_t per_block = (uint64_t)lyg::WAVE * lyg::FL_ITERS * lyg::FL_WAVES;
+    hipLaunchKernelGGL(lyg::flatten_emit_kernel, dim3((uint32_t)((total + per_block - 1) / per_block)),
+                       dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
     return hipGetLastError();
 }
 hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
@@ -275,6 +445,23 @@ hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets
                            uint32_t* overflow, hipStream_t s) {
     hipLaunchKernelGGL(lyg::scan_offsets_kernel, dim3(1), dim3(1024), 0, s, counts, n, offsets,
                        total, overflow);
+    return hipGetLastError();
+}
+// Bytes of the `tiles` scratch lyg_launch_scan_tiles needs for n counts.
+size_t lyg_scan_tiles_bytes(uint32_t n) {
+    return 8ull * ((n + lyg::SCAN_TILE - 1) / lyg::SCAN_TILE + 1);
+}
+// The same scan for large n, across the whole GPU (tiles: lyg_scan_tiles_bytes(n) bytes).
+hipError_t lyg_launch_scan_tiles(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                 uint64_t* total, uint32_t* overflow, uint64_t* tiles,
+                                 hipStream_t s) {
+    if (n <= lyg::SCAN_SMALL) return lyg_launch_scan(counts, n, offsets, total, overflow, s);
+    const uint32_t ntiles = (n + lyg::SCAN_TILE - 1) / lyg::SCAN_TILE;
+    hipLaunchKernelGGL(lyg::scan_tile_sums_kernel, dim3(ntiles), dim3(256), 0, s, counts, n, tiles);
+    hipLaunchKernelGGL(lyg::scan_tile_prefix_kernel, dim3(1), dim3(256), 0, s, tiles, ntiles, total);
+    hipLaunchKernelGGL(lyg::scan_tile_write_kernel, dim3(ntiles), dim3(256), 0, s, counts, n, tiles,
+                       offsets, overflow);
+    hipLaunchKernelGGL(lyg::scan_finish_kernel, dim3(1), dim3(64), 0, s, total, offsets + n, overflow);
     return hipGetLastError();
 }
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {

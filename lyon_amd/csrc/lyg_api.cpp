@@ -31,6 +31,10 @@ hipError_t lyg_launch_arc_to_quads(const float* arcs, uint32_t n, uint32_t* coun
 hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
                            uint32_t* overflow, hipStream_t s);
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+size_t lyg_scan_tiles_bytes(uint32_t n);
+hipError_t lyg_launch_scan_tiles(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                 uint64_t* total, uint32_t* overflow, uint64_t* tiles,
+                                 hipStream_t s);
 hipError_t lyg_launch_copy(void* dst, const void* src, uint64_t bytes, int grid, hipStream_t s);
 }
 
@@ -639,6 +643,55 @@ int32_t lyg_flatten_segments(lyg_ctx* c, const int32_t* kinds, const float* ctrl
         !check(c, hipMemcpyAsync(offsets, c->offs_v.p, 4ull * (n + 1), hipMemcpyDeviceToHost, s), "D2H") ||
         !check(c, hipStreamSynchronize(s), "sync"))
         return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lyg_flatten_segments_device(lyg_ctx* c, const int32_t* kinds, const float* ctrl,
+                                    uint32_t n, float tolerance, float* out_points,
+                                    uint32_t point_capacity, uint32_t* offsets,
+                                    uint32_t* n_points) {
+    if (!c || (n && (!kinds || !ctrl || !offsets))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!(tolerance > 0.0f)) return LYG_ERR_TOLERANCE_IS_NAN;
+    if (n_points) *n_points = 0;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    hipStream_t s = c->stream;
+    if (n == 0) {
+        return check(c, lyg_launch_fill_u32(offsets, 1, 0, s), "zero") &&
+                       check(c, hipStreamSynchronize(s), "sync")
+                   ? LYG_OK
+                   : LYG_ERR_DEVICE;
+    }
+    if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->misc, lyg_scan_tiles_bytes(n)))
+        return LYG_ERR_DEVICE;
+    lyg::FlattenLaunch L{};
+    L.kinds = kinds;
+    L.ctrl = ctrl;
+    L.n = n;
+    L.tolerance = tolerance;
+    L.counts = as<uint32_t>(c->counts_v);
+    if (c->timing) (void)hipEventRecord(c->ev[0], s);
+    if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
+        !check(c, lyg_launch_flatten_count(&L, s), "flatten count") ||
+        !check(c, lyg_launch_scan_tiles(L.counts, n, offsets, c->d_totals, c->d_counters + 4,
+                                        as<uint64_t>(c->misc), s),
+               "scan"))
+        return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[1], s);
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, c->d_totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(h + 1, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t total = h[0];
+    if ((uint32_t)h[1] != 0 || total > 0xffffffffull) return LYG_ERR_TOO_MANY_VERTICES;
+    if (n_points) *n_points = (uint32_t)total;
+    if (total > point_capacity || !out_points) return LYG_ERR_CAPACITY;
+    L.offsets = offsets;
+    L.out_points = out_points;
+    if (c->timing) (void)hipEventRecord(c->ev[2], s);
+    if (!check(c, lyg_launch_flatten_emit(&L, (uint32_t)total, s), "flatten emit")) return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[3], s);
+    if (!check(c, hipStreamSynchronize(s), "sync")) return LYG_ERR_DEVICE;
     return LYG_OK;
 }
 

@@ -56,3 +56,34 @@ def test_arcs():
     rq, rqo = O.arc_to_quads(arcs)
     assert np.array_equal(gqo, rqo)
     assert np.max(np.abs(gq.astype(np.float64) - rq)) <= 1e-5 * diag
+
+
+def test_device_segments_match_host_and_oracle():
+    """The device-memory flattening entry point gives the host entry point's bytes and the
+    oracle's on the segments of a stroke workload batch."""
+    from lyon_amd import workloads
+    from lyon_amd.batch import DeviceBuffer, flatten_segments_device
+    b = workloads.stroke_batch(600, 8, 256, 2048.0, seed=4)  # > 32768 segments: tiled scan
+    kinds, ctrl = workloads.segments_of(b)
+    n = len(kinds)
+    dk = DeviceBuffer.from_numpy(kinds)
+    dc = DeviceBuffer.from_numpy(ctrl.reshape(-1))
+    doff = DeviceBuffer(4 * (n + 1))
+    try:
+        flatten_segments_device(dk, dc, n, 0.05, None, doff)
+        raise AssertionError("expected a capacity error")
+    except Exception as e:  # the count comes back with the capacity status
+        total = e.needed
+    out = DeviceBuffer(8 * total)
+    assert flatten_segments_device(dk, dc, n, 0.05, out, doff) == total
+    dp = out.download(np.float32, 2 * total).reshape(-1, 2)
+    do = doff.download(np.uint32, n + 1)
+    hp, ho = flatten_segments(kinds, ctrl.reshape(-1, 4, 2), 0.05)
+    assert np.array_equal(do, ho)
+    assert np.array_equal(dp.view(np.uint32), hp.view(np.uint32))
+    sel = np.arange(0, n, 7)
+    rp, ro = O.flatten_segments(kinds[sel], ctrl.reshape(-1, 4, 2)[sel], 0.05)
+    for j, i in enumerate(sel):
+        a = dp[do[i]:do[i + 1]]
+        r = rp[ro[j]:ro[j + 1]]
+        assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), f"segment {i}"
