@@ -210,6 +210,13 @@ LYG_HD float tchain_value(float s, uint64_t T) {
     float s2 = u2f((uint32_t)(e2 + 127) << 23);
     return (fa * s1) * s2;
 }
+// The units T of a value t of the chain (t = T * ulp(s), t >= s > 0 normal): the exact
+// inverse of tchain_value.
+LYG_HD uint64_t tchain_units_of(float s, float t) {
+    const uint32_t es = (f2u(s) >> 23) & 0xffu, bt = f2u(t);
+    const uint32_t et = (bt >> 23) & 0xffu;
+    return (uint64_t)((bt & 0x7fffffu) | 0x800000u) << (et - es);
+}
 LYG_HD float tchain(float s, uint32_t j) {
     if (j == 0) return 0.0f;
     return tchain_value(s, tchain_units(tchain_mant(s), j));

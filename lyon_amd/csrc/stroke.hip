@@ -345,9 +345,19 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
                         const uint32_t key = ((uint32_t)lo << 1) | (inA ? 0u : 1u);
                         const uint32_t j = kk + 1;
                         const bool cont = key == ch_key && ch_j < j;
-                        const uint64_t T = tchain_units_from(tchain_mant(sstep), j,
-                                                             cont ? ch_j : 0u, cont ? ch_T : 0ull);
-                        pos = horner(vr, !inA, inA && cubic, tchain_value(sstep, T));
+                        float t;
+                        uint64_t T;
+                        if (!cont && j <= (uint32_t)WAVE) {
+                            // a curve half that starts in this window: lyon's serial sum
+                            t = 0.0f;
+                            for (uint32_t q = 0; q < j; ++q) t = t + sstep;
+                            T = tchain_units_of(sstep, t);
+                        } else {
+                            T = tchain_units_from(tchain_mant(sstep), j, cont ? ch_j : 0u,
+                                                  cont ? ch_T : 0ull);
+                            t = tchain_value(sstep, T);
+                        }
+                        pos = horner(vr, !inA, inA && cubic, t);
                         flags = F_FLAT;
                         my_key = key;
                         my_j = j;

@@ -255,7 +255,10 @@ int main() {
   long bad = 0, tot = 0;
   for (int k = 0; k < 2000; ++k) {
     float n = k < 300 ? (float)(k + 1) : u(rng); float s = 1.0f / n; float t = 0;
-    for (uint32_t j = 1; j <= seg_count(n) + 2; ++j) { t = t + s; tot++; if (f2u(tchain(s, j)) != f2u(t)) bad++; }
+    for (uint32_t j = 1; j <= seg_count(n) + 2; ++j) {
+      t = t + s; tot++; if (f2u(tchain(s, j)) != f2u(t)) bad++;
+      if (tchain_units_of(s, t) != tchain_units(tchain_mant(s), j)) bad++;  // units of a serial sum
+    }
     // continuing a chain from any earlier state (the kernels resume where the previous
     // window of steps stopped) gives the same units
     const uint32_t m = tchain_mant(s), nj = seg_count(n) + 2;
