@@ -41,6 +41,7 @@ struct lyg_ctx {
     bool stroke_staged = true;
     float stroke_room_scale = 1.0f;  // multiplies every path's staging room
     DevBuf nsub_thr;                 // round-join subdivision thresholds (31 floats)
+    DevBuf scan_tiles;               // tile sums of scans over more than 32768 counts
     DevBuf st_cap_v, st_cap_i, st_off_v, st_off_i, st_v, st_i, st_fallback;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
@@ -98,4 +99,25 @@ inline int persistent_grid(lyg_ctx* c, uint32_t n_paths) {
     uint32_t blocks_needed = (n_paths + 3) / 4;
     uint32_t cap = (uint32_t)c->cu_count * c->stroke_blocks_per_cu;
     return (int)std::max(1u, std::min(blocks_needed, cap));
+}
+
+extern "C" {
+hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
+                           uint32_t* overflow, hipStream_t s);
+size_t lyg_scan_tiles_bytes(uint32_t n);
+hipError_t lyg_launch_scan_tiles(const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                                 uint64_t* total, uint32_t* overflow, uint64_t* tiles,
+                                 hipStream_t s);
+}
+
+// Exclusive scan of n counts into offsets (n + 1) and *total: one block for small n, the
+// whole GPU (with the context's tile buffer) beyond 32768 counts.
+inline hipError_t ctx_scan(lyg_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                           uint64_t* total, uint32_t* overflow, hipStream_t s) {
+    if (n > 32768) {
+        if (!ensure(c, c->scan_tiles, lyg_scan_tiles_bytes(n))) return hipErrorOutOfMemory;
+        return lyg_launch_scan_tiles(counts, n, offsets, total, overflow,
+                                     static_cast<uint64_t*>(c->scan_tiles.p), s);
+    }
+    return lyg_launch_scan(counts, n, offsets, total, overflow, s);
 }

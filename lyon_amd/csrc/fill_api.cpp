@@ -340,8 +340,8 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     for (uint32_t p : todo) R->status[p] = LYG_ERR_NO_SPACE;  // still out of room
     // step 3: offsets
     if (!check(c, lyg_launch_fill_u32(c->d_counters + 4, 1, 0, s), "reset") ||
-        !check(c, lyg_launch_scan(L.path_nv, n, vtx_off, c->d_totals + 0, c->d_counters + 4, s), "scan") ||
-        !check(c, lyg_launch_scan(L.path_ni, n, idx_off, c->d_totals + 1, c->d_counters + 4, s), "scan"))
+        !check(c, ctx_scan(c, L.path_nv, n, vtx_off, c->d_totals + 0, c->d_counters + 4, s), "scan") ||
+        !check(c, ctx_scan(c, L.path_ni, n, idx_off, c->d_totals + 1, c->d_counters + 4, s), "scan"))
         return LYG_ERR_DEVICE;
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
     if (!check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "D2H") ||

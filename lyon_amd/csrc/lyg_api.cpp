@@ -54,10 +54,10 @@ int32_t stroke_count_and_scan(lyg_ctx* c, lyg::StrokeLaunch& L, uint32_t* offs_v
     if (!check(c, lyg_launch_stroke(&L, 0, persistent_grid(c, n), c->stream), "stroke count"))
         return LYG_ERR_DEVICE;
     if (c->timing) (void)hipEventRecord(c->ev[1], c->stream);
-    if (!check(c, lyg_launch_scan(L.path_nv, n, offs_v, c->d_totals + 0, c->d_counters + 4, c->stream),
+    if (!check(c, ctx_scan(c, L.path_nv, n, offs_v, c->d_totals + 0, c->d_counters + 4, c->stream),
                "scan"))
         return LYG_ERR_DEVICE;
-    if (!check(c, lyg_launch_scan(L.path_ni, n, offs_i, c->d_totals + 1, c->d_counters + 4, c->stream),
+    if (!check(c, ctx_scan(c, L.path_ni, n, offs_i, c->d_totals + 1, c->d_counters + 4, c->stream),
                "scan"))
         return LYG_ERR_DEVICE;
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
@@ -129,10 +129,10 @@ int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Ou
         !check(c, lyg_launch_stroke_stage_caps(&L, n, c->stroke_room_scale,
                                                as<uint32_t>(c->st_cap_v), as<uint32_t>(c->st_cap_i), s),
                "stroke stage caps") ||
-        !check(c, lyg_launch_scan(as<uint32_t>(c->st_cap_v), n, as<uint32_t>(c->st_off_v),
+        !check(c, ctx_scan(c, as<uint32_t>(c->st_cap_v), n, as<uint32_t>(c->st_off_v),
                                   c->d_totals + 0, c->d_counters + 4, s),
                "scan") ||
-        !check(c, lyg_launch_scan(as<uint32_t>(c->st_cap_i), n, as<uint32_t>(c->st_off_i),
+        !check(c, ctx_scan(c, as<uint32_t>(c->st_cap_i), n, as<uint32_t>(c->st_off_i),
                                   c->d_totals + 1, c->d_counters + 4, s),
                "scan"))
         return LYG_ERR_DEVICE;
@@ -162,9 +162,9 @@ int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Ou
     if (c->timing) (void)hipEventRecord(c->ev[3], s);
     // 3. final offsets from the exact counts
     if (!check(c, lyg_launch_fill_u32(c->d_counters + 4, 1, 0, s), "reset") ||
-        !check(c, lyg_launch_scan(L.path_nv, n, b.vtx_off, c->d_totals + 0, c->d_counters + 4, s),
+        !check(c, ctx_scan(c, L.path_nv, n, b.vtx_off, c->d_totals + 0, c->d_counters + 4, s),
                "scan") ||
-        !check(c, lyg_launch_scan(L.path_ni, n, b.idx_off, c->d_totals + 1, c->d_counters + 4, s),
+        !check(c, ctx_scan(c, L.path_ni, n, b.idx_off, c->d_totals + 1, c->d_counters + 4, s),
                "scan") ||
         !check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "copy") ||
         !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "copy") ||
@@ -345,6 +345,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->f_work);
     free_buf(c->f_inter);
     free_buf(c->nsub_thr);
+    free_buf(c->scan_tiles);
     for (DevBuf* b : {&c->st_cap_v, &c->st_cap_i, &c->st_off_v, &c->st_off_i, &c->st_v, &c->st_i,
                       &c->st_fallback})
         free_buf(*b);
@@ -626,7 +627,7 @@ int32_t lyg_flatten_segments(lyg_ctx* c, const int32_t* kinds, const float* ctrl
     L.counts = as<uint32_t>(c->counts_v);
     if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
         !check(c, lyg_launch_flatten_count(&L, s), "flatten count") ||
-        !check(c, lyg_launch_scan(L.counts, n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
+        !check(c, ctx_scan(c, L.counts, n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
         return LYG_ERR_DEVICE;
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
     if (!check(c, hipMemcpyAsync(h, c->d_totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||
@@ -661,8 +662,7 @@ int32_t lyg_flatten_segments_device(lyg_ctx* c, const int32_t* kinds, const floa
                    ? LYG_OK
                    : LYG_ERR_DEVICE;
     }
-    if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->misc, lyg_scan_tiles_bytes(n)))
-        return LYG_ERR_DEVICE;
+    if (!ensure(c, c->counts_v, 4ull * n)) return LYG_ERR_DEVICE;
     lyg::FlattenLaunch L{};
     L.kinds = kinds;
     L.ctrl = ctrl;
@@ -672,9 +672,7 @@ int32_t lyg_flatten_segments_device(lyg_ctx* c, const int32_t* kinds, const floa
     if (c->timing) (void)hipEventRecord(c->ev[0], s);
     if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
         !check(c, lyg_launch_flatten_count(&L, s), "flatten count") ||
-        !check(c, lyg_launch_scan_tiles(L.counts, n, offsets, c->d_totals, c->d_counters + 4,
-                                        as<uint64_t>(c->misc), s),
-               "scan"))
+        !check(c, ctx_scan(c, L.counts, n, offsets, c->d_totals, c->d_counters + 4, s), "scan"))
         return LYG_ERR_DEVICE;
     if (c->timing) (void)hipEventRecord(c->ev[1], s);
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
@@ -714,7 +712,7 @@ static int32_t arcs_common(lyg_ctx* c, const float* arcs, uint32_t n, float tol,
                          : lyg_launch_arc_flatten(as<float>(c->arcs), n, tol, as<uint32_t>(c->counts_v), nullptr, nullptr, s);
     if (!check(c, e, "arc count") ||
         !check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
-        !check(c, lyg_launch_scan(as<uint32_t>(c->counts_v), n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
+        !check(c, ctx_scan(c, as<uint32_t>(c->counts_v), n, as<uint32_t>(c->offs_v), c->d_totals, c->d_counters + 4, s), "scan"))
         return LYG_ERR_DEVICE;
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
     if (!check(c, hipMemcpyAsync(h, c->d_totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||

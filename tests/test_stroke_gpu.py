@@ -199,3 +199,28 @@ def test_batch_sizing_modes_agree():
         idx = squeezed[3][io[i]:io[i + 1]]
         assert_same_geometry((v, idx), r, max(bbox_diag(p.points), so.line_width),
                              what=f"squeezed batch path {i}")
+
+
+def test_large_batch_offsets():
+    """A batch of more than 32768 paths (its scans run across the whole GPU): offsets are the
+    running sums of the per-path outputs and sampled paths equal the oracle's."""
+    from lyon_amd.batch import DeviceBatch, StrokeBatcher
+    n = 40000
+    batch = workloads.stroke_batch(n, 1, 6, 512.0, seed=9)
+    opts = workloads.stroke_options(n, tolerance=0.1, seed=9)
+    res = StrokeBatcher().run(DeviceBatch(batch, opts))
+    vo = res.vertex_offsets
+    io = res.index_offsets
+    assert vo[0] == 0 and io[0] == 0
+    assert vo[-1] == res.n_vertices and io[-1] == res.n_indices
+    assert np.all(np.diff(vo.astype(np.int64)) >= 0) and np.all(np.diff(io.astype(np.int64)) % 3 == 0)
+    for i in list(range(0, n, 1733)) + [n - 1]:
+        o = opts[i]
+        so = StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
+                           tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
+                           end_cap=int(o["end_cap"]), line_join=int(o["line_join"]))
+        p = batch.path(i)
+        r = ref_stroke(p, so, vertex_offset=int(vo[i]))
+        assert len(r[0]) == vo[i + 1] - vo[i] and len(r[1]) == io[i + 1] - io[i], f"path {i}"
+        assert_same_geometry(res.path(i), r, max(bbox_diag(p.points), so.line_width),
+                             what=f"large batch path {i}")
