@@ -109,6 +109,12 @@ __device__ __forceinline__ uint32_t gather_u(uint32_t v, int src, const uint32_t
     if (src >= 0) return s;
     return sel3(c[0], c[1], c[2], -src - 1);
 }
+__device__ __forceinline__ uint32_t gather_u3(uint32_t v, int src, uint32_t c0, uint32_t c1,
+                                              uint32_t c2) {
+    uint32_t s = shfl_u(v, src < 0 ? 0 : src);
+    if (src >= 0) return s;
+    return sel3(c0, c1, c2, -src - 1);
+}
 __device__ __forceinline__ Pub gather_pub(const Pub& v, int src, const Pub c[3]) {
     Pub r;
     int s = src < 0 ? 0 : src;
@@ -451,11 +457,11 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             const uint32_t Xf = gather_u(flags, p1, c.flags);
             const vec2 PP = gather_v(pos, p2, c.pos);
 
-            // firsts of this item's sub-path
+            // firsts of this item's sub-path (only a sub-path end reads them)
             vec2 F0 = c.f0, F1 = c.f1, F2 = c.f2;
             bool FF1 = c.ff1;
             int f2lane = -1;
-            {
+            if (endm != 0) {
                 uint64_t rb = restart & mask_le(l);
                 int rbl = rb ? highest_bit(rb) : -1;
                 uint64_t m0 = ballot(kept && isp == 0);
@@ -510,7 +516,9 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             }
             const uint32_t fake_base = 0x80000000u + ((uint32_t)l << 24);
             if (is_join) pub.base = fake_base;
-            prv = gather_pub(pub, p1, c.pub);
+            // only the code and the base of the published record changed since
+            prv.code = gather_u3(pub.code, p1, c.pub[0].code, c.pub[1].code, c.pub[2].code);
+            prv.base = gather_u3(pub.base, p1, c.pub[0].base, c.pub[1].base, c.pub[2].base);
 
             // END: everything the sub-path end needs
             EndInput ein;
@@ -528,7 +536,7 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             ein.f1 = F1;
             ein.f2 = F2;
             ein.ff1 = FF1;
-            {
+            if (endm != 0) {
                 uint32_t f1b = shfl_u(pub.base, f2lane < 0 ? 0 : f2lane);
                 ein.f1_base = f2lane >= 0 ? f1b : c.f1_base;
             }
@@ -555,8 +563,8 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             const uint32_t id_v0 = L.local_ids ? 0u : path_v0;
             const uint32_t real_base = EMIT ? (L.vertex_offset + id_v0 + my_vbase) : my_vbase;
             if (is_join) pub.base = real_base;
-            prv = gather_pub(pub, p1, c.pub);
-            {
+            prv.base = gather_u3(pub.base, p1, c.pub[0].base, c.pub[1].base, c.pub[2].base);
+            if (endm != 0) {
                 uint32_t f1b = shfl_u(pub.base, f2lane < 0 ? 0 : f2lane);
                 ein.f1_base = f2lane >= 0 ? f1b : c.f1_base;
             }
