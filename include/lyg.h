@@ -144,6 +144,16 @@ int32_t lyg_stroke_path(lyg_ctx* ctx, const lyg_path_view* path, const lyg_strok
 int32_t lyg_fill_path(lyg_ctx* ctx, const lyg_path_view* path, const lyg_fill_options* opts,
                       lyg_output* out);
 
+/* FillTessellator::tessellate_rectangle (fill.rs:653-661 -> basic_shapes.rs:12-38): four
+ * vertices, two triangles; rect = min x, min y, max x, max y.  No sweep and no device work
+ * (ctx may be null). */
+int32_t lyg_fill_rectangle(lyg_ctx* ctx, const float* rect, lyg_output* out);
+/* FillTessellator::tessellate_circle (fill.rs:663-672 -> basic_shapes.rs:40-205): four
+ * quadrant vertices and recursive border arcs.  No sweep and no device work (ctx may be
+ * null). */
+int32_t lyg_fill_circle(lyg_ctx* ctx, float cx, float cy, float radius,
+                        const lyg_fill_options* opts, lyg_output* out);
+
 /* ---- lyon_geom flattening, array form -------------------------------------------------
  * for_each_flattened_with_t of QuadraticBezierSegment (quadratic_bezier.rs:368-391),
  * CubicBezierSegment (cubic_bezier.rs:590-614) and LineSegment (line.rs:544-550) for n

@@ -106,6 +106,9 @@ SIGNATURES = {
                        C.POINTER(OutputC)], C.c_int32),
     "lyg_flatten_segments": ([C.c_void_p, I32P, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32,
                               U32P, U32P], C.c_int32),
+    "lyg_fill_rectangle": ([C.c_void_p, F32P, C.POINTER(OutputC)], C.c_int32),
+    "lyg_fill_circle": ([C.c_void_p, C.c_float, C.c_float, C.c_float, C.POINTER(FillOptionsC),
+                         C.POINTER(OutputC)], C.c_int32),
     "lyg_flatten_segments_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                      C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)],
                                     C.c_int32),

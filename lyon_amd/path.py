@@ -90,6 +90,57 @@ class PathBuilder:
     def close(self):
         self.end(True)
 
+    # ---- shapes (crates/path/src/builder.rs) ------------------------------------------
+    def add_polygon(self, points, closed: bool = True):
+        """BuilderImpl::add_polygon (builder.rs:560-573)."""
+        if len(points) == 0:
+            return
+        self.begin(points[0])
+        for p in points[1:]:
+            self.line_to(p)
+        self.end(closed)
+
+    def add_rectangle(self, rect, positive: bool = True):
+        """add_rectangle (builder.rs:649-680); rect = (min x, min y, max x, max y)."""
+        x0, y0, x1, y1 = (np.float32(v) for v in rect)
+        if positive:
+            pts = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+        else:
+            pts = [(x0, y0), (x0, y1), (x1, y1), (x1, y0)]
+        self.add_polygon(pts, True)
+
+    def add_circle(self, center, radius, positive: bool = True):
+        """add_circle (builder.rs:1508-1548): four cubics with the 0.55191505 factor, in f32."""
+        f = np.float32
+        cx, cy = f(center[0]), f(center[1])
+        r = f(abs(f(radius)))
+        d_ = f(1.0) if positive else f(-1.0)
+        d = r * f(0.55191505)
+
+        def pt(dx, dy):
+            return (cx + f(dx), cy + f(dy))
+
+        self.begin(pt(-r, f(0.0)))
+        self.cubic_bezier_to(pt(-r, -d * d_), pt(-d, -r * d_), pt(f(0.0), -r * d_))
+        self.cubic_bezier_to(pt(d, -r * d_), pt(r, -d * d_), pt(r, f(0.0)))
+        self.cubic_bezier_to(pt(r, d * d_), pt(d, r * d_), pt(f(0.0), r * d_))
+        self.cubic_bezier_to(pt(-d, r * d_), pt(-r, d * d_), pt(-r, f(0.0)))
+        self.close()
+
+    def add_ellipse(self, center, radii, x_rotation: float, positive: bool = True):
+        """add_ellipse (builder.rs:606-636): the arc's quadratic approximation
+        (Arc::for_each_quadratic_bezier, evaluated by the GPU arc kernel); the sub-path begins
+        at the first quadratic's start, which samples the same angle 0 as arc.sample(0)."""
+        from .tessellation import arc_to_quadratics
+        sweep = np.float32(2.0) * np.float32(np.pi) * (np.float32(1.0) if positive else np.float32(-1.0))
+        arc = np.array([[center[0], center[1], radii[0], radii[1], 0.0, sweep, x_rotation]],
+                       np.float32)
+        q, _ = arc_to_quadratics(arc)
+        self.begin(q[0, 0])
+        for k in range(len(q)):
+            self.quadratic_bezier_to(q[k, 1], q[k, 2])
+        self.end(True)
+
     def build(self) -> Path:
         assert not self._in_sub_path, "build() with an open sub-path"
         pts = np.asarray(self._points, dtype=np.float32).reshape(-1, 2)

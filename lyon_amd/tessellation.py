@@ -235,6 +235,34 @@ def _run(call, path: Path, copts, builder: BuffersBuilder):
     bufs.indices = np.concatenate([bufs.indices, i[:out.n_indices]])
 
 
+def _shape_path(fn) -> Path:
+    from .path import PathBuilder
+    b = PathBuilder()
+    fn(b)
+    return b.build()
+
+
+def _run_direct(call, builder: BuffersBuilder):
+    """A direct construction (no path) through lyg_output, two-phase like _run."""
+    bufs = builder.buffers
+    first = len(bufs.vertices) + builder.vertex_offset
+    vcap, icap = 256, 768
+    while True:
+        v = np.empty((vcap, 2), np.float32)
+        i = np.empty(icap, np.uint32)
+        out = B.OutputC(v.ctypes.data_as(B.F32P), i.ctypes.data_as(B.U32P), vcap, icap,
+                        first, 0, 0)
+        st = call(out)
+        if st == 16:  # LYG_ERR_CAPACITY
+            vcap, icap = max(out.n_vertices, 1), max(out.n_indices, 1)
+            continue
+        if st != 0:
+            raise TessellationError(st)
+        break
+    bufs.vertices = np.concatenate([bufs.vertices, v[:out.n_vertices]])
+    bufs.indices = np.concatenate([bufs.indices, i[:out.n_indices]])
+
+
 class StrokeTessellator:
     """StrokeTessellator (stroke.rs:108-308) — fixed line width only."""
 
@@ -244,6 +272,23 @@ class StrokeTessellator:
         _run(B.lib().lyg_stroke_path, path, options.to_c(), builder)
 
     tessellate = tessellate_path
+
+    # stroke.rs:255-307: the shapes are built as paths and stroked
+    def tessellate_polygon(self, points, closed, options: StrokeOptions, builder: BuffersBuilder):
+        self.tessellate_path(_shape_path(lambda b: b.add_polygon(points, closed)), options, builder)
+
+    def tessellate_rectangle(self, rect, options: StrokeOptions, builder: BuffersBuilder):
+        self.tessellate_path(_shape_path(lambda b: b.add_rectangle(rect, True)), options, builder)
+
+    def tessellate_circle(self, center, radius, options: StrokeOptions, builder: BuffersBuilder):
+        self.tessellate_path(_shape_path(lambda b: b.add_circle(center, radius, True)), options,
+                             builder)
+
+    def tessellate_ellipse(self, center, radii, x_rotation, positive, options: StrokeOptions,
+                           builder: BuffersBuilder):
+        self.tessellate_path(
+            _shape_path(lambda b: b.add_ellipse(center, radii, x_rotation, positive)), options,
+            builder)
 
 
 class FillTessellator:
@@ -255,6 +300,30 @@ class FillTessellator:
         _run(B.lib().lyg_fill_path, path, options.to_c(), builder)
 
     tessellate = tessellate_path
+
+    def tessellate_polygon(self, points, closed, options: FillOptions, builder: BuffersBuilder):
+        """fill.rs:644-651: the polygon's path events."""
+        self.tessellate_path(_shape_path(lambda b: b.add_polygon(points, closed)), options, builder)
+
+    def tessellate_rectangle(self, rect, options: FillOptions, builder: BuffersBuilder):
+        """fill.rs:653-661 -> basic_shapes::fill_rectangle (lyg_fill_rectangle)."""
+        r = np.asarray(rect, np.float32)
+        _run_direct(lambda out: B.lib().lyg_fill_rectangle(
+            _Ctx.get(), r.ctypes.data_as(B.F32P), C.byref(out)), builder)
+
+    def tessellate_circle(self, center, radius, options: FillOptions, builder: BuffersBuilder):
+        """fill.rs:663-672 -> basic_shapes::fill_circle (lyg_fill_circle)."""
+        o = options.to_c()
+        _run_direct(lambda out: B.lib().lyg_fill_circle(
+            _Ctx.get(), float(center[0]), float(center[1]), float(radius), C.byref(o),
+            C.byref(out)), builder)
+
+    def tessellate_ellipse(self, center, radii, x_rotation, positive, options: FillOptions,
+                           builder: BuffersBuilder):
+        """fill.rs:674-690: the ellipse path, filled with intersections off."""
+        self.tessellate_path(
+            _shape_path(lambda b: b.add_ellipse(center, radii, x_rotation, positive)),
+            options.with_intersections(False), builder)
 
 
 # ---- lyon_geom flattening (array form) --------------------------------------------------

@@ -194,3 +194,39 @@ def winding_compare(path, tolerance, fill_rule, v, i, box=None, n=4096, eps=1e-4
     r = {k: int(out[j]) for j, k in enumerate(keys)}
     r["area_a"] = float(area[0])
     return r
+
+
+def fill_circle(center, radius, tolerance):
+    """basic_shapes::fill_circle (crates/tessellation/src/basic_shapes.rs:40-205) restated
+    with numpy float32 arithmetic: (vertices (n, 2) float32, indices (3 m,) local ids)."""
+    f = np.float32
+    cx, cy = f(center[0]), f(center[1])
+    r = f(abs(f(radius)))
+    if r == 0:
+        return np.zeros((0, 2), np.float32), np.zeros(0, np.uint32)
+    verts = [(cx + f(-1.0) * r, cy + f(0.0) * r), (cx + f(0.0) * r, cy + f(-1.0) * r),
+             (cx + f(1.0) * r, cy + f(0.0) * r), (cx + f(0.0) * r, cy + f(1.0) * r)]
+    tris = [(0, 3, 1), (1, 3, 2)]
+    pi = f(np.pi)
+    angles = [(pi, f(1.5) * pi), (f(1.5) * pi, f(2.0) * pi), (f(0.0), pi * f(0.5)),
+              (pi * f(0.5), pi)]
+    arc_len = f(0.5) * pi * r
+    tol = min(f(tolerance), r)
+    step = f(2.0) * np.sqrt(f(2.0) * tol * r - tol * tol, dtype=np.float32)
+    n_seg = np.ceil(arc_len / step, dtype=np.float32)
+    lg = np.log2(n_seg, dtype=np.float32)
+    n = int(lg) if lg > 0 else 0
+
+    def border(a0, a1, va, vb, k):
+        if k == 0:
+            return
+        mid = (a0 + a1) * f(0.5)
+        v = len(verts)
+        verts.append((cx + np.cos(mid, dtype=np.float32) * r, cy + np.sin(mid, dtype=np.float32) * r))
+        tris.append((vb, v, va))
+        border(a0, mid, va, v, k - 1)
+        border(mid, a1, v, vb, k - 1)
+
+    for i in range(4):
+        border(angles[i][0], angles[i][1], i, (i + 1) % 4, n)
+    return np.asarray(verts, np.float32).reshape(-1, 2), np.asarray(tris, np.uint32).reshape(-1)

@@ -278,3 +278,26 @@ int main() {
     r = subprocess.run([tmp], capture_output=True, text=True)
     tot, bad = map(int, r.stdout.split())
     assert tot > 1_000_000 and bad == 0
+
+
+@pytest.mark.parametrize("radius,tol,n_vertices", [(100.0, 0.1, 64), (3.0, 0.01, 32),
+                                                   (1e-3, 0.1, 4)])
+def test_fill_circle_restatement(radius, tol, n_vertices):
+    """basic_shapes.rs:209-231 fills a circle of radius 100 at the default tolerance.  The
+    counts are worked by hand: r=100, t=0.1 gives a step 2 sqrt(2 t r - t^2) = 8.943, a
+    quarter arc of 157.08 = 18 steps, floor(log2 18) = 4 halvings per quarter, 4 * 2^4
+    vertices.  The topology follows from the geometry: after the two base triangles, the
+    k-th triangle adds vertex k+2 on the bisector of its other two corners."""
+    v, i = O.fill_circle((1.0, 2.0), radius, tol)
+    assert len(v) == n_vertices and len(i) == 3 * (n_vertices - 2)
+    d = np.hypot(v[:, 0] - 1.0, v[:, 1] - 2.0)
+    assert np.all(np.abs(d - radius) <= 1e-5 * radius + 1e-7)
+    tri = i.reshape(-1, 3).astype(np.int64)
+    assert tri[:2].tolist() == [[0, 3, 1], [1, 3, 2]]
+    assert np.array_equal(tri[2:, 1], np.arange(4, len(v)))
+    ang = np.arctan2(v[:, 1].astype(np.float64) - 2.0, v[:, 0].astype(np.float64) - 1.0)
+    a, b, m = ang[tri[2:, 2]], ang[tri[2:, 0]], ang[tri[2:, 1]]
+    bis = a + 0.5 * np.angle(np.exp(1j * (b - a)))
+    assert np.all(np.abs(np.angle(np.exp(1j * (m - bis)))) <= 1e-4)
+    # each added triangle spans at most a quarter turn of the circle
+    assert np.all(np.abs(np.angle(np.exp(1j * (b - a)))) <= math.pi / 2 + 1e-6)
