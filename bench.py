@@ -14,6 +14,8 @@ The same JSON line carries:
     a sample of the paths, and the batched/plain ratio;
   * flatten: lyon_geom flattening of the stroke workload's segments with device-resident
     inputs and outputs (points/s and the emit kernel's roofline);
+  * validators: lyon_algorithms' approximate_signed_area over the stroke workload's paths
+    and hit-test winding numbers of 100,000 query points, device memory in and out;
   * fill: the batched fill workload (4,096 closed paths, half self-intersecting star
     polygons with 5-512 points, half loops of 4-256 cubics, EvenOdd/NonZero 50/50,
     tolerance 0.05, coordinates in [0, 4096]^2) timed the same way, with its plain
@@ -80,6 +82,7 @@ def main():
     ap.add_argument("--no-fill", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-flatten", action="store_true")
+    ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
                     help="HBM traffic per emit launch measured with rocprofv3 counters")
@@ -220,6 +223,37 @@ def main():
                              "kernel": "flatten_emit_kernel", "algorithmic_bytes": fl_bytes}}
         del dk, dc, doff, dout
 
+    # ---- path validators (lyon_algorithms hit_test / area) -----------------------------
+    validators = None
+    if not args.no_validate:
+        from lyon_amd import algorithms as A
+        from lyon_amd.batch import DeviceBuffer
+        log("validators: signed areas and winding numbers over the stroke workload")
+        rng = np.random.default_rng(0)
+        nq = 100_000
+        qp = rng.integers(0, args.paths, nq).astype(np.uint32)
+        xy = (rng.random((nq, 2)) * 2048.0).astype(np.float32)
+        dq, dp = DeviceBuffer.from_numpy(xy.reshape(-1)), DeviceBuffer.from_numpy(qp)
+        dw, da = DeviceBuffer(4 * nq), DeviceBuffer(4 * args.paths)
+        A.signed_areas_device(db, 0.05, da)
+        A.winding_numbers_device(db, dq, dp, nq, 0.05, FillRule.NonZero, dw)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            A.signed_areas_device(db, 0.05, da)
+        area_ms = (time.perf_counter() - t0) * 1e3 / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            A.winding_numbers_device(db, dq, dp, nq, 0.05, FillRule.NonZero, dw)
+        wind_ms = (time.perf_counter() - t0) * 1e3 / reps
+        validators = {
+            "workload": f"approximate_signed_area of the {args.paths} stroke-workload paths; "
+                        f"{nq} winding-number queries (uniform points in [0,2048]^2, each "
+                        "against a random path of the batch), tol 0.05, device memory in/out",
+            "signed_areas": {"ms": area_ms, "paths_per_s": args.paths / (area_ms / 1e3)},
+            "winding_numbers": {"ms": wind_ms, "queries_per_s": nq / (wind_ms / 1e3)}}
+        del dq, dp, dw, da
+
     # ---- batched fill -------------------------------------------------------------------
     fill = None
     fbatch = fopts = None
@@ -355,6 +389,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
         "flatten": flat,
+        "validators": validators,
         "fill": fill,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -213,6 +213,30 @@ int32_t lyg_stroke_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_
 int32_t lyg_fill_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
                        uint32_t n_opts, lyg_batch_out* out);
 
+/* ---- lyon_algorithms path validators ---------------------------------------------------
+ * path_winding_number_at_position / hit_test_path (crates/algorithms/src/hit_test.rs:8-83)
+ * and approximate_signed_area (crates/algorithms/src/area.rs:7-19), bit-exact with lyon's
+ * f32 arithmetic.  Paths are as lyon's builders make them (every sub-path opens with
+ * Begin).  tolerance must be >= f32::EPSILON^2 (for_each_flattened's own precondition,
+ * quadratic_bezier.rs:344), else LYG_ERR_TOLERANCE_IS_NAN.  fill_rule: 0 EvenOdd,
+ * 1 NonZero (FillRule::is_in, path/src/lib.rs:267-280).
+ *
+ * Batched, device pointers: query i is the point queries[2i], queries[2i+1] against path
+ * query_paths[i] of the batch (an index past the batch gives winding 0); writes winding[i]
+ * and, when inside is not null, inside[i] = hit_test_path's answer. */
+int32_t lyg_winding_numbers(lyg_ctx* ctx, const lyg_batch_in* in, const float* queries,
+                            const uint32_t* query_paths, uint32_t n_queries, float tolerance,
+                            uint8_t fill_rule, int32_t* winding, uint8_t* inside);
+/* approximate_signed_area of every path of a device batch into areas[n_paths] (device). */
+int32_t lyg_signed_areas(lyg_ctx* ctx, const lyg_batch_in* in, float tolerance, float* areas);
+/* One path and n query points, host pointers (inside may be null). */
+int32_t lyg_path_winding_numbers(lyg_ctx* ctx, const lyg_path_view* path, const float* xy,
+                                 uint32_t n, float tolerance, uint8_t fill_rule,
+                                 int32_t* winding, uint8_t* inside);
+/* approximate_signed_area of one path, host pointers. */
+int32_t lyg_approximate_signed_area(lyg_ctx* ctx, const lyg_path_view* path, float tolerance,
+                                    float* area);
+
 /* Size query for the batched stroke: per-path vertex/index counts into device arrays
  * (n entries each) and totals.  Same kernels as lyg_stroke_batch's first pass. */
 int32_t lyg_stroke_batch_count(lyg_ctx* ctx, const lyg_batch_in* in,

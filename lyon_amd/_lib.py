@@ -107,6 +107,13 @@ SIGNATURES = {
     "lyg_flatten_segments": ([C.c_void_p, I32P, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32,
                               U32P, U32P], C.c_int32),
     "lyg_fill_rectangle": ([C.c_void_p, F32P, C.POINTER(OutputC)], C.c_int32),
+    "lyg_winding_numbers": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_void_p, C.c_uint32,
+                             C.c_float, C.c_uint8, C.c_void_p, C.c_void_p], C.c_int32),
+    "lyg_signed_areas": ([C.c_void_p, C.POINTER(BatchInC), C.c_float, C.c_void_p], C.c_int32),
+    "lyg_path_winding_numbers": ([C.c_void_p, C.POINTER(PathViewC), F32P, C.c_uint32, C.c_float,
+                                  C.c_uint8, C.POINTER(C.c_int32), U8P], C.c_int32),
+    "lyg_approximate_signed_area": ([C.c_void_p, C.POINTER(PathViewC), C.c_float,
+                                     C.POINTER(C.c_float)], C.c_int32),
     "lyg_fill_circle": ([C.c_void_p, C.c_float, C.c_float, C.c_float, C.POINTER(FillOptionsC),
                          C.POINTER(OutputC)], C.c_int32),
     "lyg_flatten_segments_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,

@@ -109,4 +109,23 @@ struct FlattenLaunch {
     float* out_points;
 };
 
+// Path validators (lyon_algorithms hit_test.rs / area.rs) over the batched layout.
+struct ValidateLaunch {
+    const uint8_t* verbs;
+    const float* points;
+    const uint32_t* verb_off;   // n_paths + 1
+    const uint32_t* point_off;  // n_paths + 1
+    uint32_t n_paths;
+    float tolerance;
+    // winding numbers: one query point per entry, against path query_path[i]
+    const float* queries;  // x, y pairs
+    const uint32_t* query_path;
+    uint32_t n_queries;
+    int fill_rule;          // 0 EvenOdd, 1 NonZero (for `inside`)
+    int32_t* winding;       // n_queries
+    uint8_t* inside;        // n_queries or null
+    // areas: one per path
+    float* areas;
+};
+
 }  // namespace lyg

@@ -1,0 +1,225 @@
+// lyon_algorithms' path validators for gfx950, over the batched path layout:
+//
+//   * winding numbers / hit tests of query points (path_winding_number_at_position and
+//     hit_test_path, crates/algorithms/src/hit_test.rs:8-117): one wave per query point.
+//     The lanes take the path's verbs 64 at a time; a DPP scan of the points each verb
+//     consumes gives every lane its verb's points, a running maximum gives the latest
+//     Begin (the sub-path's first point, which End and Close connect back to).  Each lane
+//     walks the flattened points of its own segment and counts the crossings left of the
+//     query; the count is an integer, so the wave sum is exact in any order.
+//   * approximate signed areas (approximate_signed_area, crates/algorithms/src/area.rs:7-54):
+//     one lane per path.  The area is lyon's serial f32 sum of cross products in event order,
+//     so one lane walks one path in that order.
+//
+// Paths are as lyon's builders make them: every sub-path opens with Begin, so the current
+// point of a verb is the last point consumed before it (path.rs:843-909).  Curves are
+// flattened as for_each_flattened / Flattened do (quadratic_bezier.rs:337-359, 975-1031;
+// cubic_bezier.rs:563-582, 1405-1454): t = step, step + step, ..., the last point `to`.
+#include <hip/hip_runtime.h>
+
+#include "lyg_math.h"
+#include "lyg_types.h"
+#include "wave.h"
+
+namespace lyg {
+
+namespace {
+
+constexpr int VAL_WAVES = 4;
+
+// Points each verb consumes (path.rs:29-36): Line 1, Quadratic 2, Cubic 3, Begin 1,
+// Close 1 (the stored first point), End 0.
+__device__ __forceinline__ uint32_t points_of(uint32_t verb) {
+    return verb < 5u ? (verb == 2u ? 3u : verb == 1u ? 2u : 1u) : 0u;
+}
+
+// One path's points; reading past them gives NaN, as PointIter::next does.
+struct PathPts {
+    const float* p;
+    uint32_t n;
+    __device__ __forceinline__ vec2 at(uint32_t i) const {
+        if (i < n) return mk(p[2ull * i], p[2ull * i + 1]);
+        return mk(__builtin_nanf(""), __builtin_nanf(""));
+    }
+};
+
+// hit_test.rs:85-117 test_segment; LineSegment::sample is euclid's lerp.
+__device__ __forceinline__ int seg_winding(vec2 P, vec2 a, vec2 b) {
+    const float min_y = ffmin(a.y, b.y), max_y = ffmax(a.y, b.y);
+    if (min_y > P.y || max_y <= P.y || ffmin(a.x, b.x) > P.x) return 0;
+    if (a.y == b.y) return 0;
+    const float d = b.y - a.y;
+    const float t = (P.y - a.y) / d;
+    const float x = lerp(a, b, t).x;
+    if (x > P.x) return 0;
+    return d > 0.0f ? 1 : -1;
+}
+
+// Flattened points of a quadratic (kind 1) or cubic (kind 2) in lyon's order; f(p) is
+// called for every point after `from`, the last one being `to`.
+template <class F>
+__device__ __forceinline__ void for_each_flat_point(int kind, vec2 from, vec2 c1, vec2 c2,
+                                                    vec2 to, float tol, F&& f) {
+    vec2 a1, a2, a3 = mk(0.0f, 0.0f);
+    float n;
+    if (kind == 1) {
+        n = quad_wang(from, c1, c2, tol);  // c2 holds the quadratic's end point
+        a1 = mul(sub(c1, from), 2.0f);
+        a2 = sub(add(from, c2), mul(c1, 2.0f));
+    } else {
+        n = cubic_wang(from, c1, c2, to, tol);
+        a1 = mul(sub(c1, from), 3.0f);
+        a2 = add(sub(mul(from, 3.0f), mul(c1, 6.0f)), mul(c2, 3.0f));
+        a3 = add(sub(to, from), mul(sub(c1, c2), 3.0f));
+    }
+    const float step = 1.0f / n;
+    const uint32_t cnt = seg_count(n) - 1u;
+    float t = 0.0f;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        t = t + step;
+        vec2 v = from;
+        float tt = t;
+        v = add(v, mul(a1, tt));
+        tt = tt * t;
+        v = add(v, mul(a2, tt));
+        if (kind == 2) {
+            tt = tt * t;
+            v = add(v, mul(a3, tt));
+        }
+        f(v);
+    }
+    f(to);
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(WAVE* VAL_WAVES) winding_kernel(ValidateLaunch L) {
+    const int l = lane_id();
+    const uint32_t stride = gridDim.x * VAL_WAVES;
+    for (uint32_t q = blockIdx.x * VAL_WAVES + threadIdx.x / WAVE; q < L.n_queries; q += stride) {
+        const uint32_t pid = L.query_path[q];
+        int32_t w = 0;
+        if (pid < L.n_paths) {
+            const vec2 P = mk(L.queries[2ull * q], L.queries[2ull * q + 1]);
+            const uint32_t vb = L.verb_off[pid], nv = L.verb_off[pid + 1] - vb;
+            const uint32_t pb = L.point_off[pid];
+            const PathPts pts{L.points + 2ull * pb, L.point_off[pid + 1] - pb};
+            uint32_t pc = 0;   // points consumed by the verbs before this window
+            uint32_t beg = 0;  // 1 + point index of the latest Begin so far, 0 before any
+            for (uint32_t c = 0; c < nv; c += WAVE) {
+                const uint32_t i = c + (uint32_t)l;
+                const uint32_t verb = i < nv ? (uint32_t)L.verbs[vb + i] : 0xffu;
+                const uint32_t cons = points_of(verb);
+                const uint32_t incl = incl_scan(cons);
+                const uint32_t pi = pc + incl - cons;  // this verb's first point
+                uint32_t b = incl_max_u32(verb == 3u ? pi + 1u : 0u);
+                b = b > beg ? b : beg;
+                if (verb != 3u && verb != 0xffu) {
+                    const vec2 from = pts.at(pi - 1u);
+                    if (verb == 0u) {
+                        w += seg_winding(P, from, pts.at(pi));
+                    } else if (verb == 1u || verb == 2u) {
+                        const vec2 c1 = pts.at(pi), c2 = pts.at(pi + 1u);
+                        const vec2 to = verb == 1u ? c2 : pts.at(pi + 2u);
+                        // fast_bounding_range_y (quadratic_bezier.rs:561-566,
+                        // cubic_bezier.rs:906-920)
+                        float lo = ffmin(from.y, c1.y), hi = ffmax(from.y, c1.y);
+                        if (verb == 2u) {
+                            lo = ffmin(lo, c2.y);
+                            hi = ffmax(hi, c2.y);
+                        }
+                        lo = ffmin(lo, to.y);
+                        hi = ffmax(hi, to.y);
+                        if (!(lo > P.y || hi < P.y)) {
+                            vec2 prev = from;
+                            for_each_flat_point((int)verb, from, c1, c2, to, L.tolerance,
+                                                [&](vec2 p) {
+                                                    w += seg_winding(P, prev, p);
+                                                    prev = p;
+                                                });
+                        }
+                    } else {
+                        // End / Close: the edge from the current point back to the first
+                        w += seg_winding(P, from, pts.at(b - 1u));
+                    }
+                }
+                pc += readlane_u(incl, WAVE - 1);
+                beg = readlane_u(b, WAVE - 1);
+            }
+        }
+        w = (int32_t)readlane_u(incl_scan((uint32_t)w), WAVE - 1);
+        if (l == 0) {
+            L.winding[q] = w;
+            if (L.inside) L.inside[q] = L.fill_rule == 0 ? (w % 2 != 0) : (w != 0);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) signed_area_kernel(ValidateLaunch L) {
+    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= L.n_paths) return;
+    const uint32_t ve = L.verb_off[pid + 1];
+    const uint32_t pb = L.point_off[pid];
+    const PathPts pts{L.points + 2ull * pb, L.point_off[pid + 1] - pb};
+    uint32_t vi = L.verb_off[pid], pi = 0;
+    float area = 0.0f;
+    // approximate_signed_area: sub-path after sub-path until one is not Begin ... End
+    while (vi < ve && L.verbs[vi] == 3u) {
+        ++vi;
+        const vec2 first = pts.at(pi++);
+        vec2 cur = first;
+        float double_area = 0.0f;
+        vec2 v0 = mk(0.0f, 0.0f);
+        bool ended = false;
+        auto line_to = [&](vec2 to) {
+            const vec2 v1 = sub(to, first);
+            double_area = double_area + cross(v0, v1);
+            v0 = v1;
+        };
+        while (vi < ve) {
+            const uint32_t verb = L.verbs[vi++];
+            if (verb == 3u) break;  // a Begin inside a sub-path ends the sum
+            if (verb == 0u) {
+                cur = pts.at(pi++);
+                line_to(cur);
+            } else if (verb == 1u || verb == 2u) {
+                const vec2 c1 = pts.at(pi), c2 = pts.at(pi + 1u);
+                const vec2 to = verb == 1u ? c2 : pts.at(pi + 2u);
+                pi += verb == 1u ? 2u : 3u;
+                for_each_flat_point((int)verb, cur, c1, c2, to, L.tolerance,
+                                    [&](vec2 p) { line_to(p); });
+                cur = to;
+            } else {
+                // End / Close (path.rs:887-905): the closing term, last - first
+                if (verb == 4u) ++pi;
+                double_area = double_area + cross(v0, sub(cur, first));
+                ended = true;
+                break;
+            }
+        }
+        if (!ended) break;
+        area = area + double_area * 0.5f;
+    }
+    L.areas[pid] = area;
+}
+
+}  // namespace lyg
+
+extern "C" {
+
+hipError_t lyg_launch_winding(const lyg::ValidateLaunch* L, hipStream_t s) {
+    if (L->n_queries == 0) return hipSuccess;
+    uint32_t blocks = (L->n_queries + lyg::VAL_WAVES - 1) / lyg::VAL_WAVES;
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(lyg::winding_kernel, dim3(blocks), dim3(lyg::WAVE * lyg::VAL_WAVES), 0, s, *L);
+    return hipGetLastError();
+}
+
+hipError_t lyg_launch_signed_area(const lyg::ValidateLaunch* L, hipStream_t s) {
+    if (L->n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::signed_area_kernel, dim3((L->n_paths + 255) / 256), dim3(256), 0, s,
+                       *L);
+    return hipGetLastError();
+}
+
+}  // extern "C"

@@ -72,6 +72,24 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     x = x < t ? x : t;
     return readlane_u(x, WAVE - 1);
 }
+// Inclusive running maximum of v over the wave (same DPP pattern as wave_min_u32, with
+// 0 as the value that cannot win).
+__device__ __forceinline__ uint32_t incl_max_u32(uint32_t v) {
+    uint32_t x = v, t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x = x > t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x = x > t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x = x > t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x = x > t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x = x > t ? x : t;
+    t = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    x = x > t ? x : t;
+    return x;
+}
 __device__ __forceinline__ int highest_bit(uint64_t m) { return m ? 63 - __builtin_clzll(m) : -1; }
 __device__ __forceinline__ uint64_t mask_lt(int l) { return l >= 64 ? ~0ull : ((1ull << l) - 1ull); }
 __device__ __forceinline__ uint64_t mask_le(int l) { return l >= 63 ? ~0ull : ((1ull << (l + 1)) - 1ull); }

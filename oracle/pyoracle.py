@@ -48,6 +48,14 @@ def lib():
         L.orc_batch_triangles.argtypes = [U8P, F32P, U32P, U32P, F32P, C.c_uint32, C.c_int,
                                           C.c_int]
         L.orc_batch_triangles.restype = C.c_uint64
+        I32P = C.POINTER(C.c_int32)
+        L.orc_winding_number.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float,
+                                         C.c_float, C.c_float]
+        L.orc_winding_number.restype = C.c_int32
+        L.orc_winding_numbers.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, F32P, C.c_uint32,
+                                          C.c_float, I32P]
+        L.orc_signed_area.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float]
+        L.orc_signed_area.restype = C.c_float
         _LIB = L
     return _LIB
 
@@ -194,6 +202,31 @@ def winding_compare(path, tolerance, fill_rule, v, i, box=None, n=4096, eps=1e-4
     r = {k: int(out[j]) for j, k in enumerate(keys)}
     r["area_a"] = float(area[0])
     return r
+
+
+def winding_numbers(path, xy, tolerance):
+    """path_winding_number_at_position (hit_test.rs:21-83) of every point of xy (k, 2)."""
+    v = np.ascontiguousarray(path.verbs, np.uint8)
+    p = np.ascontiguousarray(path.points, np.float32)
+    q = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    out = np.zeros(len(q), np.int32)
+    lib().orc_winding_numbers(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), _ptr(q, F32P),
+                              len(q), tolerance, out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out
+
+
+def hit_test(path, point, fill_rule, tolerance):
+    """hit_test_path (hit_test.rs:8-18): fill_rule 0 EvenOdd, 1 NonZero."""
+    w = int(winding_numbers(path, [point], tolerance)[0])
+    return (w % 2 != 0) if fill_rule == 0 else (w != 0)
+
+
+def signed_area(path, tolerance):
+    """approximate_signed_area (area.rs:7-54)."""
+    v = np.ascontiguousarray(path.verbs, np.uint8)
+    p = np.ascontiguousarray(path.points, np.float32)
+    return float(np.float32(lib().orc_signed_area(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p),
+                                                  tolerance)))
 
 
 def fill_circle(center, radius, tolerance):

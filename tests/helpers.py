@@ -148,3 +148,73 @@ def fill_corpus():
     out += earcut_fixtures()
     out += star_paths(12)
     return out
+
+
+def flattened_subpaths(path, tolerance):
+    """The flattened sub-paths of a path as float32 polylines [first, ..., last], each
+    implicitly closed, from the oracle's segment flattening (one call for all curves)."""
+    from oracle import pyoracle as O
+    verbs = np.asarray(path.verbs)
+    pts = np.asarray(path.points, np.float32).reshape(-1, 2)
+    npv = np.array([1, 2, 3, 1, 1, 0], np.int64)[verbs]
+    first = np.cumsum(npv) - npv
+    curves = [i for i, v in enumerate(verbs) if v in (1, 2)]
+    flat = {}
+    if curves:
+        kinds = np.array([verbs[i] for i in curves], np.int32)
+        ctrl = np.zeros((len(curves), 4, 2), np.float32)
+        for k, i in enumerate(curves):
+            ctrl[k, 0] = pts[first[i] - 1]
+            for j in range(verbs[i] + 1):
+                ctrl[k, j + 1] = pts[first[i] + j]
+        fp, fo = O.flatten_segments(kinds, ctrl, tolerance)
+        for k, i in enumerate(curves):
+            flat[i] = fp[fo[k]:fo[k + 1]]
+    subs, cur = [], None
+    for i, v in enumerate(verbs):
+        if v == 3:
+            cur = [pts[first[i]]]
+        elif v == 0:
+            cur.append(pts[first[i]])
+        elif v in (1, 2):
+            cur.extend(flat[i])
+        else:
+            subs.append(np.asarray(cur, np.float32).reshape(-1, 2))
+            cur = None
+    return subs
+
+
+def area64(subs):
+    """Shoelace area of implicitly closed polylines, in float64."""
+    a = 0.0
+    for s in subs:
+        d = s.astype(np.float64) - s[0].astype(np.float64)
+        a += 0.5 * float(np.sum(d[:-1, 0] * d[1:, 1] - d[:-1, 1] * d[1:, 0]))
+    return a
+
+
+def winding64(subs, xy):
+    """Textbook winding numbers of points xy (k, 2) against the closed polylines: edges
+    whose y range [min, max) holds the point's y and that cross its horizontal line at or
+    left of it count +1 going down (+y), -1 going up; float64.  Also returns each point's
+    distance to the nearest edge."""
+    q = np.asarray(xy, np.float64).reshape(-1, 2)
+    w = np.zeros(len(q), np.int64)
+    dmin = np.full(len(q), np.inf)
+    for s in subs:
+        a = s.astype(np.float64)
+        b = np.roll(a, -1, axis=0)
+        for (x0, y0), (x1, y1) in zip(a, b):
+            lo, hi = min(y0, y1), max(y0, y1)
+            dy = y1 - y0
+            ex, ey = x1 - x0, y1 - y0
+            L2 = ex * ex + ey * ey
+            if L2 > 0:
+                t = np.clip(((q[:, 0] - x0) * ex + (q[:, 1] - y0) * ey) / L2, 0, 1)
+                dmin = np.minimum(dmin, np.hypot(q[:, 0] - (x0 + t * ex), q[:, 1] - (y0 + t * ey)))
+            if dy == 0:
+                continue
+            inr = (q[:, 1] >= lo) & (q[:, 1] < hi)
+            xc = x0 + (q[:, 1] - y0) * (x1 - x0) / dy
+            w += np.where(inr & (xc <= q[:, 0]), 1 if dy > 0 else -1, 0)
+    return w, dmin

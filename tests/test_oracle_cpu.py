@@ -301,3 +301,103 @@ def test_fill_circle_restatement(radius, tol, n_vertices):
     assert np.all(np.abs(np.angle(np.exp(1j * (m - bis)))) <= 1e-4)
     # each added triangle spans at most a quarter turn of the circle
     assert np.all(np.abs(np.angle(np.exp(1j * (b - a)))) <= math.pi / 2 + 1e-6)
+
+
+def _hit_test_kat_path():
+    # hit_test.rs:119-133
+    b = PathBuilder()
+    for sub in ([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)],
+                [(0.25, 0.25), (0.75, 0.25), (0.75, 0.75), (0.20, 0.75)]):
+        b.begin(sub[0])
+        for p in sub[1:]:
+            b.line_to(p)
+        b.end(True)
+    return b.build()
+
+
+def test_hit_test_reference_kats():
+    """hit_test.rs:119-197: the reference's own known answers."""
+    p = _hit_test_kat_path()
+    for q in [(-1.0, 0.5), (2.0, 0.5), (2.0, 0.0), (0.5, -1.0), (0.5, 2.0), (0.5, 0.5)]:
+        assert not O.hit_test(p, q, 0, 0.1), q
+    assert O.hit_test(p, (0.5, 0.5), 1, 0.1)
+    assert O.hit_test(p, (0.2, 0.5), 0, 0.1)
+    assert O.hit_test(p, (0.8, 0.5), 0, 0.1)
+
+
+def test_signed_area_reference_kats():
+    """area.rs:80-130: +1 and -1 for the two windings of the unit square, 5 for the cross."""
+    b = PathBuilder()
+    for sub in ([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)],
+                [(0.0, 0.0), (0.0, 1.0), (1.0, 1.0), (1.0, 0.0)]):
+        b.begin(sub[0])
+        for q in sub[1:]:
+            b.line_to(q)
+        b.close()
+        one = b.build()
+        b = PathBuilder()
+        assert O.signed_area(one, 0.01) in (1.0, -1.0)
+    sq = PathBuilder()
+    sq.add_polygon([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)], True)
+    assert O.signed_area(sq.build(), 0.01) == 1.0
+    sq = PathBuilder()
+    sq.add_polygon([(0.0, 0.0), (0.0, 1.0), (1.0, 1.0), (1.0, 0.0)], True)
+    assert O.signed_area(sq.build(), 0.01) == -1.0
+    cross = [(0.0, 1.0), (1.0, 1.0), (1.0, 0.0), (2.0, 0.0), (2.0, 1.0), (3.0, 1.0), (3.0, 2.0),
+             (2.0, 2.0), (2.0, 3.0), (1.0, 3.0), (1.0, 2.0), (0.0, 2.0)]
+    b = PathBuilder()
+    b.add_polygon(cross, True)
+    assert O.signed_area(b.build(), 0.01) == 5.0
+
+
+def _validator_paths():
+    from helpers import star_paths
+    from lyon_amd.workloads import stroke_batch
+    paths = [p for _, p in star_paths(4, seed=3, max_points=60)]
+    sb = stroke_batch(6, 4, 40, 512.0, seed=9)
+    for i in range(len(sb)):
+        paths.append(sb.path(i))
+    b = PathBuilder()
+    b.add_circle((100.0, 50.0), 40.0, True)
+    # an ellipse as add_ellipse builds it (builder.rs:606-636), quadratics from the oracle
+    sweep = -2.0 * float(np.float32(np.pi))
+    q, _ = O.arc_to_quads(np.array([[300.0, 60.0, 90.0, 30.0, 0.0, sweep, 0.3]], np.float32))
+    b.begin(q[0, 0])
+    for k in range(len(q)):
+        b.quadratic_bezier_to(q[k, 1], q[k, 2])
+    b.end(True)
+    paths.append(b.build())
+    return paths
+
+
+def test_signed_area_matches_shoelace():
+    """The f32 serial sum against a float64 shoelace over the same flattened points: equal
+    to f32 rounding (a dropped closing term or a wrong sign would be off by far more).
+    A circle's area is pi r^2 within the flattening error (perimeter * tolerance)."""
+    from helpers import area64, flattened_subpaths
+    for p in _validator_paths():
+        a = O.signed_area(p, 0.05)
+        ref = area64(flattened_subpaths(p, 0.05))
+        scale = float(np.abs(np.asarray(p.points, np.float64)).max()) ** 2
+        assert abs(a - ref) <= 1e-5 * scale + 1e-6 * abs(ref), (a, ref)
+    b = PathBuilder()
+    b.add_circle((10.0, 20.0), 100.0, True)
+    a = O.signed_area(b.build(), 0.01)
+    assert abs(a - math.pi * 1e4) <= 2 * math.pi * 100 * 0.01 + 1e-3 * 1e4 * 1e-2
+
+
+def test_winding_matches_crossing_count():
+    """Winding numbers against a float64 crossing count over the flattened edges, at
+    random points away (> 1e-3) from every edge."""
+    from helpers import flattened_subpaths, winding64
+    rng = np.random.default_rng(4)
+    for p in _validator_paths():
+        pts = np.asarray(p.points, np.float64)
+        lo, hi = pts.min(axis=0) - 5, pts.max(axis=0) + 5
+        xy = (lo + rng.random((400, 2)) * (hi - lo)).astype(np.float32)
+        w = O.winding_numbers(p, xy, 0.05)
+        ref, dmin = winding64(flattened_subpaths(p, 0.05), xy)
+        far = dmin > 1e-3
+        assert far.sum() > 300
+        assert np.array_equal(w[far], ref[far])
+        assert np.any(w != 0)
