@@ -16,6 +16,8 @@ The same JSON line carries:
     inputs and outputs (points/s and the emit kernel's roofline);
   * validators: lyon_algorithms' approximate_signed_area over the stroke workload's paths
     and hit-test winding numbers of 100,000 query points, device memory in and out;
+  * ingest: the stroke workload written as SVG path data and parsed on the device
+    (lyon_extra's PathParser), text and paths in device memory;
   * fill: the batched fill workload (4,096 closed paths, half self-intersecting star
     polygons with 5-512 points, half loops of 4-256 cubics, EvenOdd/NonZero 50/50,
     tolerance 0.05, coordinates in [0, 4096]^2) timed the same way, with its plain
@@ -83,6 +85,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-flatten", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
                     help="HBM traffic per emit launch measured with rocprofv3 counters")
@@ -254,6 +257,43 @@ def main():
             "winding_numbers": {"ms": wind_ms, "queries_per_s": nq / (wind_ms / 1e3)}}
         del dq, dp, dw, da
 
+    # ---- SVG path data ingest (lyon_extra PathParser) ---------------------------------
+    ingest = None
+    if not args.no_ingest:
+        from lyon_amd import svg as S
+        from lyon_amd.batch import DeviceBuffer
+        log("ingest: the stroke workload as SVG path data")
+        pts9 = np.char.mod("%.9g", batch.points.astype(np.float64)).reshape(-1, 2)
+        pairs = np.char.add(np.char.add(pts9[:, 0], " "), pts9[:, 1])
+        words = {0: "L ", 1: "Q ", 2: "C ", 3: "M "}
+        strings = []
+        for i in range(args.paths):
+            v0, v1 = int(batch.verb_offsets[i]), int(batch.verb_offsets[i + 1])
+            pi, parts = int(batch.point_offsets[i]), []
+            for v in batch.verbs[v0:v1]:
+                k = {0: 1, 1: 2, 2: 3, 3: 1}.get(int(v), 0)
+                if k:
+                    parts.append(words[int(v)] + " ".join(pairs[pi:pi + k]))
+                pi += k + (1 if int(v) == 4 else 0)
+            strings.append(" ".join(parts))
+        text, offs = S.pack_strings(strings)
+        td, od = DeviceBuffer.from_numpy(text), DeviceBuffer.from_numpy(offs)
+        r = S.parse_paths_device(td, od, args.paths)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = S.parse_paths_device(td, od, args.paths)
+        ing_ms = (time.perf_counter() - t0) * 1e3 / reps
+        ok = (r.n_verbs == len(batch.verbs) and r.n_points == len(batch.points) and
+              r.n_errors == 0)
+        ingest = {"workload": f"the {args.paths} stroke-workload paths as absolute SVG path "
+                              "data (9 significant digits), parsed on the device (count pass, "
+                              "offset scans, emit pass), text and paths in device memory",
+                  "text_bytes": int(len(text)), "ms": ing_ms,
+                  "paths_per_s": args.paths / (ing_ms / 1e3),
+                  "text_gb_per_s": len(text) / (ing_ms / 1e3) / 1e9, "round_trip_counts_ok": ok}
+        del td, od, r
+
     # ---- batched fill -------------------------------------------------------------------
     fill = None
     fbatch = fopts = None
@@ -390,6 +430,7 @@ def main():
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
         "flatten": flat,
         "validators": validators,
+        "ingest": ingest,
         "fill": fill,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -237,6 +237,46 @@ int32_t lyg_path_winding_numbers(lyg_ctx* ctx, const lyg_path_view* path, const 
 int32_t lyg_approximate_signed_area(lyg_ctx* ctx, const lyg_path_view* path, float tolerance,
                                     float* area);
 
+/* ---- SVG path data ingest ---------------------------------------------------------------
+ * lyon_extra's PathParser::parse (crates/extra/src/parser.rs:154-462) into lyon_path's Path
+ * layout (BuilderImpl, crates/path/src/path.rs:516-611): one path per string, all strings
+ * parsed at once on the device, the paths written end to end as a batch (the layout of
+ * lyon_path's PathBuffer, path_buffer.rs:14-104, with n+1 offsets).  Numbers convert as
+ * Rust's str::parse::<f32> does (correctly rounded); arcs become quadratics as
+ * SvgArc::to_arc + Arc::for_each_quadratic_bezier_with_t make them (arc.rs:58-146, 668-701).
+ *
+ * Per path, errors[4*i .. 4*i+3] = kind, line, column, character (ParseError's fields,
+ * parser.rs:13-33); a path with an error keeps what the parser had built before it, as
+ * lyon's builder does.  num_attributes: numbers read after every endpoint (the extended
+ * syntax) and dropped, since this path layout has no attributes.  stop_at: an ASCII
+ * character that ends the parse (ParserOptions::stop_at) or -1.  Only ASCII text: a
+ * non-ASCII byte reached by the parser ends the path with LYG_SVG_ERR_NON_ASCII. */
+enum {
+    LYG_SVG_OK = 0,
+    LYG_SVG_ERR_NUMBER = 1,           /* ParseError::Number */
+    LYG_SVG_ERR_FLAG = 2,             /* ParseError::Flag */
+    LYG_SVG_ERR_COMMAND = 3,          /* ParseError::Command */
+    LYG_SVG_ERR_MISSING_MOVE_TO = 4,  /* ParseError::MissingMoveTo */
+    LYG_SVG_ERR_NON_ASCII = 5,        /* not in lyon: non-ASCII input */
+    LYG_SVG_ERR_UNDECIDED_NUMBER = 6  /* not in lyon: more than 19 significant digits whose
+                                         rounding they do not decide */
+};
+/* Device pointers: text[text_offsets[i] .. text_offsets[i+1]) is string i.  verb_offsets /
+ * point_offsets (n+1) and errors (4n, may be null) are written; verbs / points are written
+ * when their capacities (in verbs and in points) hold n_verbs / n_points, else
+ * LYG_ERR_CAPACITY with the required sizes set.  n_errors: paths with an error. */
+int32_t lyg_parse_svg_paths(lyg_ctx* ctx, const uint8_t* text, const uint32_t* text_offsets,
+                            uint32_t n, uint32_t num_attributes, int32_t stop_at,
+                            uint8_t* verbs, uint64_t verb_capacity, float* points,
+                            uint64_t point_capacity, uint32_t* verb_offsets,
+                            uint32_t* point_offsets, int32_t* errors, uint64_t* n_verbs,
+                            uint64_t* n_points, uint32_t* n_errors);
+/* One string, host pointers (the lyon call: PathParser::parse into a Path builder). */
+int32_t lyg_parse_svg_path(lyg_ctx* ctx, const char* text, uint32_t len,
+                           uint32_t num_attributes, int32_t stop_at, uint8_t* verbs,
+                           uint32_t verb_capacity, float* points, uint32_t point_capacity,
+                           uint32_t* n_verbs, uint32_t* n_points, int32_t* error);
+
 /* Size query for the batched stroke: per-path vertex/index counts into device arrays
  * (n entries each) and totals.  Same kernels as lyg_stroke_batch's first pass. */
 int32_t lyg_stroke_batch_count(lyg_ctx* ctx, const lyg_batch_in* in,

@@ -107,6 +107,13 @@ SIGNATURES = {
     "lyg_flatten_segments": ([C.c_void_p, I32P, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32,
                               U32P, U32P], C.c_int32),
     "lyg_fill_rectangle": ([C.c_void_p, F32P, C.POINTER(OutputC)], C.c_int32),
+    "lyg_parse_svg_paths": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                             C.c_int32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
+                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)], C.c_int32),
+    "lyg_parse_svg_path": ([C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_int32, U8P,
+                            C.c_uint32, F32P, C.c_uint32, U32P, U32P, C.POINTER(C.c_int32)],
+                           C.c_int32),
     "lyg_winding_numbers": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_void_p, C.c_uint32,
                              C.c_float, C.c_uint8, C.c_void_p, C.c_void_p], C.c_int32),
     "lyg_signed_areas": ([C.c_void_p, C.POINTER(BatchInC), C.c_float, C.c_void_p], C.c_int32),

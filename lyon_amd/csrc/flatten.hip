@@ -16,6 +16,7 @@
 #include "lyg_math.h"
 #include "lyg_types.h"
 #include "stroke_core.h"
+#include "arc_core.h"
 
 namespace lyg {
 
@@ -177,15 +178,6 @@ __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_kernel(FlattenLau
 }
 
 // ---- arcs -------------------------------------------------------------------------------
-struct ArcD {
-    vec2 center, radii;
-    float start, sweep, xrot;
-};
-__device__ __forceinline__ vec2 arc_sample(const ArcD& a, float t) {
-    float angle = a.start + a.sweep * t;
-    vec2 e = mk(a.radii.x * t_cos(angle), a.radii.y * t_sin(angle));
-    return add(a.center, rotate_v(a.xrot, e));
-}
 __device__ __forceinline__ float arc_step(const ArcD& a, float tol) {
     float r = vlen(sub(arc_sample(a, 0.0f), a.center));
     float ang = 2.0f * t_acos((r - tol) / r);
@@ -227,38 +219,21 @@ __global__ void arc_to_quads_kernel(const float* arcs, uint32_t n, uint32_t* cou
                                     const uint32_t* offsets, float* out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float PI = 3.14159265358979323846f;
-    const float FRAC_PI_4 = 0.785398163397448309616f;
     const float* a = arcs + 7ull * i;
     ArcD arc{mk(a[0], a[1]), mk(a[2], a[3]), a[4], a[5], a[6]};
-    float sign = __builtin_copysignf(1.0f, arc.sweep);
-    if (__builtin_isnan(arc.sweep)) sign = arc.sweep;
-    float sweep = ffmin(ffabs(arc.sweep), PI * 2.0f);
-    float n_steps = __builtin_ceilf(sweep / FRAC_PI_4);
-    int ns = (n_steps >= 0.0f && n_steps < 2147483647.0f) ? (int)n_steps : 0;
     if (counts) {
-        counts[i] = (uint32_t)ns;
+        counts[i] = (uint32_t)arc_quad_count(arc);
         return;
     }
-    float step = sweep / n_steps * sign;
-    float alpha = (float)tan((double)(step * 0.5f));
     float* o = out + 6ull * offsets[i];
-    for (int k = 0; k < ns; ++k) {
-        float a1 = arc.start + step * (float)k;
-        float a2 = arc.start + step * (float)(k + 1);
-        vec2 v1 = rotate_v(arc.xrot, mk(arc.radii.x * t_cos(a1), arc.radii.y * t_sin(a1)));
-        vec2 v2 = rotate_v(arc.xrot, mk(arc.radii.x * t_cos(a2), arc.radii.y * t_sin(a2)));
-        vec2 from = add(arc.center, v1);
-        vec2 to = add(arc.center, v2);
-        vec2 tan_ = rotate_v(arc.xrot, mk(-arc.radii.x * t_sin(a1), arc.radii.y * t_cos(a1)));
-        vec2 ctrl = add(from, mul(tan_, alpha));
+    arc_to_quads_each(arc, [&](int k, vec2 from, vec2 ctrl, vec2 to) {
         o[6 * k + 0] = from.x;
         o[6 * k + 1] = from.y;
         o[6 * k + 2] = ctrl.x;
         o[6 * k + 3] = ctrl.y;
         o[6 * k + 4] = to.x;
         o[6 * k + 5] = to.y;
-    }
+    });
 }
 
 // ---- exclusive scan of u32 counts into n+1 offsets (single workgroup) -----------------

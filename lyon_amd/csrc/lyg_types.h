@@ -128,4 +128,23 @@ struct ValidateLaunch {
     float* areas;
 };
 
+// SVG path data parse (svg.hip): one lane per string.
+struct SvgLaunch {
+    const uint8_t* text;
+    const uint32_t* text_off;  // n + 1
+    uint32_t n;
+    int num_attributes;
+    int stop_at;               // ASCII character or -1
+    // count pass
+    uint32_t* counts_v;
+    uint32_t* counts_p;
+    int32_t* errors;           // 4 per string or null
+    uint32_t* n_errors;
+    // emit pass
+    const uint32_t* off_v;
+    const uint32_t* off_p;
+    uint8_t* verbs;
+    float* points;
+};
+
 }  // namespace lyg

@@ -56,6 +56,10 @@ def lib():
                                           C.c_float, I32P]
         L.orc_signed_area.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float]
         L.orc_signed_area.restype = C.c_float
+        L.orc_parse_svg.argtypes = [C.c_char_p, C.c_uint32, C.c_int, C.c_int, U8P, C.c_uint32,
+                                    F32P, C.c_uint32, U32P, U32P, C.POINTER(C.c_int32)]
+        L.orc_parse_f32.argtypes = [C.c_char_p, C.POINTER(C.c_float)]
+        L.orc_parse_f32.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -227,6 +231,31 @@ def signed_area(path, tolerance):
     p = np.ascontiguousarray(path.points, np.float32)
     return float(np.float32(lib().orc_signed_area(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p),
                                                   tolerance)))
+
+
+def parse_svg(text, num_attributes=0, stop_at=None):
+    """PathParser::parse (parser.rs:154-175) of one string into a Path layout.  Returns
+    (verbs uint8, points float32 (n, 2), error int32[4]: kind, line, column, char)."""
+    b = text.encode() if isinstance(text, str) else bytes(text)
+    nv, npt = C.c_uint32(0), C.c_uint32(0)
+    err = (C.c_int32 * 4)()
+    sa = -1 if stop_at is None else ord(stop_at)
+    cap_v = cap_p = 0
+    while True:
+        verbs = np.zeros(max(cap_v, 1), np.uint8)
+        pts = np.zeros((max(cap_p, 1), 2), np.float32)
+        lib().orc_parse_svg(b, len(b), num_attributes, sa, _ptr(verbs, U8P), cap_v,
+                            _ptr(pts, F32P), cap_p, C.byref(nv), C.byref(npt), err)
+        if nv.value <= cap_v and npt.value <= cap_p:
+            return verbs[:nv.value], pts[:npt.value], np.array(list(err), np.int32)
+        cap_v, cap_p = nv.value, npt.value
+
+
+def parse_f32(text):
+    """str::parse::<f32>: the float, or None for text Rust rejects."""
+    v = C.c_float(0)
+    ok = lib().orc_parse_f32(text.encode(), C.byref(v))
+    return float(np.float32(v.value)) if ok else None
 
 
 def fill_circle(center, radius, tolerance):

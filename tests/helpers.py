@@ -218,3 +218,135 @@ def winding64(subs, xy):
             xc = x0 + (q[:, 1] - y0) * (x1 - x0) / dy
             w += np.where(inr & (xc <= q[:, 0]), 1 if dy > 0 else -1, 0)
     return w, dmin
+
+
+def f32_exact(text):
+    """Correctly rounded (half to even) float32 of a decimal string, by exact rational
+    arithmetic: an independent restatement of what Rust's str::parse::<f32> returns."""
+    from fractions import Fraction
+    neg = text.startswith("-")
+    x = Fraction(text.lstrip("-") if not text.lstrip("-").startswith(".") else "0" + text.lstrip("-"))
+    if x == 0:
+        return np.float32(-0.0 if neg else 0.0)
+    k = x.numerator.bit_length() - x.denominator.bit_length()
+    if Fraction(2) ** k > x:
+        k -= 1
+    if k > 127:
+        bits = 0x7f800000
+    else:
+        s = 23 - k if k >= -126 else 149
+        y = x * Fraction(2) ** s
+        q, r = divmod(y.numerator, y.denominator)
+        c = 2 * r - y.denominator
+        if c > 0 or (c == 0 and q & 1):
+            q += 1
+        if k >= -126:
+            if q == 1 << 24:
+                q, k = 1 << 23, k + 1
+            bits = 0x7f800000 if k > 127 else ((k + 127) << 23) | (q - (1 << 23))
+        else:
+            bits = q
+    if neg:
+        bits |= 0x80000000
+    return np.array([bits], np.uint32).view(np.float32)[0]
+
+
+def number_corpus(n=3000, seed=0):
+    """Decimal strings in the parser's number syntax: random digit counts (1-25), points,
+    exponents from -60 to 45, and the hard cases (ties, FLT_MAX, the subnormal edge)."""
+    rng = np.random.default_rng(seed)
+    out = ["0", "-0", "0.1", "1", "16777216", "16777217", "16777218", "16777219", "33554435",
+           "3.4028235e38", "3.4028236e38", "3.40282357e38", "3.4028234e38", "1e39", "1e-45",
+           "7e-46", "7.1e-46", "7.006e-46", "1.401298464324817e-45", "1.1754943e-38",
+           "1.17549435e-38", "1.1754942e-38", "0.000000000000000000000000000000000000000000001",
+           "123456789012345678901234567890", "1.5", "2.5", "0.3", "1e10", "1.e5", ".5", "-.25",
+           "1e-5", "9999999999999999999", "99999999999999999999", "4.9e-324", "1e-400",
+           "1e400", "0e999", "8388608.5", "8388609.5", "0.1e1", "100000000000000000000e-20"]
+    for _ in range(n):
+        nd = int(rng.integers(1, 26))
+        digits = "".join(str(int(d)) for d in rng.integers(0, 10, nd))
+        p = int(rng.integers(0, nd + 1))
+        s = digits[:p] + ("." + digits[p:] if p < nd else "")
+        if s.startswith("."):
+            s = "0" + s if rng.random() < 0.5 else s
+        if rng.random() < 0.6:
+            s += ("e" if rng.random() < 0.5 else "E") + ("-" if rng.random() < 0.5 else "") + \
+                 str(int(rng.integers(0, 46)))
+        if rng.random() < 0.3:
+            s = "-" + s
+        out.append(s)
+    return out
+
+
+def to_svg(path):
+    """A path as absolute SVG path data; '%.9g' gives every f32 back exactly under a
+    correctly rounded parse."""
+    verbs = np.asarray(path.verbs)
+    pts = np.asarray(path.points, np.float32).reshape(-1, 2)
+    f = lambda p: "%.9g %.9g" % (float(p[0]), float(p[1]))
+    out, pi = [], 0
+    for v in verbs:
+        if v == 3:
+            out.append("M " + f(pts[pi]))
+            pi += 1
+        elif v == 0:
+            out.append("L " + f(pts[pi]))
+            pi += 1
+        elif v == 1:
+            out.append("Q " + f(pts[pi]) + " " + f(pts[pi + 1]))
+            pi += 2
+        elif v == 2:
+            out.append("C " + f(pts[pi]) + " " + f(pts[pi + 1]) + " " + f(pts[pi + 2]))
+            pi += 3
+        elif v == 4:
+            out.append("Z")
+            pi += 1
+    return " ".join(out)
+
+
+def random_svg(rng, n_cmds=12, arcs=True, corrupt=0.0):
+    """Random SVG path data over every command (absolute and relative, implicit repeats,
+    mixed separators, exponents, signs glued to the previous number), optionally with
+    stray characters."""
+    def num():
+        r = rng.random()
+        if r < 0.3:
+            s = str(int(rng.integers(-500, 500)))
+        elif r < 0.7:
+            s = "%.*f" % (int(rng.integers(0, 5)), rng.uniform(-500, 500))
+        elif r < 0.85:
+            s = ("%.3e" % rng.uniform(-500, 500)).replace("e+", "e")
+        else:
+            s = ("-" if rng.random() < 0.5 else "") + "." + str(int(rng.integers(0, 1000)))
+        return s
+
+    def sep(nxt):
+        if nxt.startswith("-") and rng.random() < 0.5:
+            return ""
+        return [" ", ",", ", ", "\n", "\t", "  "][int(rng.integers(0, 6))]
+
+    def args(n, flags=()):
+        out = ""
+        for k in range(n):
+            tok = str(int(rng.integers(0, 2))) if k in flags else num()
+            out += sep(tok) + tok
+        return out
+
+    spec = {"M": (2, ()), "L": (2, ()), "H": (1, ()), "V": (1, ()), "Q": (4, ()), "T": (2, ()),
+            "C": (6, ()), "S": (4, ()), "A": (7, (3, 4)), "Z": (0, ())}
+    letters = [c for c in spec if arcs or c != "A"]
+    s = "M" + args(2)
+    for _ in range(n_cmds):
+        c = letters[int(rng.integers(0, len(letters)))]
+        if rng.random() < 0.5:
+            c = c.lower()
+        n, fl = spec[c.upper()]
+        s += sep("x") + c + args(n, fl)
+        if n and rng.random() < 0.3:  # implicit repetition
+            s += args(n, fl)
+        if c in "Zz" and rng.random() < 0.8:
+            s += " M" + args(2)
+    if corrupt and rng.random() < corrupt:
+        k = int(rng.integers(0, len(s)))
+        s = s[:k] + ["x", "*", "+", "#", "e"][int(rng.integers(0, 5))] + s[k:]
+    return s

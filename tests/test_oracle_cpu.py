@@ -401,3 +401,54 @@ def test_winding_matches_crossing_count():
         assert far.sum() > 300
         assert np.array_equal(w[far], ref[far])
         assert np.any(w != 0)
+
+
+def test_parse_f32_matches_exact_rounding():
+    """The oracle's number conversion (grammar check + strtof) against exact rational
+    rounding, bit for bit, including ties, overflow and the subnormal range."""
+    from helpers import f32_exact, number_corpus
+    for s in number_corpus(2000, seed=1):
+        got = O.parse_f32(s)
+        assert got is not None, s
+        want = f32_exact(s)
+        assert np.float32(got).view(np.uint32) == want.view(np.uint32), (s, got, want)
+    for bad in ["", "-", ".", "-.", "e5", "1e", "1e-", "1ee2", "--1", "+1", "1.2.3"]:
+        assert O.parse_f32(bad) is None, bad
+
+
+def test_svg_parser_reference_kats():
+    """parser.rs:498-700: the reference's own parser tests."""
+    assert O.parse_svg("")[2][0] == 0 and len(O.parse_svg(" ")[0]) == 0
+    v, p, e = O.parse_svg("M 0 0 L 1 0 L 1 1 L 0 1 Z")
+    assert v.tolist() == [3, 0, 0, 0, 4] and e[0] == 0
+    assert p.tolist() == [[0, 0], [1, 0], [1, 1], [0, 1], [0, 0]]
+    assert O.parse_svg("M 0 0 1.0 L 1 0 2.0 L 1 1 3.0 L 0 1 4.0 Z", 1)[2][0] == 0
+    v, p, e = O.parse_svg("0 0 0 1 1 1.0 2 2 2.0 3 3 3", 1)
+    assert e[0] == 0 and v.tolist() == [3, 0, 0, 0, 5]
+    assert O.parse_svg("x 0 0 0", 1)[2].tolist() == [3, 0, 0, ord("x")]
+    assert O.parse_svg("\n M 0 \n0 1 x 1 1 1", 1)[2].tolist() == [3, 2, 4, ord("x")]
+    v, p, e = O.parse_svg("M 0.6.5")
+    assert v.tolist() == [3, 5] and p.tolist() == [[np.float32(0.6), 0.5]] and e[0] == 0
+    assert O.parse_svg("M 1e-2 -1E3")[2][0] == 0
+    for bad in ["M 0 --1", "M 0 1ee2", "M 0 1e--1", "M 0 *2", "M 0 e", "M 0 1e", "M 0 +1"]:
+        assert O.parse_svg(bad)[2][0] == 1, bad
+    for s in ["M 0 0 | xxxxxx", "M 0 0| xxxxxx", "| xxxxxx", "    | xxxxxx"]:
+        assert O.parse_svg(s, stop_at="|")[2][0] == 0, s
+    v1, p1, e1 = O.parse_svg("M 0 0 Z L 1 1 2 2 L 3 3 Z M 4 4", stop_at="|")
+    v2, p2, e2 = O.parse_svg("M 0 0 Z", stop_at="|")
+    assert e1[0] == 4 and e2[0] == 0
+    assert v1.tolist() == v2.tolist() and p1.tolist() == p2.tolist()
+
+
+def test_svg_arcs_follow_the_geometry():
+    """An SVG arc's quadratics start at the current point and end near the arc's end point
+    (its angles come from euclid's fast_atan2, an approximation good to ~2e-4 rad, so the
+    end point moves by up to radius * 2e-4), for every flag combination; a zero radius or a
+    zero-length arc is a line (SvgArc::is_straight_line)."""
+    for large in (0, 1):
+        for sweep in (0, 1):
+            v, p, e = O.parse_svg(f"M 10 10 A 30 20 25 {large} {sweep} 50 35")
+            assert e[0] == 0 and v[0] == 3 and v[-1] == 5 and set(v[1:-1]) == {1}
+            assert np.allclose(p[-1], [50, 35], atol=30 * 3e-4)
+    assert O.parse_svg("M 1 1 A 0 5 0 0 1 9 9")[0].tolist() == [3, 0, 5]
+    assert O.parse_svg("M 1 1 A 5 5 0 0 1 1 1")[0].tolist() == [3, 0, 5]

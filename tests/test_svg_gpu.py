@@ -1,0 +1,104 @@
+"""SVG path data ingest on the GPU (lyon_extra's PathParser, parser.rs) against the oracle:
+verbs, errors, numbers and points equal (arcs with correctly rounded sin / cos / tan on both
+sides)."""
+import numpy as np
+import pytest
+
+from helpers import number_corpus, random_svg, to_svg
+from lyon_amd import svg as S
+from lyon_amd.path import PathBatch
+from lyon_amd.workloads import stroke_batch
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _err(e):
+    return [0, 0, 0, 0] if e is None else [e.kind, e.line, e.column, ord(e.char) if e.char else 0]
+
+
+def test_reference_kats():
+    """parser.rs:498-700."""
+    p, e = S.parse_path("M 0 0 L 1 0 L 1 1 L 0 1 Z")
+    assert e is None and p.verbs.tolist() == [3, 0, 0, 0, 4]
+    assert S.parse_path("M 0 0 1.0 L 1 0 2.0 L 1 1 3.0 L 0 1 4.0 Z", S.ParserOptions(1))[1] is None
+    p, e = S.parse_path("0 0 0 1 1 1.0 2 2 2.0 3 3 3", S.ParserOptions(1))
+    assert e is None and p.verbs.tolist() == [3, 0, 0, 0, 5]
+    e = S.parse_path("x 0 0 0", S.ParserOptions(1))[1]
+    assert (e.kind, e.line, e.column, e.char) == (S.ERR_COMMAND, 0, 0, "x")
+    e = S.parse_path("\n M 0 \n0 1 x 1 1 1", S.ParserOptions(1))[1]
+    assert (e.kind, e.line, e.column, e.char) == (S.ERR_COMMAND, 2, 4, "x")
+    p, e = S.parse_path("M 0.6.5")
+    assert e is None and p.points.tolist() == [[np.float32(0.6), 0.5]]
+    assert S.parse_path("M 1e-2 -1E3")[1] is None
+    for bad in ["M 0 --1", "M 0 1ee2", "M 0 1e--1", "M 0 *2", "M 0 e", "M 0 1e", "M 0 +1"]:
+        assert S.parse_path(bad)[1].kind == S.ERR_NUMBER, bad
+    for s in ["M 0 0 | xxxxxx", "M 0 0| xxxxxx", "| xxxxxx", "    | xxxxxx"]:
+        assert S.parse_path(s, S.ParserOptions(stop_at="|"))[1] is None, s
+    p1, e1 = S.parse_path("M 0 0 Z L 1 1 2 2 L 3 3 Z M 4 4", S.ParserOptions(stop_at="|"))
+    p2, e2 = S.parse_path("M 0 0 Z", S.ParserOptions(stop_at="|"))
+    assert e1.kind == S.ERR_MISSING_MOVE_TO and e2 is None
+    assert p1.verbs.tolist() == p2.verbs.tolist() and p1.points.tolist() == p2.points.tolist()
+    for s in ["", " "]:
+        p, e = S.parse_path(s)
+        assert e is None and len(p.verbs) == 0
+
+
+def test_numbers_bit_exact():
+    corpus = number_corpus(4000, seed=5)
+    batch, errs = S.parse_paths([f"M {s} 0" for s in corpus])
+    assert all(e is None for e in errs)
+    xs = batch.points[batch.point_offsets[:-1], 0]
+    want = np.array([O.parse_f32(s) for s in corpus], np.float32)
+    bad = np.nonzero(xs.view(np.uint32) != want.view(np.uint32))[0]
+    assert len(bad) == 0, [(corpus[i], xs[i], want[i]) for i in bad[:5]]
+
+
+@pytest.mark.parametrize("arcs", [False, True])
+def test_random_svg_matches_oracle(arcs):
+    rng = np.random.default_rng(17 if arcs else 4)
+    strings = [random_svg(rng, int(rng.integers(1, 40)), arcs=arcs, corrupt=0.25)
+               for _ in range(1500)]
+    strings += ["", "   ", "M 1 2", "Z", "z 1 1", "M 0 0 Z Z", "m 1 1 2 2 3 3 z 4 4"]
+    batch, errs = S.parse_paths(strings)
+    n_err = 0
+    for i, s in enumerate(strings):
+        rv, rp, re = O.parse_svg(s)
+        p = batch.path(i)
+        assert _err(errs[i]) == re.tolist(), (s, errs[i], re)
+        n_err += re[0] != 0
+        assert p.verbs.tolist() == rv.tolist(), s
+        assert p.points.shape == rp.shape, s
+        if not arcs:
+            assert np.array_equal(p.points.view(np.uint32), rp.view(np.uint32)), s
+        else:
+            # arcs: sin / cos / tan rounded correctly to f32 on both sides (NaN where an
+            # overflowing number made one)
+            assert np.array_equal(p.points, rp, equal_nan=True), s
+    assert 0 < n_err < len(strings)
+
+
+def test_round_trip_of_the_stroke_workload():
+    """Full size: the 10,000-path stroke workload (and a closed variant) written as SVG
+    path data with 9 significant digits parses back to the same verbs and bit-identical
+    points."""
+    for closed in (False, True):
+        src = stroke_batch(10000 if not closed else 2000, 8, 256, 2048.0, seed=0, closed=closed)
+        strings = [to_svg(src.path(i)) for i in range(len(src))]
+        batch, errs = S.parse_paths(strings)
+        assert all(e is None for e in errs)
+        assert np.array_equal(batch.verbs, src.verbs)
+        assert np.array_equal(batch.verb_offsets, src.verb_offsets)
+        assert np.array_equal(batch.point_offsets, src.point_offsets)
+        assert np.array_equal(batch.points.view(np.uint32), src.points.view(np.uint32))
+
+
+def test_non_ascii_and_options():
+    p, e = S.parse_path("M 0 0 L 1 1 L 2 é")
+    assert e.kind == S.ERR_NON_ASCII
+    assert O.parse_svg("M 0 0 L 1 1 L 2 é")[2][0] == S.ERR_NON_ASCII
+    # text after the stop character is never read
+    p, e = S.parse_path("M 0 0 L 1 1 | é", S.ParserOptions(stop_at="|"))
+    assert e is None and p.verbs.tolist() == [3, 0, 5]
+    with pytest.raises(ValueError):
+        S.parse_path("M 0 0", S.ParserOptions(stop_at="é"))
