@@ -22,6 +22,51 @@ VERB_CLOSE = 4
 VERB_END = 5
 
 
+class PathEvent:
+    """lyon_path's PathEvent (crates/path/src/events.rs): ``kind`` is one of "begin",
+    "line", "quadratic", "cubic", "end"; Begin carries ``at``; Line / Quadratic / Cubic
+    carry ``from_``, ``ctrl1`` / ``ctrl2`` where they have them, and ``to``; End carries
+    ``last``, ``first`` and ``close``."""
+
+    __slots__ = ("kind", "at", "from_", "ctrl1", "ctrl2", "to", "last", "first", "close")
+
+    def __init__(self, kind, at=None, from_=None, ctrl1=None, ctrl2=None, to=None, last=None,
+                 first=None, close=False):
+        self.kind, self.at, self.from_, self.ctrl1, self.ctrl2 = kind, at, from_, ctrl1, ctrl2
+        self.to, self.last, self.first, self.close = to, last, first, close
+
+    @staticmethod
+    def begin(at):
+        return PathEvent("begin", at=at)
+
+    @staticmethod
+    def line(from_, to):
+        return PathEvent("line", from_=from_, to=to)
+
+    @staticmethod
+    def quadratic(from_, ctrl, to):
+        return PathEvent("quadratic", from_=from_, ctrl1=ctrl, to=to)
+
+    @staticmethod
+    def cubic(from_, ctrl1, ctrl2, to):
+        return PathEvent("cubic", from_=from_, ctrl1=ctrl1, ctrl2=ctrl2, to=to)
+
+    @staticmethod
+    def end(last, first, close):
+        return PathEvent("end", last=last, first=first, close=bool(close))
+
+    def _key(self):
+        t = lambda p: None if p is None else (float(p[0]), float(p[1]))
+        return (self.kind, t(self.at), t(self.from_), t(self.ctrl1), t(self.ctrl2), t(self.to),
+                t(self.last), t(self.first), self.close)
+
+    def __eq__(self, other):
+        return isinstance(other, PathEvent) and self._key() == other._key()
+
+    def __repr__(self):
+        return "PathEvent" + repr(self._key())
+
+
 class Path:
     """An immutable lyon path: ``verbs`` (uint8) and ``points`` (float32, shape (n, 2))."""
 
@@ -40,6 +85,62 @@ class Path:
 
     def num_segments(self) -> int:
         return int(np.count_nonzero(self.verbs <= VERB_CUBIC))
+
+    def iter(self):
+        """Path::iter (path.rs:817-909): the path's events.  Close reads the stored copy of
+        the first point; End moves the current point back to the first."""
+        pts = [tuple(p) for p in self.points.tolist()]
+        k = 0
+
+        def nxt():
+            nonlocal k
+            p = pts[k] if k < len(pts) else (float("nan"), float("nan"))
+            k += 1
+            return p
+
+        current = first = (0.0, 0.0)
+        for v in self.verbs.tolist():
+            if v == VERB_BEGIN:
+                current = first = nxt()
+                yield PathEvent.begin(current)
+            elif v == VERB_LINE:
+                f, current = current, nxt()
+                yield PathEvent.line(f, current)
+            elif v == VERB_QUADRATIC:
+                f, c = current, nxt()
+                current = nxt()
+                yield PathEvent.quadratic(f, c, current)
+            elif v == VERB_CUBIC:
+                f, c1, c2 = current, nxt(), nxt()
+                current = nxt()
+                yield PathEvent.cubic(f, c1, c2, current)
+            elif v == VERB_CLOSE:
+                nxt()
+                yield PathEvent.end(current, first, True)
+            else:
+                last, current = current, first
+                yield PathEvent.end(last, first, False)
+
+    @staticmethod
+    def from_events(events) -> "Path":
+        """A path built from PathEvents the way the tessellators consume them (only the
+        events' end and control points are read: Begin's ``at``, the segments' ``ctrl`` and
+        ``to``, End's ``close``)."""
+        b = PathBuilder()
+        for e in events:
+            if e.kind == "begin":
+                b.begin(e.at)
+            elif e.kind == "line":
+                b.line_to(e.to)
+            elif e.kind == "quadratic":
+                b.quadratic_bezier_to(e.ctrl1, e.to)
+            elif e.kind == "cubic":
+                b.cubic_bezier_to(e.ctrl1, e.ctrl2, e.to)
+            elif e.kind == "end":
+                b.end(e.close)
+            else:
+                raise ValueError(f"unknown path event {e.kind!r}")
+        return b.build()
 
 
 class PathBuilder:

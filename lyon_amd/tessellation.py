@@ -274,6 +274,10 @@ class StrokeTessellator:
     tessellate = tessellate_path
 
     # stroke.rs:255-307: the shapes are built as paths and stroked
+    def tessellate(self, events, options: StrokeOptions, builder: BuffersBuilder):
+        """StrokeTessellator::tessellate (stroke.rs:122-138) over PathEvents."""
+        self.tessellate_path(Path.from_events(events), options, builder)
+
     def tessellate_polygon(self, points, closed, options: StrokeOptions, builder: BuffersBuilder):
         self.tessellate_path(_shape_path(lambda b: b.add_polygon(points, closed)), options, builder)
 
@@ -300,6 +304,10 @@ class FillTessellator:
         _run(B.lib().lyg_fill_path, path, options.to_c(), builder)
 
     tessellate = tessellate_path
+
+    def tessellate(self, events, options: FillOptions, builder: BuffersBuilder):
+        """FillTessellator::tessellate (fill.rs:578-598) over PathEvents."""
+        self.tessellate_path(Path.from_events(events), options, builder)
 
     def tessellate_polygon(self, points, closed, options: FillOptions, builder: BuffersBuilder):
         """fill.rs:644-651: the polygon's path events."""

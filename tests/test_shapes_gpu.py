@@ -94,3 +94,18 @@ def test_fill_polygon_and_ellipse_coverage():
     ep = b.build()
     w = O.winding_compare(ep, 0.05, 0, v, i, n=1024)
     assert w["mismatch"] == 0
+
+
+def test_tessellate_over_events_matches_tessellate_path():
+    """FillTessellator / StrokeTessellator::tessellate (fill.rs:578-598, stroke.rs:122-138)
+    over a path's events give what tessellate_path gives."""
+    from lyon_amd.workloads import stroke_batch
+    sb = stroke_batch(6, 4, 40, 512.0, seed=3, closed=True)
+    for i in range(len(sb)):
+        p = sb.path(i)
+        for tess, opts in ((FillTessellator(), FillOptions(tolerance=0.05)),
+                           (StrokeTessellator(), StrokeOptions(line_width=3.0, tolerance=0.05))):
+            a, b = VertexBuffers(), VertexBuffers()
+            tess.tessellate_path(p, opts, BuffersBuilder(a))
+            tess.tessellate(p.iter(), opts, BuffersBuilder(b))
+            assert np.array_equal(a.vertices, b.vertices) and np.array_equal(a.indices, b.indices)
