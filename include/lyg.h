@@ -369,7 +369,8 @@ int32_t lyg_server_get_stats(lyg_server* server, lyg_server_stats* out);
  * / 25% quadratic / 50% cubic, points in [0, 4096]^2, the batched stroke workload's
  * options) or else a fill (a path of the batched fill workload, EvenOdd/NonZero 50/50);
  * tolerance for both.  Requests draw from n_distinct pre-generated paths of each kind;
- * every result is released when done.  Latency percentiles cover the server's lifetime. */
+ * every result is released when done.  Latency percentiles and the mean batch cover this
+ * stream's requests only. */
 typedef struct lyg_stream_stats {
     uint64_t requests, triangles, batches, pool_high_water;
     double offered_rate, submit_rate, completed_rate;  /* requests/s */

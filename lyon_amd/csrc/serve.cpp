@@ -646,6 +646,12 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
     std::atomic<int32_t> err{LYG_OK};
     lyg_server_stats before{};
     lyg_server_get_stats(s, &before);
+    size_t lat0, batched0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        lat0 = s->latencies_us.size();
+        batched0 = s->batched_requests;
+    }
     const Clock::time_point t0 = Clock::now();
     std::thread reaper([&] {
         for (uint32_t i = 0; i < n_requests; ++i) {
@@ -684,10 +690,24 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
     out->completed_rate = n_requests / total_s;
     out->triangles = after.triangles - before.triangles;
     out->triangles_per_s = (double)out->triangles / total_s;
-    out->p50_us = after.p50_us;
-    out->p99_us = after.p99_us;
-    out->max_us = after.max_us;
-    out->mean_batch = after.mean_batch;
+    // latencies and batch sizes of this stream's requests only (a server that ran a warm-up
+    // stream before reports its steady state here)
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        std::vector<double> l(s->latencies_us.begin() + (std::ptrdiff_t)lat0, s->latencies_us.end());
+        if (!l.empty()) {
+            auto pct = [&](double q) {
+                size_t k = (size_t)std::min<double>((double)l.size() - 1, std::floor(q * (double)(l.size() - 1) + 0.5));
+                std::nth_element(l.begin(), l.begin() + k, l.end());
+                return l[k];
+            };
+            out->p50_us = pct(0.50);
+            out->p99_us = pct(0.99);
+            out->max_us = *std::max_element(l.begin(), l.end());
+        }
+        const uint64_t nb = after.batches - before.batches;
+        out->mean_batch = nb ? (double)(s->batched_requests - batched0) / (double)nb : 0.0;
+    }
     out->batches = after.batches - before.batches;
     out->seconds = total_s;
     out->pool_high_water = after.pool_high_water;
