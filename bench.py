@@ -354,13 +354,13 @@ def main():
             with Server(pool_bytes=8 << 30, max_batch_paths=1024, max_inflight=4) as srv:
                 # warm-up stream (first launches on the workers' streams, their buffers
                 # growing to size), then the measured one on the same server
-                srv.run_stream(n_requests=max(nreq // 10, 100), rate_per_s=rate,
+                srv.run_stream(n_requests=max(nreq // 4, 100), rate_per_s=rate,
                                stroke_frac=sfrac, min_segs=smin, max_segs=smax,
                                tolerance=0.05, n_distinct=256, seed=2)
                 r = srv.run_stream(n_requests=nreq, rate_per_s=rate, stroke_frac=sfrac,
                                    min_segs=smin, max_segs=smax, tolerance=0.05,
                                    n_distinct=256, seed=1)
-            serving.append({"stream": name, "requests": nreq, "warmup_requests": max(nreq // 10, 100),
+            serving.append({"stream": name, "requests": nreq, "warmup_requests": max(nreq // 4, 100),
                             "offered_rate": rate, "completed_rate": r["completed_rate"],
                             "p50_us": r["p50_us"], "p99_us": r["p99_us"], "max_us": r["max_us"],
                             "triangles_per_s": r["triangles_per_s"],

@@ -335,6 +335,8 @@ typedef struct lyg_server_config {
     uint32_t max_inflight;     /* batches on the device at once */
     uint32_t linger_us;        /* wait this long for a short queue to grow (0: none) */
     int32_t device;
+    uint32_t stroke_sizing;    /* 0: one staged emit pass (as lyg_stroke_batch), 1: count pass
+                                  then emit (lyg_ctx_set_stroke_staging(ctx, 0, ..)) */
 } lyg_server_config;
 typedef struct lyg_server_result {
     int32_t status;            /* the request's lyg_status */

@@ -56,7 +56,8 @@ class BatchOutC(C.Structure):
 
 class ServerConfigC(C.Structure):
     _fields_ = [("pool_bytes", C.c_uint64), ("max_batch_paths", C.c_uint32),
-                ("max_inflight", C.c_uint32), ("linger_us", C.c_uint32), ("device", C.c_int32)]
+                ("max_inflight", C.c_uint32), ("linger_us", C.c_uint32), ("device", C.c_int32),
+                ("stroke_sizing", C.c_uint32)]
 
 
 class ServerResultC(C.Structure):

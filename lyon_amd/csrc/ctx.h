@@ -78,10 +78,15 @@ inline bool check(lyg_ctx* c, hipError_t e, const char* what) {
 inline bool ensure(lyg_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return true;
+    const size_t old_cap = b.cap;
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+    // Buffers below 256 MiB at least double when they grow: a growth frees the old buffer,
+    // and hipFree waits for the whole device, stalling every stream of the process (the
+    // serving workers' batches grow their buffers in many small steps otherwise).
     size_t cap = std::max(bytes, (size_t)4096);
+    if (bytes < ((size_t)256 << 20)) cap = std::max(cap, 2 * old_cap);
     cap = (cap + 4095) & ~(size_t)4095;
     if (!check(c, hipMalloc(&b.p, cap), "hipMalloc")) return false;
     b.cap = cap;

@@ -443,7 +443,8 @@ void fill_result(lyg_server* s, const Request& r, lyg_server_result* out) {
 extern "C" {
 
 lyg_server* lyg_server_create(const lyg_server_config* cfg) {
-    if (!cfg || cfg->pool_bytes == 0 || cfg->max_inflight == 0 || cfg->max_batch_paths == 0)
+    if (!cfg || cfg->pool_bytes == 0 || cfg->max_inflight == 0 || cfg->max_batch_paths == 0 ||
+        cfg->stroke_sizing > 1)
         return nullptr;
     auto s = std::make_unique<lyg_server>();
     s->cfg = *cfg;
@@ -451,6 +452,7 @@ lyg_server* lyg_server_create(const lyg_server_config* cfg) {
     for (uint32_t i = 0; i < cfg->max_inflight; ++i) {
         auto w = std::make_unique<lyg_server::Worker>();
         w->ctx = lyg_ctx_create(cfg->device);
+        if (w->ctx && cfg->stroke_sizing == 1) lyg_ctx_set_stroke_staging(w->ctx, 0, 1.0f);
         if (!w->ctx) {
             for (auto& x : s->workers)
                 if (x->ctx) lyg_ctx_destroy(x->ctx);

@@ -17,8 +17,9 @@ LYG_PENDING = 21
 
 class Server:
     def __init__(self, pool_bytes=1 << 30, max_batch_paths=1024, max_inflight=4, linger_us=0,
-                 device=0):
-        cfg = B.ServerConfigC(pool_bytes, max_batch_paths, max_inflight, linger_us, device)
+                 device=0, stroke_sizing=0):
+        cfg = B.ServerConfigC(pool_bytes, max_batch_paths, max_inflight, linger_us, device,
+                              stroke_sizing)
         self._h = B.lib().lyg_server_create(C.byref(cfg))
         if not self._h:
             raise RuntimeError("lyg_server_create failed (no usable HIP device or pool)")
