@@ -282,13 +282,14 @@ def main():
         reps = 5
         t0 = time.perf_counter()
         for _ in range(reps):
-            r = S.parse_paths_device(td, od, args.paths)
+            r = S.parse_paths_device(td, od, args.paths, out=r)
         ing_ms = (time.perf_counter() - t0) * 1e3 / reps
         ok = (r.n_verbs == len(batch.verbs) and r.n_points == len(batch.points) and
               r.n_errors == 0)
         ingest = {"workload": f"the {args.paths} stroke-workload paths as absolute SVG path "
-                              "data (9 significant digits), parsed on the device (count pass, "
-                              "offset scans, emit pass), text and paths in device memory",
+                              "data (9 significant digits), parsed on the device (one parse "
+                              "pass into per-string staging, offset scans, compaction) into the "
+                              "previous call's buffers, text and paths in device memory",
                   "text_bytes": int(len(text)), "ms": ing_ms,
                   "paths_per_s": args.paths / (ing_ms / 1e3),
                   "text_gb_per_s": len(text) / (ing_ms / 1e3) / 1e9, "round_trip_counts_ok": ok}

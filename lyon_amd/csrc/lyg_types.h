@@ -135,16 +135,15 @@ struct SvgLaunch {
     uint32_t n;
     int num_attributes;
     int stop_at;               // ASCII character or -1
-    // count pass
     uint32_t* counts_v;
     uint32_t* counts_p;
     int32_t* errors;           // 4 per string or null
-    uint32_t* n_errors;
-    // emit pass
+    uint32_t* n_errors;        // [0] strings with an error, [1] staging rooms overflowed
+    uint8_t* verbs;            // staging rooms (see svg_parse_kernel)
+    float* points;
+    // compaction
     const uint32_t* off_v;
     const uint32_t* off_p;
-    uint8_t* verbs;
-    float* points;
 };
 
 }  // namespace lyg

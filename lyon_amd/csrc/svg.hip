@@ -1,8 +1,8 @@
 // SVG path data on the device: lyon_extra's PathParser (crates/extra/src/parser.rs:140-462)
 // writing into lyon_path's Path layout (BuilderImpl, crates/path/src/path.rs:516-611), one
 // lane per path string.  The parse is a serial state machine over the characters, so a
-// lane walks one string; a count pass sizes every path, a scan places them end to end, and
-// an emit pass runs the same parser again writing verbs and points.
+// lane walks one string, writing into a staging room sized by the string's length; scans
+// of the counts place the paths end to end and a copy kernel moves them there.
 //
 // Numbers: the parser collects `-? digits* (. digits*)? ([eE] -? digits*)?` and converts it
 // with Rust's `str::parse::<f32>`, which is correctly rounded (round half to even) for
@@ -308,12 +308,13 @@ struct Out {
     float2* points;
     uint32_t nv, np;
     vec2 first;
+    uint32_t cap_v, cap_p;  // staging room of the string (writes beyond it are dropped)
     __device__ void pt(vec2 p) {
-        if (EMIT) points[np] = make_float2(p.x, p.y);
+        if (EMIT && np < cap_p) points[np] = make_float2(p.x, p.y);
         ++np;
     }
     __device__ void verb(uint32_t v) {
-        if (EMIT) verbs[nv] = (uint8_t)v;
+        if (EMIT && nv < cap_v) verbs[nv] = (uint8_t)v;
         ++nv;
     }
     __device__ void begin(vec2 p) {
@@ -583,28 +584,45 @@ done:
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) svg_parse_kernel(SvgLaunch L, bool emit) {
+// Parses string i into its staging room: verbs at text_off[i] + 2 i (room: its bytes + 2),
+// points at 2 text_off[i] + 2 i (room: twice its bytes + 2).  Every command consumes at
+// least one character and makes at most one verb and two points per character (an arc's
+// eight quadratics need eleven), plus the End of the last sub-path; a string whose output
+// would not fit anyway raises the overflow counter instead of writing past its room.
+__global__ void __launch_bounds__(256) svg_parse_kernel(SvgLaunch L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
     const uint32_t a = L.text_off[i], b = L.text_off[i + 1];
     PErr E;
-    if (!emit) {
-        Out<false> O{nullptr, nullptr, 0, 0, mk(0.0f, 0.0f)};
-        parse_svg<false>(L.text, a, b, L.num_attributes, L.stop_at, O, E);
-        L.counts_v[i] = O.nv;
-        L.counts_p[i] = O.np;
-        if (L.errors) {
-            int32_t* e = L.errors + 4ull * i;
-            e[0] = E.kind;
-            e[1] = E.line;
-            e[2] = E.col;
-            e[3] = (int32_t)E.ch;
-        }
-        if (E.kind != LYG_SVG_OK) atomicAdd(L.n_errors, 1u);
-    } else {
-        Out<true> O{L.verbs + L.off_v[i], reinterpret_cast<float2*>(L.points) + L.off_p[i], 0, 0,
-                    mk(0.0f, 0.0f)};
-        parse_svg<true>(L.text, a, b, L.num_attributes, L.stop_at, O, E);
+    Out<true> O{L.verbs + a + 2ull * i, reinterpret_cast<float2*>(L.points) + 2ull * a + 2ull * i,
+                0, 0, mk(0.0f, 0.0f), b - a + 2, 2 * (b - a) + 2};
+    parse_svg<true>(L.text, a, b, L.num_attributes, L.stop_at, O, E);
+    L.counts_v[i] = O.nv;
+    L.counts_p[i] = O.np;
+    if (O.nv > O.cap_v || O.np > O.cap_p) atomicAdd(L.n_errors + 1, 1u);
+    if (L.errors) {
+        int32_t* e = L.errors + 4ull * i;
+        e[0] = E.kind;
+        e[1] = E.line;
+        e[2] = E.col;
+        e[3] = (int32_t)E.ch;
+    }
+    if (E.kind != LYG_SVG_OK) atomicAdd(L.n_errors, 1u);
+}
+
+// Copies every string's verbs and points from its staging room to its place in the batch:
+// one block per string.
+__global__ void __launch_bounds__(256) svg_compact_kernel(SvgLaunch L, uint8_t* verbs,
+                                                          float* points) {
+    for (uint32_t i = blockIdx.x; i < L.n; i += gridDim.x) {
+        const uint32_t a = L.text_off[i];
+        const uint8_t* sv = L.verbs + a + 2ull * i;
+        const float2* sp = reinterpret_cast<const float2*>(L.points) + 2ull * a + 2ull * i;
+        const uint32_t ov = L.off_v[i], nv = L.off_v[i + 1] - ov;
+        const uint32_t op = L.off_p[i], np = L.off_p[i + 1] - op;
+        for (uint32_t k = threadIdx.x; k < nv; k += blockDim.x) verbs[ov + k] = sv[k];
+        float2* dp = reinterpret_cast<float2*>(points) + op;
+        for (uint32_t k = threadIdx.x; k < np; k += blockDim.x) dp[k] = sp[k];
     }
 }
 
@@ -612,9 +630,17 @@ __global__ void __launch_bounds__(256) svg_parse_kernel(SvgLaunch L, bool emit) 
 
 extern "C" {
 
-hipError_t lyg_launch_svg_parse(const lyg::SvgLaunch* L, bool emit, hipStream_t s) {
+hipError_t lyg_launch_svg_parse(const lyg::SvgLaunch* L, hipStream_t s) {
     if (L->n == 0) return hipSuccess;
-    hipLaunchKernelGGL(lyg::svg_parse_kernel, dim3((L->n + 255) / 256), dim3(256), 0, s, *L, emit);
+    hipLaunchKernelGGL(lyg::svg_parse_kernel, dim3((L->n + 255) / 256), dim3(256), 0, s, *L);
+    return hipGetLastError();
+}
+
+hipError_t lyg_launch_svg_compact(const lyg::SvgLaunch* L, uint8_t* verbs, float* points,
+                                  hipStream_t s) {
+    if (L->n == 0) return hipSuccess;
+    const uint32_t blocks = L->n < 65536u ? L->n : 65536u;
+    hipLaunchKernelGGL(lyg::svg_compact_kernel, dim3(blocks), dim3(256), 0, s, *L, verbs, points);
     return hipGetLastError();
 }
 

@@ -4,7 +4,9 @@
 #include "lyg_types.h"
 
 extern "C" {
-hipError_t lyg_launch_svg_parse(const lyg::SvgLaunch* L, bool emit, hipStream_t s);
+hipError_t lyg_launch_svg_parse(const lyg::SvgLaunch* L, hipStream_t s);
+hipError_t lyg_launch_svg_compact(const lyg::SvgLaunch* L, uint8_t* verbs, float* points,
+                                  hipStream_t s);
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 }
 
@@ -31,7 +33,16 @@ int32_t lyg_parse_svg_paths(lyg_ctx* c, const uint8_t* text, const uint32_t* tex
                    ? LYG_OK
                    : LYG_ERR_DEVICE;
     }
-    if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->counts_i, 4ull * n))
+    // the staging rooms follow the text: string i's verbs at text_offsets[i] + 2 i (its
+    // length + 2), its points at 2 text_offsets[i] + 2 i (twice its length + 2)
+    uint32_t* h32 = reinterpret_cast<uint32_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h32, text_offsets + n, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t text_bytes = h32[0];
+    if (!ensure(c, c->counts_v, 4ull * n) || !ensure(c, c->counts_i, 4ull * n) ||
+        !ensure(c, c->st_i, text_bytes + 2ull * n) ||
+        !ensure(c, c->st_v, 8ull * (2 * text_bytes + 2ull * n)))
         return LYG_ERR_DEVICE;
     lyg::SvgLaunch L{};
     L.text = text;
@@ -42,22 +53,29 @@ int32_t lyg_parse_svg_paths(lyg_ctx* c, const uint8_t* text, const uint32_t* tex
     L.counts_v = as<uint32_t>(c->counts_v);
     L.counts_p = as<uint32_t>(c->counts_i);
     L.errors = errors;
-    L.n_errors = c->d_counters + 5;
+    L.n_errors = c->d_counters + 5;  // [5] paths with an error, [6] rooms overflowed
+    L.verbs = as<uint8_t>(c->st_i);
+    L.points = as<float>(c->st_v);
     if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
-        !check(c, lyg_launch_svg_parse(&L, false, s), "svg count") ||
+        !check(c, lyg_launch_svg_parse(&L, s), "svg parse") ||
         !check(c, ctx_scan(c, L.counts_v, n, verb_offsets, c->d_totals, c->d_counters + 4, s), "scan") ||
         !check(c, ctx_scan(c, L.counts_p, n, point_offsets, c->d_totals + 1, c->d_counters + 4, s), "scan"))
         return LYG_ERR_DEVICE;
     uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
     if (!check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "D2H") ||
-        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 8, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 12, hipMemcpyDeviceToHost, s), "D2H") ||
         !check(c, hipStreamSynchronize(s), "sync"))
         return LYG_ERR_DEVICE;
     const uint64_t tv = h[0], tp = h[1];
-    const uint32_t overflow = (uint32_t)h[2], nerr = (uint32_t)(h[2] >> 32);
+    const uint32_t* hc = reinterpret_cast<const uint32_t*>(h + 2);
+    const uint32_t overflow = hc[0], nerr = hc[1], room_overflow = hc[2];
     if (n_verbs) *n_verbs = tv;
     if (n_points) *n_points = tp;
     if (n_errors) *n_errors = nerr;
+    if (room_overflow) {
+        c->last_error = "svg parse: a path outgrew its staging room";
+        return LYG_ERR_INTERNAL;
+    }
     if (overflow) {
         c->last_error = "more than 2^32 verbs or points";
         return LYG_ERR_TOO_MANY_VERTICES;
@@ -66,9 +84,7 @@ int32_t lyg_parse_svg_paths(lyg_ctx* c, const uint8_t* text, const uint32_t* tex
         return LYG_ERR_CAPACITY;
     L.off_v = verb_offsets;
     L.off_p = point_offsets;
-    L.verbs = verbs;
-    L.points = points;
-    if (!check(c, lyg_launch_svg_parse(&L, true, s), "svg emit") ||
+    if (!check(c, lyg_launch_svg_compact(&L, verbs, points, s), "svg compact") ||
         !check(c, hipStreamSynchronize(s), "sync"))
         return LYG_ERR_DEVICE;
     return LYG_OK;
@@ -96,21 +112,20 @@ int32_t lyg_parse_svg_path(lyg_ctx* c, const char* text, uint32_t len, uint32_t 
     uint32_t* offs = as<uint32_t>(c->in_offs);
     uint64_t tv = 0, tp = 0;
     uint32_t nerr = 0;
+    // one parse into buffers of the staging bound (every output of `len` bytes fits)
+    const uint64_t cap_v = (uint64_t)len + 2, cap_p = 2ull * len + 2;
+    if (!ensure(c, c->out_v, 8 * cap_p) || !ensure(c, c->out_i, cap_v)) return LYG_ERR_DEVICE;
     int32_t st = lyg_parse_svg_paths(c, as<uint8_t>(c->in_verbs), offs, 1, num_attributes,
-                                     stop_at, nullptr, 0, nullptr, 0, offs + 2, offs + 4,
-                                     as<int32_t>(c->misc), &tv, &tp, &nerr);
-    if (st != LYG_OK && st != LYG_ERR_CAPACITY) return st;
+                                     stop_at, as<uint8_t>(c->out_i), cap_v, as<float>(c->out_v),
+                                     cap_p, offs + 2, offs + 4, as<int32_t>(c->misc), &tv, &tp,
+                                     &nerr);
+    if (st != LYG_OK) return st;
     *n_verbs = (uint32_t)tv;
     *n_points = (uint32_t)tp;
     if (error && !check(c, hipMemcpy(error, c->misc.p, 16, hipMemcpyDeviceToHost), "D2H"))
         return LYG_ERR_DEVICE;
     if (tv > verb_capacity || tp > point_capacity || (tv && !verbs) || (tp && !points))
         return LYG_ERR_CAPACITY;
-    if (!ensure(c, c->out_v, 8 * tp) || !ensure(c, c->out_i, tv)) return LYG_ERR_DEVICE;
-    st = lyg_parse_svg_paths(c, as<uint8_t>(c->in_verbs), offs, 1, num_attributes, stop_at,
-                             as<uint8_t>(c->out_i), tv, as<float>(c->out_v), tp, offs + 2,
-                             offs + 4, nullptr, &tv, &tp, &nerr);
-    if (st != LYG_OK) return st;
     if ((tv && !check(c, hipMemcpyAsync(verbs, c->out_i.p, tv, hipMemcpyDeviceToHost, s), "D2H")) ||
         (tp && !check(c, hipMemcpyAsync(points, c->out_v.p, 8 * tp, hipMemcpyDeviceToHost, s), "D2H")) ||
         !check(c, hipStreamSynchronize(s), "sync"))

@@ -98,18 +98,31 @@ class DeviceSvgBatch:
                           self.point_offsets.ptr, self.n)
 
 
-def parse_paths_device(text_dev, offsets_dev, n: int, options: ParserOptions = ParserOptions()):
+def parse_paths_device(text_dev, offsets_dev, n: int, options: ParserOptions = ParserOptions(),
+                       out: "DeviceSvgBatch | None" = None):
     """Parses n strings held in device memory (text bytes and n+1 u32 offsets) into a
-    DeviceSvgBatch."""
+    DeviceSvgBatch; `out`, a previous result, is refilled when its buffers are large
+    enough (one parse call then), else new buffers are sized by a first call."""
     from .batch import DeviceBuffer
     L = B.lib()
     ctx = _Ctx.get()
+    tv, tp, ne = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
+    r = out if out is not None and out.n == n else None
+    if r is not None:
+        st = L.lyg_parse_svg_paths(ctx, text_dev.ptr, offsets_dev.ptr, n, options.num_attributes,
+                                   options._stop(), r.verbs.ptr, r.verbs.nbytes, r.points.ptr,
+                                   r.points.nbytes // 8, r.verb_offsets.ptr, r.point_offsets.ptr,
+                                   r.errors.ptr, C.byref(tv), C.byref(tp), C.byref(ne))
+        if st == 0:
+            r.n_verbs, r.n_points, r.n_errors = tv.value, tp.value, ne.value
+            return r
+        if st != 16:
+            _raise(st)
     r = DeviceSvgBatch()
     r.n = n
     r.verb_offsets = DeviceBuffer(4 * (n + 1))
     r.point_offsets = DeviceBuffer(4 * (n + 1))
     r.errors = DeviceBuffer(16 * max(n, 1))
-    tv, tp, ne = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
     st = L.lyg_parse_svg_paths(ctx, text_dev.ptr, offsets_dev.ptr, n, options.num_attributes,
                                options._stop(), None, 0, None, 0, r.verb_offsets.ptr,
                                r.point_offsets.ptr, r.errors.ptr, C.byref(tv), C.byref(tp),
