@@ -8,8 +8,9 @@
 //     walks the flattened points of its own segment and counts the crossings left of the
 //     query; the count is an integer, so the wave sum is exact in any order.
 //   * approximate signed areas (approximate_signed_area, crates/algorithms/src/area.rs:7-54):
-//     one lane per path.  The area is lyon's serial f32 sum of cross products in event order,
-//     so one lane walks one path in that order.
+//     lyon's serial f32 sum of cross products in event order; one wave per path computes the
+//     terms 64 at a time and adds them in order (signed_area_kernel), or one lane walks each
+//     path (signed_area_serial_kernel).
 //
 // Paths are as lyon's builders make them: every sub-path opens with Begin, so the current
 // point of a verb is the last point consumed before it (path.rs:843-909).  Curves are
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(WAVE* VAL_WAVES) winding_kernel(ValidateLaunch
     }
 }
 
-__global__ void __launch_bounds__(256) signed_area_kernel(ValidateLaunch L) {
+__global__ void __launch_bounds__(256) signed_area_serial_kernel(ValidateLaunch L) {
     const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= L.n_paths) return;
     const uint32_t ve = L.verb_off[pid + 1];
@@ -203,6 +204,195 @@ __global__ void __launch_bounds__(256) signed_area_kernel(ValidateLaunch L) {
     L.areas[pid] = area;
 }
 
+// approximate_signed_area with one wave per path.  lyon's result is a serial f32 sum in
+// event order, so the additions stay serial, but everything before them is spread over the
+// lanes: the verbs are read 64 at a time; between two boundary verbs (Begin, End, Close) the
+// flattened points of a run of segments are computed 64 at a time, one per lane (t resumed
+// from the lane's previous point of the same segment, as in flatten_emit_kernel, so it is
+// lyon's serial sum), each lane forms its cross product with the point before it, and the
+// 64 terms are added to the running sum in order through readlane.  Begin / End / Close
+// are applied in order between the runs, with area.rs's rules for ill-formed paths.
+constexpr int AREA_WAVES = 4;
+
+__global__ void __launch_bounds__(WAVE* AREA_WAVES) signed_area_kernel(ValidateLaunch L) {
+    __shared__ uint32_t pre_all[AREA_WAVES][WAVE + 1];  // term prefix of the window's verbs
+    __shared__ uint32_t pi_all[AREA_WAVES][WAVE];       // first point of each verb
+    __shared__ uint32_t vb_all[AREA_WAVES][WAVE];       // verb codes
+    const int l = lane_id();
+    const uint32_t w = threadIdx.x / WAVE;
+    uint32_t* pre = pre_all[w];
+    uint32_t* pis = pi_all[w];
+    uint32_t* vbs = vb_all[w];
+    const uint32_t stride = gridDim.x * AREA_WAVES;
+    for (uint32_t pid = blockIdx.x * AREA_WAVES + w; pid < L.n_paths; pid += stride) {
+        const uint32_t vb = L.verb_off[pid], nv = L.verb_off[pid + 1] - vb;
+        const uint32_t pb = L.point_off[pid];
+        const PathPts pts{L.points + 2ull * pb, L.point_off[pid + 1] - pb};
+        float area = 0.0f, da = 0.0f;
+        vec2 first = mk(0.0f, 0.0f), cur = mk(0.0f, 0.0f), v0c = mk(0.0f, 0.0f);
+        bool in_sub = false, stop = false;
+        uint32_t pc = 0;
+        for (uint32_t c = 0; c < nv && !stop; c += WAVE) {
+            const uint32_t i = c + (uint32_t)l;
+            const uint32_t verb = i < nv ? (uint32_t)L.verbs[vb + i] : 0xffu;
+            const uint32_t cons = points_of(verb);
+            const uint32_t incl = incl_scan(cons);
+            const uint32_t pi = pc + incl - cons;
+            // flattened points of each segment verb (Line 1, curves Wang's count)
+            uint32_t cnt = 0;
+            if (verb == 0u) {
+                cnt = 1;
+            } else if (verb == 1u) {
+                cnt = seg_count(quad_wang(pts.at(pi - 1u), pts.at(pi), pts.at(pi + 1u), L.tolerance));
+            } else if (verb == 2u) {
+                cnt = seg_count(cubic_wang(pts.at(pi - 1u), pts.at(pi), pts.at(pi + 1u),
+                                           pts.at(pi + 2u), L.tolerance));
+            }
+            const uint32_t tincl = incl_scan(cnt);
+            __builtin_amdgcn_wave_barrier();
+            pre[l + 1] = tincl;
+            if (l == 0) pre[0] = 0;
+            pis[l] = pi;
+            vbs[l] = verb;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t bmask = ballot(verb >= 3u);  // boundaries and lanes past the path
+            int j = 0;
+            while (j < WAVE && !stop) {
+                const uint32_t vj = vbs[j];
+                if (vj >= 3u) {
+                    if (vj == 0xffu) break;
+                    if (vj == 3u) {
+                        // Begin: a sub-path starts (one inside a sub-path ends the sum)
+                        if (in_sub) {
+                            stop = true;
+                            break;
+                        }
+                        first = pts.at(pis[j]);
+                        cur = first;
+                        da = 0.0f;
+                        v0c = mk(0.0f, 0.0f);
+                        in_sub = true;
+                    } else {
+                        // End / Close: the closing term, then the sub-path's area
+                        if (!in_sub) {
+                            stop = true;
+                            break;
+                        }
+                        da = da + cross(v0c, sub(cur, first));
+                        area = area + da * 0.5f;
+                        in_sub = false;
+                    }
+                    ++j;
+                    continue;
+                }
+                // a run of segment verbs j .. j2 - 1
+                const uint64_t rest = bmask & ~mask_lt(j);
+                const int j2 = rest ? __builtin_ctzll(rest) : WAVE;
+                if (!in_sub) {
+                    stop = true;
+                    break;
+                }
+                const uint32_t r0 = pre[j], r1 = pre[j2];
+                uint32_t seg = 0xffffffffu, k_prev = 0;
+                float t_prev = 0.0f;
+                int kind = 0;
+                vec2 a0 = mk(0.0f, 0.0f), a1 = a0, a2 = a0, a3 = a0, to = a0;
+                float step = 1.0f;
+                uint32_t n_pts = 1;
+                for (uint32_t q0 = r0; q0 < r1; q0 += WAVE) {
+                    const uint32_t g = q0 + (uint32_t)l;
+                    const bool valid = g < r1;
+                    vec2 p = cur;
+                    if (valid) {
+                        int lo = j, hi = j2 - 1;  // last verb of the run with pre <= g
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (pre[mid] <= g) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        const uint32_t my = (uint32_t)lo;
+                        if (my != seg) {
+                            seg = my;
+                            const uint32_t sp = pis[my];
+                            kind = (int)vbs[my];
+                            k_prev = 0;
+                            t_prev = 0.0f;
+                            if (kind == 0) {
+                                to = pts.at(sp);
+                                n_pts = 1;
+                            } else {
+                                const vec2 f = pts.at(sp - 1u), c1 = pts.at(sp), c2 = pts.at(sp + 1u);
+                                float nf;
+                                if (kind == 1) {
+                                    nf = quad_wang(f, c1, c2, L.tolerance);
+                                    a1 = mul(sub(c1, f), 2.0f);
+                                    a2 = sub(add(f, c2), mul(c1, 2.0f));
+                                    to = c2;
+                                } else {
+                                    const vec2 c3 = pts.at(sp + 2u);
+                                    nf = cubic_wang(f, c1, c2, c3, L.tolerance);
+                                    a1 = mul(sub(c1, f), 3.0f);
+                                    a2 = add(sub(mul(f, 3.0f), mul(c1, 6.0f)), mul(c2, 3.0f));
+                                    a3 = add(sub(c3, f), mul(sub(c1, c2), 3.0f));
+                                    to = c3;
+                                }
+                                a0 = f;
+                                step = 1.0f / nf;
+                                n_pts = seg_count(nf);
+                            }
+                        }
+                        const uint32_t k = g - pre[my];  // point k + 1 of the segment
+                        if (k + 1 >= n_pts) {
+                            p = to;
+                        } else {
+                            float t = t_prev;
+                            const uint32_t d = k + 1 - k_prev;
+                            if (d == WAVE) {
+#pragma unroll
+                                for (int u = 0; u < WAVE; ++u) t = t + step;
+                            } else if (d < WAVE) {
+                                for (uint32_t u = 0; u < d; ++u) t = t + step;
+                            } else {
+                                t = tchain(step, k + 1);
+                            }
+                            k_prev = k + 1;
+                            t_prev = t;
+                            vec2 v = a0;
+                            float tt = t;
+                            v = add(v, mul(a1, tt));
+                            tt = tt * t;
+                            v = add(v, mul(a2, tt));
+                            if (kind == 2) {
+                                tt = tt * t;
+                                v = add(v, mul(a3, tt));
+                            }
+                            p = v;
+                        }
+                    }
+                    const vec2 v1 = sub(p, first);
+                    vec2 v0 = shfl_v(v1, l == 0 ? 0 : l - 1);
+                    if (l == 0) v0 = v0c;
+                    const float term = cross(v0, v1);
+                    const uint32_t m = r1 - q0 < (uint32_t)WAVE ? r1 - q0 : (uint32_t)WAVE;
+                    if (m == (uint32_t)WAVE) {
+#pragma unroll
+                        for (int u = 0; u < WAVE; ++u) da = da + readlane_f(term, u);
+                    } else {
+                        for (uint32_t u = 0; u < m; ++u) da = da + readlane_f(term, (int)u);
+                    }
+                    v0c = readlane_v(v1, (int)m - 1);
+                    cur = readlane_v(p, (int)m - 1);
+                }
+                j = j2;
+            }
+            pc += readlane_u(incl, WAVE - 1);
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (l == 0) L.areas[pid] = area;
+    }
+}
+
 }  // namespace lyg
 
 extern "C" {
@@ -217,8 +407,18 @@ hipError_t lyg_launch_winding(const lyg::ValidateLaunch* L, hipStream_t s) {
 
 hipError_t lyg_launch_signed_area(const lyg::ValidateLaunch* L, hipStream_t s) {
     if (L->n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(lyg::signed_area_kernel, dim3((L->n_paths + 255) / 256), dim3(256), 0, s,
-                       *L);
+    uint32_t blocks = (L->n_paths + lyg::AREA_WAVES - 1) / lyg::AREA_WAVES;
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(lyg::signed_area_kernel, dim3(blocks), dim3(lyg::WAVE * lyg::AREA_WAVES), 0,
+                       s, *L);
+    return hipGetLastError();
+}
+
+// The same result with one lane walking each path (for comparison).
+hipError_t lyg_launch_signed_area_serial(const lyg::ValidateLaunch* L, hipStream_t s) {
+    if (L->n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::signed_area_serial_kernel, dim3((L->n_paths + 255) / 256), dim3(256),
+                       0, s, *L);
     return hipGetLastError();
 }
 
