@@ -413,7 +413,37 @@ __device__ ArcD svg_arc_to_arc(vec2 from, vec2 to, vec2 radii, float xrot, bool 
     return ArcD{center, mk(rx, ry), start_angle, sweep_angle, xrot};
 }
 
-// PathParser::parse (parser.rs:154-175) and parse_path (:177-357)
+// Arguments a command reads (parser.rs:225-337): L/M/T 2, H/V 1, Q/S 4, C 6, A 7 (flags at
+// 3 and 4), Z 0; -1 for a character that is no command.  Endpoint attributes come on top.
+__device__ __forceinline__ int cmd_args(uint32_t c) {
+    switch (c | 0x20u) {
+        case 'm':
+        case 'l':
+        case 't':
+            return 2;
+        case 'h':
+        case 'v':
+            return 1;
+        case 'q':
+        case 's':
+            return 4;
+        case 'c':
+            return 6;
+        case 'a':
+            return 7;
+        case 'z':
+            return 0;
+        default:
+            return -1;
+    }
+}
+
+// PathParser::parse (parser.rs:154-175) and parse_path (:177-357), driven by tokens: every
+// iteration of the loop reads one number or flag, so the lanes of a wave (one string each)
+// spend their time in the same code whatever command each of them is in; a command is
+// applied when its last argument has been read.  Relative coordinates, smooth control
+// points and an arc's start use the position at the command's start, as parse_point /
+// get_smooth_ctrl do (the position only changes at the command's endpoint).
 template <bool EMIT>
 __device__ void parse_svg(const uint8_t* s, uint32_t a, uint32_t b, int nattr, int stop_at,
                           Out<EMIT>& O, PErr& E) {
@@ -426,158 +456,165 @@ __device__ void parse_svg(const uint8_t* s, uint32_t a, uint32_t b, int nattr, i
     bool has_pc = false, has_pq = false;
     vec2 pc = cur, pq = cur;
     uint32_t implicit = 'M';
-    float x, y, dummy;
+    uint32_t cmd = 0;
+    int k = 0, nargs = 0, nums = 0;  // next argument, arguments in all, of them before attrs
+    float g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, g6 = 0;
     skip_ws(S);
-    while (!S.finished) {
-        uint32_t cmd = S.cur;
-        const int cl = S.line, cc = S.col;
-        if (stop_at >= 0 && (uint32_t)stop_at == cmd) break;
-        if ((cmd | 0x20u) >= 'a' && (cmd | 0x20u) <= 'z') src_advance(S);
-        else cmd = implicit;
-        if (need_start && cmd != 'm' && cmd != 'M') {
-            E = PErr{LYG_SVG_ERR_MISSING_MOVE_TO, cl, cc, cmd};
-            break;
+    for (;;) {
+        if (k == 0) {
+            // a command boundary
+            if (S.finished) break;
+            uint32_t c = S.cur;
+            const int cl = S.line, cc = S.col;
+            if (stop_at >= 0 && (uint32_t)stop_at == c) break;
+            if ((c | 0x20u) >= 'a' && (c | 0x20u) <= 'z') src_advance(S);
+            else c = implicit;
+            if (need_start && c != 'm' && c != 'M') {
+                E = PErr{LYG_SVG_ERR_MISSING_MOVE_TO, cl, cc, c};
+                break;
+            }
+            const int na = cmd_args(c);
+            if (na < 0) {
+                E = PErr{LYG_SVG_ERR_COMMAND, cl, cc, c};
+                break;
+            }
+            cmd = c;
+            if (na == 0) {
+                // Z
+                O.end(true);
+                cur = first_pos;
+                need_end = false;
+                need_start = true;
+                has_pc = has_pq = false;
+                implicit = cmd == 'z' ? 'm' : 'M';
+                skip_ws(S);
+                continue;
+            }
+            if ((cmd | 0x20u) == 'm' && need_end) {
+                O.end(false);
+                need_end = false;
+            }
+            nums = na;
+            nargs = na + nattr;
         }
+        // one argument: a flag (an arc's 4th and 5th) or a number
+        float v = 0.0f;
+        if ((cmd | 0x20u) == 'a' && (k == 3 || k == 4)) {
+            bool f;
+            if (!parse_flag(S, &f, E)) break;
+            v = f ? 1.0f : 0.0f;
+        } else {
+            if (!parse_number(S, &v, E)) break;
+        }
+        if (k < nums) {
+            g0 = k == 0 ? v : g0;
+            g1 = k == 1 ? v : g1;
+            g2 = k == 2 ? v : g2;
+            g3 = k == 3 ? v : g3;
+            g4 = k == 4 ? v : g4;
+            g5 = k == 5 ? v : g5;
+            g6 = k == 6 ? v : g6;
+        }
+        ++k;
+        if (k < nargs) continue;
+        // the command's arguments are all read: apply it
+        k = 0;
         const bool rel = cmd >= 'a' && cmd <= 'z';
-        // parse_point / parse_endpoint / parse_attributes (:371-419)
-#define LYG_NUM(v) \
-    if (!parse_number(S, &(v), E)) goto done
-#define LYG_POINT(P)                 \
-    LYG_NUM(x);                      \
-    LYG_NUM(y);                      \
-    if (rel) {                       \
-        x += cur.x;                  \
-        y += cur.y;                  \
-    }                                \
-    P = mk(x, y)
-#define LYG_ATTRS() \
-    for (int k_ = 0; k_ < nattr; ++k_) LYG_NUM(dummy)
-#define LYG_ENDPOINT(P) \
-    LYG_POINT(P);       \
-    cur = P;            \
-    LYG_ATTRS()
-        vec2 to, c1, c2;
-        switch (cmd) {
-            case 'l':
-            case 'L':
-                LYG_ENDPOINT(to);
-                O.line_to(to);
-                break;
-            case 'h':
-            case 'H':
-                LYG_NUM(x);
-                if (rel) x += cur.x;
-                to = mk(x, cur.y);
+        const vec2 o = rel ? cur : mk(0.0f, 0.0f);
+        auto P = [&](float x, float y) { return rel ? mk(x + o.x, y + o.y) : mk(x, y); };
+        switch (cmd | 0x20u) {
+            case 'l': {
+                const vec2 to = P(g0, g1);
                 cur = to;
-                LYG_ATTRS();
                 O.line_to(to);
                 break;
-            case 'v':
-            case 'V':
-                LYG_NUM(y);
-                if (rel) y += cur.y;
-                to = mk(cur.x, y);
+            }
+            case 'h': {
+                const vec2 to = mk(rel ? g0 + cur.x : g0, cur.y);
                 cur = to;
-                LYG_ATTRS();
                 O.line_to(to);
                 break;
-            case 'q':
-            case 'Q':
-                LYG_POINT(c1);
-                LYG_ENDPOINT(to);
+            }
+            case 'v': {
+                const vec2 to = mk(cur.x, rel ? g0 + cur.y : g0);
+                cur = to;
+                O.line_to(to);
+                break;
+            }
+            case 'q': {
+                const vec2 c1 = P(g0, g1), to = P(g2, g3);
+                cur = to;
                 pq = c1;
                 has_pq = true;
                 O.quad_to(c1, to);
                 break;
-            case 't':
-            case 'T':
-                c1 = has_pq ? add(cur, sub(cur, pq)) : cur;
-                LYG_ENDPOINT(to);
+            }
+            case 't': {
+                const vec2 c1 = has_pq ? add(cur, sub(cur, pq)) : cur;
+                const vec2 to = P(g0, g1);
+                cur = to;
                 pq = c1;
                 has_pq = true;
                 O.quad_to(c1, to);
                 break;
-            case 'c':
-            case 'C':
-                LYG_POINT(c1);
-                LYG_POINT(c2);
-                LYG_ENDPOINT(to);
+            }
+            case 'c': {
+                const vec2 c1 = P(g0, g1), c2 = P(g2, g3), to = P(g4, g5);
+                cur = to;
                 pc = c2;
                 has_pc = true;
                 O.cubic_to(c1, c2, to);
                 break;
-            case 's':
-            case 'S':
-                c1 = has_pc ? add(cur, sub(cur, pc)) : cur;
-                LYG_POINT(c2);
-                LYG_ENDPOINT(to);
+            }
+            case 's': {
+                const vec2 c1 = has_pc ? add(cur, sub(cur, pc)) : cur;
+                const vec2 c2 = P(g0, g1), to = P(g2, g3);
+                cur = to;
                 pc = c2;
                 has_pc = true;
                 O.cubic_to(c1, c2, to);
                 break;
-            case 'a':
-            case 'A': {
+            }
+            case 'a': {
                 const vec2 from = cur;
-                float rx, ry, xdeg;
-                bool large_arc, sweep;
-                LYG_NUM(rx);
-                LYG_NUM(ry);
-                LYG_NUM(xdeg);
-                if (!parse_flag(S, &large_arc, E)) goto done;
-                if (!parse_flag(S, &sweep, E)) goto done;
-                LYG_ENDPOINT(to);
+                const vec2 to = P(g5, g6);
+                cur = to;
                 // Angle::degrees: f32::to_radians, x * (PI / 180)
-                const float xrot = xdeg * (3.14159265358979323846f / 180.0f);
+                const float xrot = g2 * (3.14159265358979323846f / 180.0f);
                 // SvgArc::is_straight_line (arc.rs:534-538)
                 const float EPS = 1.1920929e-07f;
-                if (ffabs(rx) <= EPS || ffabs(ry) <= EPS || veq(from, to)) {
+                if (ffabs(g0) <= EPS || ffabs(g1) <= EPS || veq(from, to)) {
                     O.line_to(to);
                 } else {
-                    const ArcD arc = svg_arc_to_arc(from, to, mk(rx, ry), xrot, large_arc, sweep);
+                    const ArcD arc = svg_arc_to_arc(from, to, mk(g0, g1), xrot, g3 != 0.0f,
+                                                    g4 != 0.0f);
                     arc_to_quads_each(arc, [&](int, vec2, vec2 ctrl, vec2 qto) {
                         O.quad_to(ctrl, qto);
                     });
                 }
                 break;
             }
-            case 'm':
-            case 'M':
-                if (need_end) {
-                    O.end(false);
-                    need_end = false;
-                }
-                LYG_ENDPOINT(to);
+            default: {  // m
+                const vec2 to = P(g0, g1);
+                cur = to;
                 first_pos = to;
                 O.begin(to);
                 need_end = true;
                 need_start = false;
                 break;
-            case 'z':
-            case 'Z':
-                O.end(true);
-                cur = first_pos;
-                need_end = false;
-                need_start = true;
-                break;
-            default:
-                E = PErr{LYG_SVG_ERR_COMMAND, cl, cc, cmd};
-                goto done;
+            }
         }
-#undef LYG_ENDPOINT
-#undef LYG_ATTRS
-#undef LYG_POINT
-#undef LYG_NUM
-        if (cmd == 'c' || cmd == 'C' || cmd == 's' || cmd == 'S') {
+        const uint32_t lc = cmd | 0x20u;
+        if (lc == 'c' || lc == 's') {
             has_pq = false;
-        } else if (cmd == 'q' || cmd == 'Q' || cmd == 't' || cmd == 'T') {
+        } else if (lc == 'q' || lc == 't') {
             has_pc = false;
         } else {
             has_pc = has_pq = false;
         }
-        implicit = cmd == 'm' ? 'l' : cmd == 'M' ? 'L' : cmd == 'z' ? 'm' : cmd == 'Z' ? 'M' : cmd;
+        implicit = cmd == 'm' ? 'l' : cmd == 'M' ? 'L' : cmd;
         skip_ws(S);
     }
-done:
     if (need_end) O.end(false);
     if (S.non_ascii) E = PErr{LYG_SVG_ERR_NON_ASCII, S.na_line, S.na_col, 0u};
 }
