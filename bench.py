@@ -346,8 +346,8 @@ def main():
         from lyon_amd.serve import Server
         serving = []
         streams = [
-            ("stroke, 4-64 segments", 100000.0, 20000, 1.0, 4, 64),
-            ("stroke, 4-256 segments", 100000.0, 20000, 1.0, 4, 256),
+            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64),
+            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256),
             ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256),
         ]
         for name, rate, nreq, sfrac, smin, smax in streams:
