@@ -452,3 +452,31 @@ def test_svg_arcs_follow_the_geometry():
             assert np.allclose(p[-1], [50, 35], atol=30 * 3e-4)
     assert O.parse_svg("M 1 1 A 0 5 0 0 1 9 9")[0].tolist() == [3, 0, 5]
     assert O.parse_svg("M 1 1 A 5 5 0 0 1 1 1")[0].tolist() == [3, 0, 5]
+
+
+def test_svg_output_fits_the_staging_room():
+    """The device parser writes string i into a room of len + 2 verbs and 2 len + 2 points:
+    every command consumes a character and makes at most one verb and two points per
+    character.  Checked on the densest inputs (arcs packed as tightly as the syntax allows,
+    glued signs, '.5.5' runs, repeated moves and closes) and on random strings."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from helpers import random_svg
+    dense = ["M0 0A1 1 0 111 0" + " 1 1 0 110 0 1 1 0 111 0" * 50,  # 7 quadratics per arc
+             "M0 0A1 1 0 011 1" + " 1 1 0 011 1" * 50,
+             "M0 0a1-1-0 01-1-1" * 20,
+             "M0 0T1 1" + "-1-1" * 100,
+             "M.5.5q.5.5.5.5" + ".5.5.5.5" * 60,
+             "M1 1Z" * 40, "M1 1M1 1M1 1", "m1 1z1 1z1 1z",
+             "M0 0c1 1 1 1 1 1" + " 1-1-1-1-1-1" * 40,
+             "M0 0A1 1 0 111 0" * 30, "M0 0a9 9 0 1 1 1e-3 0" * 20]
+    rng = np.random.default_rng(9)
+    dense += [random_svg(rng, int(rng.integers(1, 60)), arcs=True, corrupt=0.2)
+              for _ in range(500)]
+    worst = 0.0
+    for s in dense:
+        v, p, e = O.parse_svg(s)
+        n = len(s.encode())
+        assert len(v) <= n + 2 and len(p) <= 2 * n + 2, (s[:40], len(v), len(p), n)
+        worst = max(worst, len(p) / max(n, 1))
+    assert worst > 1.0  # the arcs really are denser than one point per byte

@@ -60,6 +60,8 @@ def test_random_svg_matches_oracle(arcs):
     strings = [random_svg(rng, int(rng.integers(1, 40)), arcs=arcs, corrupt=0.25)
                for _ in range(1500)]
     strings += ["", "   ", "M 1 2", "Z", "z 1 1", "M 0 0 Z Z", "m 1 1 2 2 3 3 z 4 4"]
+    if arcs:  # the densest output per byte (7 quadratics per 12 characters)
+        strings.append("M0 0A1 1 0 111 0" + " 1 1 0 110 0 1 1 0 111 0" * 50)
     batch, errs = S.parse_paths(strings)
     n_err = 0
     for i, s in enumerate(strings):
@@ -102,3 +104,21 @@ def test_non_ascii_and_options():
     assert e is None and p.verbs.tolist() == [3, 0, 5]
     with pytest.raises(ValueError):
         S.parse_path("M 0 0", S.ParserOptions(stop_at="é"))
+
+
+def test_attributes_syntax_matches_oracle():
+    """The extended syntax (ParserOptions::num_attributes numbers after every endpoint) on
+    random strings: verbs, points and errors as the oracle parses them."""
+    rng = np.random.default_rng(23)
+    for nattr in (1, 3):
+        strings = []
+        for _ in range(300):
+            s = random_svg(rng, int(rng.integers(1, 20)), arcs=True, corrupt=0.1)
+            strings.append(s)
+        batch, errs = S.parse_paths(strings, S.ParserOptions(num_attributes=nattr))
+        for i, s in enumerate(strings):
+            rv, rp, re = O.parse_svg(s, num_attributes=nattr)
+            p = batch.path(i)
+            assert _err(errs[i]) == re.tolist(), s
+            assert p.verbs.tolist() == rv.tolist(), s
+            assert np.array_equal(p.points, rp, equal_nan=True), s
