@@ -156,3 +156,33 @@ def test_batch_matches_plain_calls():
         assert np.array_equal(gv.view(np.uint32), np.asarray(rv, np.float32).view(np.uint32)), \
             f"batch path {i}: vertices differ"
         assert np.array_equal(canon(gi), canon(ri)), f"batch path {i}: triangle sets differ"
+
+
+def test_very_large_path():
+    """fill_tests.rs:2140-2166: a path of 1,000,000 endpoints (two wiggling sides of a long
+    band) fills without error; the same vertices and triangle count as the oracle and the
+    same area."""
+    from lyon_amd import PathBuilder
+    n = 1_000_000
+    d = np.float32(0.0)
+    step = np.float32(0.1)
+    pts = np.zeros((n, 2), np.float32)
+    for k in range(n // 2):
+        pts[k] = (np.cos(d, dtype=np.float32), d)
+        d = np.float32(d + step)
+    for k in range(n // 2, n):
+        pts[k] = (np.float32(np.cos(d, dtype=np.float32) + np.float32(30.0)), d)
+        d = np.float32(d - step)
+    b = PathBuilder()
+    b.begin((0.0, 0.0))
+    for p in pts:
+        b.line_to(p)
+    b.end(True)
+    path = b.build()
+    vb = VertexBuffers()
+    FillTessellator().tessellate_path(path, FillOptions(), BuffersBuilder(vb))
+    st, rv, ri = O.fill(path, 0.1, 0)
+    assert st == 0
+    assert len(vb.vertices) == len(rv) and len(vb.indices) == len(ri)
+    c = O.coverage_compare(rv, ri, vb.vertices, vb.indices, n=256)
+    assert c["mismatch"] == 0 and abs(c["area_a"] - c["area_b"]) <= 1e-6 * c["area_b"]

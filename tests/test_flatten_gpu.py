@@ -87,3 +87,31 @@ def test_device_segments_match_host_and_oracle():
         a = dp[do[i]:do[i + 1]]
         r = rp[ro[j]:ro[j + 1]]
         assert np.array_equal(a.view(np.uint32), r.view(np.uint32)), f"segment {i}"
+
+
+def test_issue_652_degenerate_cubic_terminates():
+    """cubic_bezier.rs:1648-1666: a cubic whose control points lie on one vertical line
+    and return to its start flattens (and stops) at tolerances 1, 0.1 and 0.01; its last
+    point is `to` exactly."""
+    ctrl = np.array([[[-1061.0, -3327.0], [-1061.0, -3177.0], [-1061.0, -3477.0],
+                      [-1061.0, -3327.0]]], np.float32)
+    for tol in (1.0, 0.1, 0.01):
+        gp, go = flatten_segments(np.array([2], np.int32), ctrl, tol)
+        rp, ro = O.flatten_segments(np.array([2], np.int32), ctrl, tol)
+        assert np.array_equal(go, ro) and 1 <= go[1] < 10000
+        assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+        assert gp[-1].tolist() == [-1061.0, -3327.0]
+
+
+def test_arc_negative_flattening_step():
+    """arc.rs:1207-1236: an arc of sweep -898 rad flattens to a finite list, and a
+    negative sweep of -2 rad flattens to more than one segment."""
+    arcs = np.array([[-100.0, -150.0, 50.0, 50.0, 0.982944787, -898.0, 0.0],
+                     [0.0, 0.0, 100.0, 10.0, 0.2, -2.0, 0.0]], np.float32)
+    tols = (0.100000001, 0.1)
+    for k in range(2):
+        gp, go = flatten_arcs(arcs[k:k + 1], tols[k])
+        rp, ro = O.flatten_arcs(arcs[k:k + 1], tols[k])
+        assert np.array_equal(go, ro)
+        assert np.max(np.abs(gp.astype(np.float64) - rp)) <= 1e-5 * 300
+    assert go[1] - go[0] > 1
