@@ -1,8 +1,10 @@
 """The C-ABI library loads and exports every function include/lyg.h declares; the
 synthetic generators are deterministic.  No compute calls (no GPU needed)."""
 import ctypes as C
+import os
 
 import numpy as np
+import pytest
 
 from lyon_amd import _lib, workloads
 
@@ -34,3 +36,33 @@ def test_plain_path_is_closed_64_segments():
     p = workloads.plain_path(0)
     assert p.num_segments() == 64 and p.verbs[-1] == 4
     assert np.array_equal(p.points[-1], p.points[0])
+
+
+def test_header_is_c_and_links():
+    """include/lyg.h compiles as C99 and a C program links against liblyg.so and calls a
+    host-only entry point (the FFI a C or Rust caller uses)."""
+    import shutil
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib_dir = os.path.join(root, "lyon_amd")
+    if not os.path.exists(os.path.join(lib_dir, "liblyg.so")) or not shutil.which("gcc"):
+        pytest.skip("liblyg.so not built")
+    src = ('#include "lyg.h"\n#include <stdio.h>\n#include <string.h>\n'
+           'int main(void) {\n'
+           '  lyg_path_view v; lyg_output o; lyg_fill_options f; lyg_stroke_options s;\n'
+           '  memset(&v, 0, sizeof v); memset(&o, 0, sizeof o); (void)f; (void)s;\n'
+           '  const char* m = lyg_status_string(LYG_ERR_CAPACITY);\n'
+           '  printf("%s\\n", m ? m : "");\n'
+           '  return lyg_fill_rectangle(0, 0, &o) == LYG_ERR_INVALID_ARGUMENT ? 0 : 3;\n}\n')
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        with open(c, "w") as fh:
+            fh.write(src)
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        c, "-o", exe, "-L", lib_dir, "-llyg", "-Wl,-rpath," + lib_dir],
+                       check=True, capture_output=True)
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+        assert r.stdout.strip()
