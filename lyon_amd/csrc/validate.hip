@@ -9,8 +9,7 @@
 //     query; the count is an integer, so the wave sum is exact in any order.
 //   * approximate signed areas (approximate_signed_area, crates/algorithms/src/area.rs:7-54):
 //     lyon's serial f32 sum of cross products in event order; one wave per path computes the
-//     terms 64 at a time and adds them in order (signed_area_kernel), or one lane walks each
-//     path (signed_area_serial_kernel).
+//     terms 64 at a time and adds them in order.
 //
 // Paths are as lyon's builders make them: every sub-path opens with Begin, so the current
 // point of a verb is the last point consumed before it (path.rs:843-909).  Curves are
@@ -154,54 +153,6 @@ __global__ void __launch_bounds__(WAVE* VAL_WAVES) winding_kernel(ValidateLaunch
             if (L.inside) L.inside[q] = L.fill_rule == 0 ? (w % 2 != 0) : (w != 0);
         }
     }
-}
-
-__global__ void __launch_bounds__(256) signed_area_serial_kernel(ValidateLaunch L) {
-    const uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pid >= L.n_paths) return;
-    const uint32_t ve = L.verb_off[pid + 1];
-    const uint32_t pb = L.point_off[pid];
-    const PathPts pts{L.points + 2ull * pb, L.point_off[pid + 1] - pb};
-    uint32_t vi = L.verb_off[pid], pi = 0;
-    float area = 0.0f;
-    // approximate_signed_area: sub-path after sub-path until one is not Begin ... End
-    while (vi < ve && L.verbs[vi] == 3u) {
-        ++vi;
-        const vec2 first = pts.at(pi++);
-        vec2 cur = first;
-        float double_area = 0.0f;
-        vec2 v0 = mk(0.0f, 0.0f);
-        bool ended = false;
-        auto line_to = [&](vec2 to) {
-            const vec2 v1 = sub(to, first);
-            double_area = double_area + cross(v0, v1);
-            v0 = v1;
-        };
-        while (vi < ve) {
-            const uint32_t verb = L.verbs[vi++];
-            if (verb == 3u) break;  // a Begin inside a sub-path ends the sum
-            if (verb == 0u) {
-                cur = pts.at(pi++);
-                line_to(cur);
-            } else if (verb == 1u || verb == 2u) {
-                const vec2 c1 = pts.at(pi), c2 = pts.at(pi + 1u);
-                const vec2 to = verb == 1u ? c2 : pts.at(pi + 2u);
-                pi += verb == 1u ? 2u : 3u;
-                for_each_flat_point((int)verb, cur, c1, c2, to, L.tolerance,
-                                    [&](vec2 p) { line_to(p); });
-                cur = to;
-            } else {
-                // End / Close (path.rs:887-905): the closing term, last - first
-                if (verb == 4u) ++pi;
-                double_area = double_area + cross(v0, sub(cur, first));
-                ended = true;
-                break;
-            }
-        }
-        if (!ended) break;
-        area = area + double_area * 0.5f;
-    }
-    L.areas[pid] = area;
 }
 
 // approximate_signed_area with one wave per path.  lyon's result is a serial f32 sum in
@@ -414,12 +365,5 @@ hipError_t lyg_launch_signed_area(const lyg::ValidateLaunch* L, hipStream_t s) {
     return hipGetLastError();
 }
 
-// The same result with one lane walking each path (for comparison).
-hipError_t lyg_launch_signed_area_serial(const lyg::ValidateLaunch* L, hipStream_t s) {
-    if (L->n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(lyg::signed_area_serial_kernel, dim3((L->n_paths + 255) / 256), dim3(256),
-                       0, s, *L);
-    return hipGetLastError();
-}
 
 }  // extern "C"
