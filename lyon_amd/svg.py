@@ -91,7 +91,20 @@ def pack_strings(strings):
 
 class DeviceSvgBatch:
     """Parsed paths left in device memory: verbs, points, verb_offsets, point_offsets
-    (DeviceBuffers, the DeviceBatch layout), errors (n x 4 int32) and the totals."""
+    (DeviceBuffers, the DeviceBatch layout), errors (n x 4 int32) and the totals.  With
+    options attached (:meth:`with_options`) it feeds StrokeBatcher / FillBatcher directly:
+    text in, triangles out, nothing on the host in between."""
+
+    opts = None
+    n_opts = 0
+
+    def with_options(self, opts) -> "DeviceSvgBatch":
+        """Attaches stroke or fill option records: one per path, or one for all."""
+        from .batch import DeviceBuffer
+        raw = np.ascontiguousarray(opts)
+        self.opts = DeviceBuffer.from_numpy(raw.view(np.uint8).reshape(-1))
+        self.n_opts = len(raw)
+        return self
 
     def c_in(self) -> B.BatchInC:
         return B.BatchInC(self.verbs.ptr, self.points.ptr, self.verb_offsets.ptr,

@@ -122,3 +122,24 @@ def test_attributes_syntax_matches_oracle():
             assert _err(errs[i]) == re.tolist(), s
             assert p.verbs.tolist() == rv.tolist(), s
             assert np.array_equal(p.points, rp, equal_nan=True), s
+
+
+def test_text_to_triangles_on_the_device():
+    """SVG text parsed on the device feeds the batched stroke and fill directly; the
+    triangles are byte-identical to those of the original paths."""
+    from lyon_amd.batch import DeviceBatch, DeviceBuffer, FillBatcher, StrokeBatcher
+    from lyon_amd.workloads import fill_batch, fill_options, stroke_options
+    cases = [(stroke_batch(800, 8, 128, 2048.0, seed=4), stroke_options(800, tolerance=0.05),
+              StrokeBatcher),
+             (fill_batch(300, seed=2), fill_options(300, tolerance=0.05), FillBatcher)]
+    for src, opts, Batcher in cases:
+        strings = [to_svg(src.path(i)) for i in range(len(src))]
+        text, offs = S.pack_strings(strings)
+        parsed = S.parse_paths_device(DeviceBuffer.from_numpy(text), DeviceBuffer.from_numpy(offs),
+                                      len(strings)).with_options(opts)
+        assert parsed.n_errors == 0
+        a = Batcher().run(parsed)
+        b = Batcher().run(DeviceBatch(src, opts))
+        assert a.n_vertices == b.n_vertices and a.n_indices == b.n_indices
+        assert np.array_equal(a.all_vertices().view(np.uint32), b.all_vertices().view(np.uint32))
+        assert np.array_equal(a.all_indices(), b.all_indices())
