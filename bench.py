@@ -24,7 +24,9 @@ The same JSON line carries:
     per-path sample;
   * roofline: algorithmic bytes of the stroke emit kernel (1 B/verb + 8 B/point +
     8 B/vertex + 4 B/index) over its average duration (HIP events on the library's
-    stream); traffic from rocprofv3 counters when given (--traffic-json);
+    stream), as a fraction of the spec peak and of this device's measured copy rate (a
+    1 GiB device copy, read + write); traffic from rocprofv3 counters when given
+    (--traffic-json);
   * serving: the serving loop (continuous batching, 8 GiB device pool, 4 in-flight
     batches) under seeded Poisson request streams, with p50/p99/max request latency;
   * cpu_baseline: the oracle (C++ restatement of lyon's tessellators) on the box's host
@@ -172,6 +174,18 @@ def main():
     alg_bytes = len(batch.verbs) + 8 * len(batch.points) + 8 * n_vertices + 4 * n_indices
     emit_avg = float(np.mean(emit_ms))
     achieved = alg_bytes / (emit_avg / 1e3) / 1e9
+
+    # measured copy rate of this device (1 GiB device-to-device copy kernel, read + write)
+    from lyon_amd.batch import DeviceBuffer as _DB
+    _src, _dst = _DB(1 << 30), _DB(1 << 30)
+    L.lyg_device_copy(ctx, _dst.ptr, _src.ptr, 1 << 30)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        L.lyg_device_copy(ctx, _dst.ptr, _src.ptr, 1 << 30)
+    synchronize()
+    copy_gbs = 2.0 * (1 << 30) * 5 / (time.perf_counter() - t0) / 1e9
+    del _src, _dst
 
     log("stroke: plain per-path drop-in")
     ns = min(args.plain_sample, args.paths)
@@ -433,6 +447,7 @@ def main():
         "kernel_ms": {"sizing_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,
                      "kernel": "stroke_kernel<true> (emit)", "algorithmic_bytes": alg_bytes},
         "flatten": flat,
         "validators": validators,
