@@ -2,12 +2,12 @@
 //
 // Segments (LineSegment / QuadraticBezierSegment / CubicBezierSegment): a count kernel
 // evaluates Wang's formula per segment (one lane each), the host-side runtime scans the
-// counts, and an emit kernel gives every output point its own lane: the lane finds its
-// segment in a window of segment offsets, recovers t with the closed-form serial chain
-// (lyg_math.h: tchain, resumed from the lane's previous point of the same segment) and
-// evaluates the polynomial form with Horner's rule in lyon's operation order
-// (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).  Output is fully coalesced:
-// consecutive lanes write consecutive points.
+// counts, and an emit kernel gives every lane a run of 16 consecutive output points: the
+// lane finds the segment of its first point in an LDS window of segment offsets, recovers
+// that point's t with the closed-form serial chain (lyg_math.h: tchain) and continues
+// lyon's serial sum from there, evaluating the polynomial form with Horner's rule in
+// lyon's operation order (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).  The
+// points pass through LDS so that the stores are fully coalesced.
 //
 // Arcs (Arc::for_each_flattened_with_t, arc.rs:339-368) re-split the remaining arc at
 // every step, a serial recurrence, so one lane walks one arc.
@@ -66,24 +66,30 @@ __global__ void flatten_count_kernel(FlattenLaunch L) {
     L.counts[i] = s.n;
 }
 
-// Emit: each wave writes FL_ITERS * 64 consecutive output points, 64 per iteration (lane l
-// the point base + 64 i + l: coalesced stores).  The segments of an iteration's 64 points
-// are found in an LDS window of the next 64 segment offsets (binary search in LDS); a lane
-// keeps the polynomial of the segment it is in and its last t, so its next point of the
-// same segment continues lyon's serial sum 64 steps on (a point near its segment's start
-// sums from 0); only a wave's first points, deep inside a segment, use the closed form.
-static constexpr uint32_t FL_ITERS = 16;
 static constexpr uint32_t FL_WAVES = 4;  // waves per block
 
-__global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_kernel(FlattenLaunch L,
-                                                                      uint32_t total) {
-    __shared__ uint32_t win_all[FL_WAVES][WAVE];
+// Emit: each wave covers 1,024 consecutive output points; lane l owns points
+// [16 l, 16 l + 16) of them.  The wave first loads the offsets of every segment its
+// points touch into LDS (each segment has at least one point, so at most 1,025 offsets);
+// a lane finds the segment of its first point there, takes t from the closed form (or the
+// serial sum near the segment's start) once, then adds the step once per point, which is
+// lyon's own serial sum continued.  The points go through LDS so that every store writes
+// 64 consecutive points.
+static constexpr uint32_t FR_RUN = 16;
+static constexpr uint32_t FR_WIN = WAVE * FR_RUN + 1;
+
+__global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_runs_kernel(FlattenLaunch L,
+                                                                           uint32_t total) {
+    __shared__ uint32_t win_all[FL_WAVES][FR_WIN];
+    __shared__ float2 stage_all[FL_WAVES][WAVE * (FR_RUN + 1)];  // a run per lane, padded
     const int l = lane_id();
     const uint32_t w = threadIdx.x / WAVE;
     uint32_t* win = win_all[w];
+    float2* stage = stage_all[w];
     const uint64_t wave = (uint64_t)blockIdx.x * FL_WAVES + w;
-    const uint64_t base = wave * WAVE * FL_ITERS;
-    if (base >= total) return;
+    const uint64_t base = wave * WAVE * FR_RUN;
+    if (base >= total) return;  // wave-uniform
+    const uint64_t wend = base + WAVE * FR_RUN < total ? base + WAVE * FR_RUN : total;
     // segment of the wave's first point: last segment whose offset <= base
     uint32_t cur = 0;
     {
@@ -95,85 +101,84 @@ __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_kernel(FlattenLau
         }
         cur = lo;
     }
-    uint32_t seg = 0xffffffffu, seg_begin = 0, k_prev = 0;
-    float t_prev = 0.0f;
-    SegPoly sp{};
-    for (uint32_t it = 0; it < FL_ITERS; ++it) {
-        const uint64_t first = base + (uint64_t)it * WAVE;
-        if (first >= total) break;  // uniform
-        const uint32_t g = (uint32_t)(first + (uint32_t)l);
-        const bool valid = (uint64_t)g < total;
-        // window of segment starts cur .. cur + 63 (segments past the last start "never")
-        const uint32_t wi = cur + (uint32_t)l;
-        const uint32_t wv = wi < L.n ? L.offsets[wi] : 0xffffffffu;
-        __builtin_amdgcn_wave_barrier();
-        win[l] = wv;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        uint32_t my = cur;
-        if (valid) {
-            if (win[WAVE - 1] <= g) {
-                // more than 64 segments start before this point: search all of them
-                uint32_t lo = cur, hi = L.n - 1;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi + 1) >> 1;
-                    if (L.offsets[mid] <= g) lo = mid;
-                    else hi = mid - 1;
-                }
-                my = lo;
-            } else {
-                int lo = 0, hi = WAVE - 1;  // win[0] <= g < win[63]
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (win[mid] <= g) lo = mid;
-                    else hi = mid - 1;
-                }
-                my = cur + (uint32_t)lo;
-            }
-            if (my != seg) {
-                seg = my;
-                sp = seg_poly(L.kinds[seg], L.ctrl + 8ull * seg, L.tolerance);
-                seg_begin = L.offsets[seg];
-                k_prev = 0;
-                t_prev = 0.0f;
-            }
-            const uint32_t k = g - seg_begin;
-            vec2 p;
-            if (k + 1 >= sp.n) {
-                p = sp.to;
-            } else {
-                // t after k + 1 steps: lyon's own serial sum continued from this lane's
-                // previous point of the segment (64 steps back, or from 0 near the segment's
-                // start); the closed form only for a wave's first points deep in a segment
-                float t = t_prev;
-                const uint32_t d = k + 1 - k_prev;
-                if (d == WAVE) {
-#pragma unroll
-                    for (int q = 0; q < WAVE; ++q) t = t + sp.step;
-                } else if (d < WAVE) {
-                    for (uint32_t q = 0; q < d; ++q) t = t + sp.step;
-                } else {
-                    t = tchain(sp.step, k + 1);
-                }
-                k_prev = k + 1;
-                t_prev = t;
-                vec2 v = sp.a0;
-                float tt = t;
-                v = add(v, mul(sp.a1, tt));
-                tt = tt * t;
-                v = add(v, mul(sp.a2, tt));
-                if (sp.cubic) {
-                    tt = tt * t;
-                    v = add(v, mul(sp.a3, tt));
-                }
-                p = v;
-            }
-            reinterpret_cast<float2*>(L.out_points)[g] = make_float2(p.x, p.y);
+    // offsets of segments cur .. until one starts at or beyond the wave's end (L.offsets
+    // has n + 1 entries; the last is the total)
+    uint32_t nwin = 0;
+    for (;;) {
+        const uint32_t i = cur + nwin + (uint32_t)l;
+        const uint32_t v = i <= L.n ? L.offsets[i] : 0xffffffffu;
+        if (nwin + (uint32_t)l < FR_WIN) win[nwin + l] = v;
+        const uint32_t last = readlane_u(v, WAVE - 1);
+        nwin += WAVE;
+        if (last >= wend || nwin >= FR_WIN || cur + nwin > L.n) break;  // uniform
+    }
+    if (nwin > FR_WIN) nwin = FR_WIN;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t g0 = base + (uint64_t)l * FR_RUN;
+    if (g0 < wend) {
+    const uint32_t g_end = (uint32_t)(g0 + FR_RUN < wend ? g0 + FR_RUN : wend);
+    // segment of g0 in the window: last j with win[j] <= g0
+    int lo = 0, hi = (int)nwin - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (win[mid] <= g0) lo = mid;
+        else hi = mid - 1;
+    }
+    int j = lo;
+    uint32_t seg = cur + (uint32_t)j;
+    SegPoly sp = seg_poly(L.kinds[seg], L.ctrl + 8ull * seg, L.tolerance);
+    uint32_t seg_begin = win[j], seg_end = win[j + 1];
+    // t of the lane's first point (point k + 1 of its segment)
+    uint32_t k = (uint32_t)g0 - seg_begin;
+    float t = 0.0f;
+    if (k + 1 < sp.n) {
+        if (k + 1 <= WAVE) {
+            for (uint32_t q = 0; q <= k; ++q) t = t + sp.step;
+        } else {
+            t = tchain(sp.step, k + 1);
         }
-        // the next iteration starts in the segment of this iteration's last point
-        const uint64_t left = total - first;
-        const int last = left >= (uint64_t)WAVE ? WAVE - 1 : (int)left - 1;
-        cur = readlane_u(my, last);
+    }
+    float2* st = stage + (uint32_t)l * (FR_RUN + 1);
+    for (uint32_t g = (uint32_t)g0; g < g_end; ++g) {
+        if (g >= seg_end) {
+            // the next segment (every segment has at least one point)
+            ++j;
+            seg = cur + (uint32_t)j;
+            sp = seg_poly(L.kinds[seg], L.ctrl + 8ull * seg, L.tolerance);
+            seg_begin = seg_end;
+            seg_end = win[j + 1];
+            k = 0;
+            t = 0.0f + sp.step;
+        }
+        vec2 p;
+        if (k + 1 >= sp.n) {
+            p = sp.to;
+        } else {
+            vec2 v = sp.a0;
+            float tt = t;
+            v = add(v, mul(sp.a1, tt));
+            tt = tt * t;
+            v = add(v, mul(sp.a2, tt));
+            if (sp.cubic) {
+                tt = tt * t;
+                v = add(v, mul(sp.a3, tt));
+            }
+            p = v;
+        }
+        st[g - (uint32_t)g0] = make_float2(p.x, p.y);
+        ++k;
+        t = t + sp.step;
+    }
+    }
+    // the wave's points leave LDS in order: 64 consecutive points per store
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    float2* out = reinterpret_cast<float2*>(L.out_points);
+#pragma unroll
+    for (uint32_t r = 0; r < FR_RUN; ++r) {
+        const uint32_t q = r * WAVE + (uint32_t)l;  // point base + q
+        if (base + q < wend) out[base + q] = stage[(q / FR_RUN) * (FR_RUN + 1) + q % FR_RUN];
     }
 }
 
@@ -397,8 +402,9 @@ hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s) 
 }
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s) {
     if (total == 0) return hipSuccess;
-    const uint64_t per_block = (uint64_t)lyg::WAVE * lyg::FL_ITERS * lyg::FL_WAVES;
-    hipLaunchKernelGGL(lyg::flatten_emit_kernel, dim3((uint32_t)((total + per_block - 1) / per_block)),
+    const uint64_t per_block = (uint64_t)lyg::WAVE * lyg::FR_RUN * lyg::FL_WAVES;
+    hipLaunchKernelGGL(lyg::flatten_emit_runs_kernel,
+                       dim3((uint32_t)((total + per_block - 1) / per_block)),
                        dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
     return hipGetLastError();
 }

@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(WAVE* VAL_WAVES) winding_kernel(ValidateLaunch
 // event order, so the additions stay serial, but everything before them is spread over the
 // lanes: the verbs are read 64 at a time; between two boundary verbs (Begin, End, Close) the
 // flattened points of a run of segments are computed 64 at a time, one per lane (t resumed
-// from the lane's previous point of the same segment, as in flatten_emit_kernel, so it is
+// from the lane's previous point of the same segment, so it is
 // lyon's serial sum), each lane forms its cross product with the point before it, and the
 // 64 terms are added to the running sum in order through readlane.  Begin / End / Close
 // are applied in order between the runs, with area.rs's rules for ill-formed paths.
