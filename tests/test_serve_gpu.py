@@ -50,8 +50,11 @@ def same(a, b):
         np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) and np.array_equal(a[2], b[2]))))
 
 
-def test_results_match_oracle():
-    with Server(pool_bytes=1 << 30, max_batch_paths=64, max_inflight=2) as s:
+@pytest.mark.parametrize("stroke_sizing", [0, 1])
+def test_results_match_oracle(stroke_sizing):
+    # stroke_sizing 0: one staged emit pass per batch; 1: count pass then emit
+    with Server(pool_bytes=1 << 30, max_batch_paths=64, max_inflight=2,
+                stroke_sizing=stroke_sizing) as s:
         for k, (kind, p, o) in enumerate(requests(6, seed=3)):
             t = submit(s, kind, p, o)
             st, v, i = s.result(t)
