@@ -4,7 +4,7 @@ Drop-in mirror of lyon_tessellation's API (FillTessellator / StrokeTessellator /
 FillOptions / StrokeOptions / VertexBuffers / BuffersBuilder) and of lyon_geom's
 flattening, computed by the HIP kernels in liblyg.so.  See include/lyg.h for the C ABI.
 """
-from .path import (Path, PathBatch, PathBuilder, PathEvent, polygon, VERB_BEGIN, VERB_CLOSE, VERB_CUBIC,
+from .path import (Path, PathBatch, PathBuffer, PathBuilder, PathEvent, polygon, VERB_BEGIN, VERB_CLOSE, VERB_CUBIC,
                    VERB_END, VERB_LINE, VERB_QUADRATIC)
 from .tessellation import (BuffersBuilder, FillOptions, FillRule, FillTessellator, LineCap,
                            LineJoin, Orientation, StrokeOptions, StrokeTessellator,
@@ -12,7 +12,7 @@ from .tessellation import (BuffersBuilder, FillOptions, FillRule, FillTessellato
                            flatten_segments, simple_builder)
 
 __all__ = [
-    "Path", "PathBatch", "PathBuilder", "PathEvent", "polygon", "VERB_BEGIN", "VERB_CLOSE", "VERB_CUBIC",
+    "Path", "PathBatch", "PathBuffer", "PathBuilder", "PathEvent", "polygon", "VERB_BEGIN", "VERB_CLOSE", "VERB_CUBIC",
     "VERB_END", "VERB_LINE", "VERB_QUADRATIC", "BuffersBuilder", "FillOptions", "FillRule",
     "FillTessellator", "LineCap", "LineJoin", "Orientation", "StrokeOptions",
     "StrokeTessellator", "TessellationError", "VertexBuffers", "arc_to_quadratics",

@@ -293,3 +293,66 @@ class PathBatch:
         a, b = int(self.verb_offsets[i]), int(self.verb_offsets[i + 1])
         c, d = int(self.point_offsets[i]), int(self.point_offsets[i + 1])
         return Path(self.verbs[a:b], self.points[c:d])
+
+
+class PathBuffer:
+    """lyon_path's ``PathBuffer`` (path_buffer.rs:14-104): many paths stored contiguously.
+    Paths are appended with :meth:`builder` (whose ``build()`` returns the new path's
+    index, path_buffer.rs:230-245) or :meth:`push`; :meth:`as_batch` gives the packed
+    layout the batched tessellators and the SVG parser use (verbs and points end to end,
+    n + 1 offsets)."""
+
+    def __init__(self):
+        self._paths: list[Path] = []
+        self._batch = None
+
+    def __len__(self):
+        return len(self._paths)
+
+    def is_empty(self) -> bool:
+        return not self._paths
+
+    def get(self, index: int) -> Path:
+        return self._paths[index]
+
+    def indices(self):
+        return range(len(self._paths))
+
+    def iter(self):
+        return iter(self._paths)
+
+    def clear(self):
+        self._paths.clear()
+        self._batch = None
+
+    def push(self, path: Path) -> int:
+        self._paths.append(path)
+        self._batch = None
+        return len(self._paths) - 1
+
+    def builder(self) -> "PathBufferBuilder":
+        return PathBufferBuilder(self)
+
+    def as_batch(self) -> PathBatch:
+        if self._batch is None:
+            self._batch = PathBatch.from_paths(self._paths)
+        return self._batch
+
+    @staticmethod
+    def from_batch(batch: PathBatch) -> "PathBuffer":
+        b = PathBuffer()
+        for i in range(len(batch)):
+            b.push(batch.path(i))
+        return b
+
+
+class PathBufferBuilder(PathBuilder):
+    """PathBuffer::builder (path_buffer.rs:195-260): a path builder whose ``build()``
+    appends the path to its buffer and returns the path's index."""
+
+    def __init__(self, buffer: PathBuffer):
+        super().__init__()
+        self._buffer = buffer
+
+    def build(self) -> int:  # type: ignore[override]
+        return self._buffer.push(PathBuilder.build(self))

@@ -38,3 +38,26 @@ def test_from_events_round_trip():
         q = Path.from_events(p.iter())
         assert np.array_equal(q.verbs, p.verbs)
         assert np.array_equal(q.points.view(np.uint32), p.points.view(np.uint32))
+
+
+def test_path_buffer():
+    """PathBuffer (path_buffer.rs:14-104): paths appended through its builder get
+    consecutive indices, read back unchanged, and pack into the batched layout."""
+    from lyon_amd.path import PathBuffer
+    buf = PathBuffer()
+    assert buf.is_empty() and len(buf.as_batch()) == 0
+    b = buf.builder()
+    b.add_polygon([(0.0, 0.0), (1.0, 0.0), (1.0, 1.0)], True)
+    assert b.build() == 0
+    assert buf.push(_sample()) == 1
+    assert list(buf.indices()) == [0, 1]
+    batch = buf.as_batch()
+    assert len(batch) == 2
+    for i, p in enumerate(buf.iter()):
+        q = batch.path(i)
+        assert np.array_equal(q.verbs, p.verbs) and np.array_equal(q.points, p.points)
+    assert buf.get(1).verbs.tolist() == _sample().verbs.tolist()
+    again = PathBuffer.from_batch(batch)
+    assert len(again) == 2 and np.array_equal(again.get(0).points, buf.get(0).points)
+    buf.clear()
+    assert len(buf) == 0
