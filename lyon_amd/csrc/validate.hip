@@ -158,12 +158,13 @@ __global__ void __launch_bounds__(WAVE* VAL_WAVES) winding_kernel(ValidateLaunch
 // approximate_signed_area with one wave per path.  lyon's result is a serial f32 sum in
 // event order, so the additions stay serial, but everything before them is spread over the
 // lanes: the verbs are read 64 at a time; between two boundary verbs (Begin, End, Close) the
-// flattened points of a run of segments are computed 64 at a time, one per lane (t resumed
-// from the lane's previous point of the same segment, so it is
-// lyon's serial sum), each lane forms its cross product with the point before it, and the
-// 64 terms are added to the running sum in order through readlane.  Begin / End / Close
+// flattened points of a run of segments are computed 1,024 at a time, 16 consecutive ones
+// per lane (t from the closed form once, then lyon's serial sum), each lane forms the cross
+// products of its points with the points before them, and the terms are added to the
+// running sum in point order through readlane.  Begin / End / Close
 // are applied in order between the runs, with area.rs's rules for ill-formed paths.
 constexpr int AREA_WAVES = 4;
+constexpr int AREA_RUN = 16;  // consecutive points per lane
 
 __global__ void __launch_bounds__(WAVE* AREA_WAVES) signed_area_kernel(ValidateLaunch L) {
     __shared__ uint32_t pre_all[AREA_WAVES][WAVE + 1];  // term prefix of the window's verbs
@@ -245,95 +246,121 @@ __global__ void __launch_bounds__(WAVE* AREA_WAVES) signed_area_kernel(ValidateL
                     break;
                 }
                 const uint32_t r0 = pre[j], r1 = pre[j2];
-                uint32_t seg = 0xffffffffu, k_prev = 0;
-                float t_prev = 0.0f;
-                int kind = 0;
-                vec2 a0 = mk(0.0f, 0.0f), a1 = a0, a2 = a0, a3 = a0, to = a0;
-                float step = 1.0f;
-                uint32_t n_pts = 1;
-                for (uint32_t q0 = r0; q0 < r1; q0 += WAVE) {
-                    const uint32_t g = q0 + (uint32_t)l;
-                    const bool valid = g < r1;
-                    vec2 p = cur;
-                    if (valid) {
-                        int lo = j, hi = j2 - 1;  // last verb of the run with pre <= g
+                for (uint32_t q0 = r0; q0 < r1; q0 += WAVE * AREA_RUN) {
+                    // lane l: points [g0, gend) of the run, in order
+                    const uint32_t g0 = q0 + (uint32_t)l * AREA_RUN;
+                    const uint32_t gend = g0 < r1 ? (g0 + AREA_RUN < r1 ? g0 + AREA_RUN : r1) : g0;
+                    vec2 v1s[AREA_RUN];
+                    vec2 p_last = cur, v1_last = mk(0.0f, 0.0f);
+#pragma unroll
+                    for (int u = 0; u < AREA_RUN; ++u) v1s[u] = mk(0.0f, 0.0f);
+                    if (g0 < gend) {
+                        int lo = j, hi = j2 - 1;  // last verb of the run with pre <= g0
                         while (lo < hi) {
                             const int mid = (lo + hi + 1) >> 1;
-                            if (pre[mid] <= g) lo = mid;
+                            if (pre[mid] <= g0) lo = mid;
                             else hi = mid - 1;
                         }
-                        const uint32_t my = (uint32_t)lo;
-                        if (my != seg) {
-                            seg = my;
-                            const uint32_t sp = pis[my];
-                            kind = (int)vbs[my];
-                            k_prev = 0;
-                            t_prev = 0.0f;
+                        int my = lo;
+                        int kind = 0;
+                        vec2 a0 = mk(0.0f, 0.0f), a1 = a0, a2 = a0, a3 = a0, to = a0;
+                        float step = 1.0f, t = 0.0f;
+                        uint32_t n_pts = 1, k = 0;
+                        auto setup = [&](int v, uint32_t kk) {
+                            const uint32_t sp = pis[v];
+                            kind = (int)vbs[v];
+                            k = kk;
                             if (kind == 0) {
                                 to = pts.at(sp);
                                 n_pts = 1;
+                                return;
+                            }
+                            const vec2 f = pts.at(sp - 1u), c1 = pts.at(sp), c2 = pts.at(sp + 1u);
+                            float nf;
+                            if (kind == 1) {
+                                nf = quad_wang(f, c1, c2, L.tolerance);
+                                a1 = mul(sub(c1, f), 2.0f);
+                                a2 = sub(add(f, c2), mul(c1, 2.0f));
+                                to = c2;
                             } else {
-                                const vec2 f = pts.at(sp - 1u), c1 = pts.at(sp), c2 = pts.at(sp + 1u);
-                                float nf;
-                                if (kind == 1) {
-                                    nf = quad_wang(f, c1, c2, L.tolerance);
-                                    a1 = mul(sub(c1, f), 2.0f);
-                                    a2 = sub(add(f, c2), mul(c1, 2.0f));
-                                    to = c2;
+                                const vec2 c3 = pts.at(sp + 2u);
+                                nf = cubic_wang(f, c1, c2, c3, L.tolerance);
+                                a1 = mul(sub(c1, f), 3.0f);
+                                a2 = add(sub(mul(f, 3.0f), mul(c1, 6.0f)), mul(c2, 3.0f));
+                                a3 = add(sub(c3, f), mul(sub(c1, c2), 3.0f));
+                                to = c3;
+                            }
+                            a0 = f;
+                            step = 1.0f / nf;
+                            n_pts = seg_count(nf);
+                            // t of point kk + 1: lyon's serial sum (closed form deep inside)
+                            t = 0.0f;
+                            if (kk + 1 < n_pts) {
+                                if (kk + 1 <= (uint32_t)WAVE) {
+                                    for (uint32_t q = 0; q <= kk; ++q) t = t + step;
                                 } else {
-                                    const vec2 c3 = pts.at(sp + 2u);
-                                    nf = cubic_wang(f, c1, c2, c3, L.tolerance);
-                                    a1 = mul(sub(c1, f), 3.0f);
-                                    a2 = add(sub(mul(f, 3.0f), mul(c1, 6.0f)), mul(c2, 3.0f));
-                                    a3 = add(sub(c3, f), mul(sub(c1, c2), 3.0f));
-                                    to = c3;
+                                    t = tchain(step, kk + 1);
                                 }
-                                a0 = f;
-                                step = 1.0f / nf;
-                                n_pts = seg_count(nf);
                             }
-                        }
-                        const uint32_t k = g - pre[my];  // point k + 1 of the segment
-                        if (k + 1 >= n_pts) {
-                            p = to;
-                        } else {
-                            float t = t_prev;
-                            const uint32_t d = k + 1 - k_prev;
-                            if (d == WAVE) {
+                        };
+                        setup(my, g0 - pre[my]);
 #pragma unroll
-                                for (int u = 0; u < WAVE; ++u) t = t + step;
-                            } else if (d < WAVE) {
-                                for (uint32_t u = 0; u < d; ++u) t = t + step;
-                            } else {
-                                t = tchain(step, k + 1);
+                        for (int u = 0; u < AREA_RUN; ++u) {
+                            const uint32_t g = g0 + (uint32_t)u;
+                            if (g < gend) {
+                                if (g >= pre[my + 1]) {
+                                    ++my;
+                                    setup(my, 0);
+                                }
+                                vec2 p;
+                                if (k + 1 >= n_pts) {
+                                    p = to;
+                                } else {
+                                    vec2 v = a0;
+                                    float tt = t;
+                                    v = add(v, mul(a1, tt));
+                                    tt = tt * t;
+                                    v = add(v, mul(a2, tt));
+                                    if (kind == 2) {
+                                        tt = tt * t;
+                                        v = add(v, mul(a3, tt));
+                                    }
+                                    p = v;
+                                }
+                                ++k;
+                                t = t + step;
+                                v1s[u] = sub(p, first);
+                                v1_last = v1s[u];
+                                p_last = p;
                             }
-                            k_prev = k + 1;
-                            t_prev = t;
-                            vec2 v = a0;
-                            float tt = t;
-                            v = add(v, mul(a1, tt));
-                            tt = tt * t;
-                            v = add(v, mul(a2, tt));
-                            if (kind == 2) {
-                                tt = tt * t;
-                                v = add(v, mul(a3, tt));
-                            }
-                            p = v;
                         }
                     }
-                    const vec2 v1 = sub(p, first);
-                    vec2 v0 = shfl_v(v1, l == 0 ? 0 : l - 1);
+                    // the point before the lane's first: the previous lane's last (its run is
+                    // full whenever this one has points), or the carry for lane 0
+                    vec2 v0 = shfl_v(v1s[AREA_RUN - 1], l == 0 ? 0 : l - 1);
                     if (l == 0) v0 = v0c;
-                    const float term = cross(v0, v1);
-                    const uint32_t m = r1 - q0 < (uint32_t)WAVE ? r1 - q0 : (uint32_t)WAVE;
-                    if (m == (uint32_t)WAVE) {
+                    float terms[AREA_RUN];
 #pragma unroll
-                        for (int u = 0; u < WAVE; ++u) da = da + readlane_f(term, u);
-                    } else {
-                        for (uint32_t u = 0; u < m; ++u) da = da + readlane_f(term, (int)u);
+                    for (int u = 0; u < AREA_RUN; ++u) {
+                        terms[u] = cross(v0, v1s[u]);
+                        v0 = v1s[u];
                     }
-                    v0c = readlane_v(v1, (int)m - 1);
-                    cur = readlane_v(p, (int)m - 1);
+                    // lyon's serial sum, in point order
+                    const uint32_t m = r1 - q0 < (uint32_t)(WAVE * AREA_RUN) ? r1 - q0
+                                                                             : (uint32_t)(WAVE * AREA_RUN);
+                    const uint32_t full = m / AREA_RUN, rem = m % AREA_RUN;
+                    for (uint32_t ln = 0; ln < full; ++ln) {
+#pragma unroll
+                        for (int u = 0; u < AREA_RUN; ++u) da = da + readlane_f(terms[u], (int)ln);
+                    }
+                    if (rem) {
+#pragma unroll
+                        for (int u = 0; u < AREA_RUN; ++u)
+                            if ((uint32_t)u < rem) da = da + readlane_f(terms[u], (int)full);
+                    }
+                    const int last_lane = (int)((m - 1) / AREA_RUN);
+                    v0c = readlane_v(v1_last, last_lane);
+                    cur = readlane_v(p_last, last_lane);
                 }
                 j = j2;
             }
