@@ -10,6 +10,7 @@ enums), geometry_builder.rs (VertexBuffers, BuffersBuilder) and error.rs
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from enum import IntEnum
 
 import numpy as np
@@ -191,19 +192,40 @@ def simple_builder(buffers: VertexBuffers) -> BuffersBuilder:
     return BuffersBuilder(buffers)
 
 
-class _Ctx:
-    """One lyg context per process-thread (created on first use, device 0)."""
+class _CtxHandle:
+    """Owns one lyg context; destroyed with the thread that made it."""
 
-    _ctx = None
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __del__(self):
+        try:
+            B.lib().lyg_ctx_destroy(self.handle)
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
+
+
+class _Ctx:
+    """One lyg context per Python thread (created on first use, device 0).
+
+    A lyg context is not thread-safe (its scratch buffers, pinned staging and stream are
+    shared by every call made through it), and ctypes releases the GIL during a call, so
+    each thread gets its own context, kept in thread-local storage."""
+
+    _local = threading.local()
+    _lock = threading.Lock()
 
     @classmethod
     def get(cls):
-        if cls._ctx is None:
-            h = B.lib().lyg_ctx_create(0)
-            if not h:
+        h = getattr(cls._local, "ctx", None)
+        if h is None:
+            with cls._lock:
+                handle = B.lib().lyg_ctx_create(0)
+            if not handle:
                 raise RuntimeError("lyg_ctx_create failed (no usable HIP device?)")
-            cls._ctx = h
-        return cls._ctx
+            h = _CtxHandle(handle)
+            cls._local.ctx = h
+        return h.handle
 
 
 def _path_view(path: Path) -> B.PathViewC:

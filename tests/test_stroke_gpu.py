@@ -224,3 +224,45 @@ def test_large_batch_offsets():
         assert len(r[0]) == vo[i + 1] - vo[i] and len(r[1]) == io[i + 1] - io[i], f"path {i}"
         assert_same_geometry(res.path(i), r, max(bbox_diag(p.points), so.line_width),
                              what=f"large batch path {i}")
+
+
+def test_threads_use_their_own_contexts():
+    """Python threads calling the tessellators at once each get their own lyg context
+    (a context is not thread-safe): results equal the single-threaded ones."""
+    import threading
+    from lyon_amd import FillOptions, FillTessellator
+    batch = workloads.stroke_batch(48, 8, 96, 1024.0, seed=21)
+    opts = StrokeOptions(line_width=3.0, line_join=LineJoin.Round, start_cap=LineCap.Round,
+                         end_cap=LineCap.Square, tolerance=0.05)
+    fopts = FillOptions(tolerance=0.05)
+
+    def run(indices):
+        out = {}
+        for i in indices:
+            p = batch.path(i)
+            s = gpu_stroke(p, opts)
+            fb = VertexBuffers()
+            FillTessellator().tessellate_path(p, fopts, BuffersBuilder(fb))
+            out[i] = (s[0].copy(), s[1].copy(), fb.vertices.copy(), fb.indices.copy())
+        return out
+
+    alone = run(range(len(batch)))
+    results = [None, None]
+    errors = []
+
+    def worker(k):
+        try:
+            results[k] = run(range(k, len(batch), 2))
+        except Exception as e:  # surfaced below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for k in (0, 1):
+        for i, got in results[k].items():
+            for a, b in zip(got, alone[i]):
+                assert np.array_equal(a, b), f"path {i} differs when run in thread {k}"
