@@ -203,6 +203,17 @@ uint64_t orc_batch_triangles(const uint8_t* verbs, const float* points, const ui
     return tot;
 }
 
+// stroke.rs:3178-3212 find_sharp_turn: returns 1 and the split t when the quadratic turns
+// sharply, else 0.
+int orc_find_sharp_turn(float fx, float fy, float cx, float cy, float tx, float ty, float* t) {
+    *t = 0.0f;
+    return refstroke::find_sharp_turn_export(fx, fy, cx, cy, tx, ty, t) ? 1 : 0;
+}
+// stroke.rs:2002-2004 / 2488-2490: round join / cap subdivision depth.
+uint32_t orc_arc_subdivisions(float diff, float radius, float tolerance) {
+    return refstroke::arc_subdivisions_export(diff, radius, tolerance);
+}
+
 }  // extern "C"
 
 extern "C" {

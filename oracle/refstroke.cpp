@@ -652,4 +652,13 @@ int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output
     return OK;
 }
 
+// Exposed for the tests that pin these rules against the geometry they encode.
+bool find_sharp_turn_export(float fx, float fy, float cx, float cy, float tx, float ty,
+                            float* t_out) {
+    return find_sharp_turn(v2(fx, fy), v2(cx, cy), v2(tx, ty), t_out);
+}
+uint32_t arc_subdivisions_export(float diff, float radius, float tolerance) {
+    return arc_subdivisions(diff, radius, tolerance);
+}
+
 }  // namespace refstroke

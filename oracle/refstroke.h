@@ -110,5 +110,8 @@ struct PointBuffer {
 };
 
 int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output& out);
+bool find_sharp_turn_export(float fx, float fy, float cx, float cy, float tx, float ty,
+                            float* t_out);
+uint32_t arc_subdivisions_export(float diff, float radius, float tolerance);
 
 }  // namespace refstroke
