@@ -191,6 +191,26 @@ int orc_winding_coverage_compare(const uint8_t* verbs, uint32_t n_verbs, const f
     size_t ns = (size_t)n * n;
     std::vector<uint8_t> ca(ns, 0), near(ns, 0), wb(ns, 0);
     double ar = 0;
+    // the edges each row of samples can meet (within eps), as lists per row
+    std::vector<uint32_t> rstart((size_t)n + 1, 0);
+    auto row_range = [&](size_t e, long* j0, long* j1) {
+        const double a = std::min(edges[4 * e + 1], edges[4 * e + 3]) - eps;
+        const double b = std::max(edges[4 * e + 1], edges[4 * e + 3]) + eps;
+        *j0 = std::max(0L, (long)std::floor((a - g.y0) / g.dy - 0.5));
+        *j1 = std::min((long)n - 1, (long)std::ceil((b - g.y0) / g.dy - 0.5));
+    };
+    for (size_t e = 0; e < ne; ++e) {
+        long j0, j1;
+        row_range(e, &j0, &j1);
+        for (long j = j0; j <= j1; ++j) ++rstart[j + 1];
+    }
+    for (uint32_t j = 0; j < n; ++j) rstart[j + 1] += rstart[j];
+    std::vector<uint32_t> rlist(rstart[n]), rcur(rstart.begin(), rstart.end() - 1);
+    for (size_t e = 0; e < ne; ++e) {
+        long j0, j1;
+        row_range(e, &j0, &j1);
+        for (long j = j0; j <= j1; ++j) rlist[rcur[j]++] = (uint32_t)e;
+    }
     parallel_rows(n, [&](unsigned, uint32_t r0, uint32_t r1) {
         raster(v, idx, 0, nt, g, eps, r0, r1, ca.data(), near.data(), r0 == 0 ? &ar : nullptr);
         // winding per row: crossings with half-open y rule, then prefix over x
@@ -198,7 +218,8 @@ int orc_winding_coverage_compare(const uint8_t* verbs, uint32_t n_verbs, const f
         for (uint32_t j = r0; j < r1; ++j) {
             double sy = g.sy(j);
             xs.clear();
-            for (size_t e = 0; e < ne; ++e) {
+            for (uint32_t q = rstart[j]; q < rstart[j + 1]; ++q) {
+                const size_t e = rlist[q];
                 double ax = edges[4 * e], ay = edges[4 * e + 1], bx = edges[4 * e + 2], by = edges[4 * e + 3];
                 if ((ay <= sy) == (by <= sy)) continue;
                 double t = (sy - ay) / (by - ay);
@@ -215,7 +236,8 @@ int orc_winding_coverage_compare(const uint8_t* verbs, uint32_t n_verbs, const f
                 wb[(size_t)j * n + i] = in;
             }
             // exclusion around the path's edges
-            for (size_t e = 0; e < ne; ++e) {
+            for (uint32_t q = rstart[j]; q < rstart[j + 1]; ++q) {
+                const size_t e = rlist[q];
                 double ax = edges[4 * e], ay = edges[4 * e + 1], bx = edges[4 * e + 2], by = edges[4 * e + 3];
                 if (sy < std::min(ay, by) - eps || sy > std::max(ay, by) + eps) continue;
                 double mnx = std::min(ax, bx) - eps, mxx = std::max(ax, bx) + eps;

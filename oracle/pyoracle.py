@@ -208,6 +208,27 @@ def winding_compare(path, tolerance, fill_rule, v, i, box=None, n=4096, eps=1e-4
     return r
 
 
+def fill_areas(path, tolerance):
+    """Exact (f64) areas of the path's fill from its region boundary, independent of any
+    triangulation (oracle/refarea.cpp).  Returns (EvenOdd area, NonZero area, number of
+    edge crossings)."""
+    L = lib()
+    L.orc_fill_area.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float,
+                                C.POINTER(C.c_double)]
+    L.orc_fill_area.restype = C.c_int
+    v = np.ascontiguousarray(path.verbs, np.uint8)
+    p = np.ascontiguousarray(path.points, np.float32)
+    out = (C.c_double * 3)()
+    L.orc_fill_area(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), tolerance, out)
+    return float(out[0]), float(out[1]), int(out[2])
+
+
+def fill_area(path, tolerance, fill_rule):
+    """(area of the fill under fill_rule, number of edge crossings)."""
+    eo, nz, k = fill_areas(path, tolerance)
+    return (nz if fill_rule else eo), k
+
+
 def winding_numbers(path, xy, tolerance):
     """path_winding_number_at_position (hit_test.rs:21-83) of every point of xy (k, 2)."""
     v = np.ascontiguousarray(path.verbs, np.uint8)
