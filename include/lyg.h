@@ -171,6 +171,19 @@ int32_t lyg_flatten_segments_device(lyg_ctx* ctx, const int32_t* kinds, const fl
                                     uint32_t n, float tolerance, float* out_points,
                                     uint32_t point_capacity, uint32_t* offsets,
                                     uint32_t* n_points);
+/* No counterpart in lyon.  lyg_flatten_segments_device for callers that cannot allocate
+ * (code under HIP graph capture, serving loops and harnesses that forbid hipMalloc): the
+ * same kernels and results, but the per-segment counts, scan partials and totals live in a
+ * fixed device array of the library (one per GPU, about 4.2 MiB, held until the process
+ * ends) instead of context buffers, and the call allocates nothing.  n is limited to
+ * lyg_flatten_noalloc_max_segments() (1,048,576); larger n returns LYG_ERR_CAPACITY.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several contexts or threads) race and must be serialised by the caller. */
+int32_t lyg_flatten_segments_device_noalloc(lyg_ctx* ctx, const int32_t* kinds,
+                                            const float* ctrl, uint32_t n, float tolerance,
+                                            float* out_points, uint32_t point_capacity,
+                                            uint32_t* offsets, uint32_t* n_points);
+uint32_t lyg_flatten_noalloc_max_segments(void);
 /* Arc::for_each_flattened_with_t (arc.rs:339-368) for n arcs of 7 floats each
  * (center x, y, radii x, y, start angle, sweep angle, x rotation).  Host pointers. */
 int32_t lyg_flatten_arcs(lyg_ctx* ctx, const float* arcs, uint32_t n, float tolerance,

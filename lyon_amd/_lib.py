@@ -127,6 +127,10 @@ SIGNATURES = {
     "lyg_flatten_segments_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
                                      C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)],
                                     C.c_int32),
+    "lyg_flatten_segments_device_noalloc": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                             C.c_float, C.c_void_p, C.c_uint32, C.c_void_p,
+                                             C.POINTER(C.c_uint32)], C.c_int32),
+    "lyg_flatten_noalloc_max_segments": ([], C.c_uint32),
     "lyg_flatten_arcs": ([C.c_void_p, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32, U32P, U32P],
                          C.c_int32),
     "lyg_arc_to_quadratics": ([C.c_void_p, F32P, C.c_uint32, F32P, C.c_uint32, U32P, U32P],

@@ -13,6 +13,8 @@
 // every step, a serial recurrence, so one lane walks one arc.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "lyg_math.h"
 #include "lyg_types.h"
 #include "stroke_core.h"
@@ -449,6 +451,62 @@ hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t 
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(lyg::fill_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
+}
+
+}  // extern "C"
+
+// ---- fixed scratch of the allocation-free flatten entry point -------------------------
+// lyg_flatten_segments_device_noalloc keeps its per-segment counts, the tile sums of its
+// offset scan, its totals and its overflow flag in this array instead of buffers it would
+// allocate.  Every GPU holds its own copy for the life of the process; host code reaches
+// it only through the address hipGetSymbolAddress returns for the current device (the
+// array's name in host code is not a device address).
+namespace {
+constexpr uint32_t NOALLOC_MAX_SEGMENTS = 1u << 20;
+constexpr size_t NOALLOC_COUNTERS_AT = 0;    // u32[16]: [0] scan overflow flag
+constexpr size_t NOALLOC_TOTALS_AT = 256;    // u64[4]: [0] flattened points
+constexpr size_t NOALLOC_COUNTS_AT = 512;    // u32[NOALLOC_MAX_SEGMENTS]
+constexpr size_t NOALLOC_TILES_AT = NOALLOC_COUNTS_AT + 4ull * NOALLOC_MAX_SEGMENTS;
+constexpr size_t NOALLOC_TILES_BYTES =
+    8ull * ((NOALLOC_MAX_SEGMENTS + lyg::SCAN_TILE - 1) / lyg::SCAN_TILE + 1);
+constexpr size_t NOALLOC_SCRATCH_BYTES = (NOALLOC_TILES_AT + NOALLOC_TILES_BYTES + 255) & ~size_t(255);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+extern "C" {
+
+uint32_t lyg_noalloc_max_segments() { return NOALLOC_MAX_SEGMENTS; }
+
+// Once per device, with that device current (lyg_ctx_create calls it): looks up this
+// device's copy of `scratch` and keeps its address.
+hipError_t lyg_noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (noalloc_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    noalloc_ptr[device] = p;
+    return hipSuccess;
+}
+
+// The scratch layout for `device` (null pointers before lyg_noalloc_scratch_init).
+void lyg_noalloc_scratch(int device, uint32_t** counters, uint64_t** totals, uint32_t** counts,
+                         uint64_t** tiles) {
+    uint8_t* base = nullptr;
+    if (device >= 0 && device < NOALLOC_MAX_DEVICES) {
+        std::lock_guard<std::mutex> g(noalloc_mu);
+        base = static_cast<uint8_t*>(noalloc_ptr[device]);
+    }
+    *counters = base ? reinterpret_cast<uint32_t*>(base + NOALLOC_COUNTERS_AT) : nullptr;
+    *totals = base ? reinterpret_cast<uint64_t*>(base + NOALLOC_TOTALS_AT) : nullptr;
+    *counts = base ? reinterpret_cast<uint32_t*>(base + NOALLOC_COUNTS_AT) : nullptr;
+    *tiles = base ? reinterpret_cast<uint64_t*>(base + NOALLOC_TILES_AT) : nullptr;
 }
 
 }  // extern "C"

@@ -36,6 +36,10 @@ hipError_t lyg_launch_scan_tiles(const uint32_t* counts, uint32_t n, uint32_t* o
                                  uint64_t* total, uint32_t* overflow, uint64_t* tiles,
                                  hipStream_t s);
 hipError_t lyg_launch_copy(void* dst, const void* src, uint64_t bytes, int grid, hipStream_t s);
+uint32_t lyg_noalloc_max_segments();
+hipError_t lyg_noalloc_scratch_init(int device);
+void lyg_noalloc_scratch(int device, uint32_t** counters, uint64_t** totals, uint32_t** counts,
+                         uint64_t** tiles);
 }
 
 #include "ctx.h"
@@ -316,7 +320,8 @@ lyg_ctx* lyg_ctx_create(int device) {
         return nullptr;
     }
     if (hipMalloc(&c->d_counters, 64) != hipSuccess || hipMalloc(&c->d_totals, 64) != hipSuccess ||
-        hipHostMalloc(&c->h_pinned, 256, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&c->h_pinned, 256, hipHostMallocDefault) != hipSuccess ||
+        lyg_noalloc_scratch_init(device) != hipSuccess) {
         lyg_ctx_destroy(c);
         return nullptr;
     }
@@ -692,6 +697,67 @@ int32_t lyg_flatten_segments_device(lyg_ctx* c, const int32_t* kinds, const floa
     if (!check(c, hipStreamSynchronize(s), "sync")) return LYG_ERR_DEVICE;
     return LYG_OK;
 }
+
+// lyg_flatten_segments_device without any allocation: the counts, the scan's tile sums,
+// the totals and the overflow flag live in the library's fixed per-device scratch array
+// (set up by lyg_ctx_create); the host reads the total through the context's pinned
+// staging.  No hipMalloc / hipFree / allocating library call on this path.
+int32_t lyg_flatten_segments_device_noalloc(lyg_ctx* c, const int32_t* kinds, const float* ctrl,
+                                            uint32_t n, float tolerance, float* out_points,
+                                            uint32_t point_capacity, uint32_t* offsets,
+                                            uint32_t* n_points) {
+    if (!c || (n && (!kinds || !ctrl || !offsets))) return LYG_ERR_INVALID_ARGUMENT;
+    if (!(tolerance > 0.0f)) return LYG_ERR_TOLERANCE_IS_NAN;
+    if (n_points) *n_points = 0;
+    if (n > lyg_noalloc_max_segments()) {
+        c->last_error = "lyg_flatten_segments_device_noalloc: more segments than its fixed "
+                        "scratch holds (lyg_noalloc_max_segments)";
+        return LYG_ERR_CAPACITY;
+    }
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    uint32_t *counters, *counts;
+    uint64_t *totals, *tiles;
+    lyg_noalloc_scratch(c->device, &counters, &totals, &counts, &tiles);
+    if (!counters) {
+        c->last_error = "allocation-free scratch not set up on this device";
+        return LYG_ERR_DEVICE;
+    }
+    hipStream_t s = c->stream;
+    if (n == 0) {
+        return check(c, lyg_launch_fill_u32(offsets, 1, 0, s), "zero") &&
+                       check(c, hipStreamSynchronize(s), "sync")
+                   ? LYG_OK
+                   : LYG_ERR_DEVICE;
+    }
+    lyg::FlattenLaunch L{};
+    L.kinds = kinds;
+    L.ctrl = ctrl;
+    L.n = n;
+    L.tolerance = tolerance;
+    L.counts = counts;
+    // the scratch keeps the previous call's contents: the overflow flag must start at zero
+    if (!check(c, lyg_launch_fill_u32(counters, 16, 0, s), "reset") ||
+        !check(c, lyg_launch_flatten_count(&L, s), "flatten count") ||
+        !check(c, lyg_launch_scan_tiles(counts, n, offsets, totals, counters, tiles, s), "scan"))
+        return LYG_ERR_DEVICE;
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    if (!check(c, hipMemcpyAsync(h, totals, 8, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipMemcpyAsync(h + 1, counters, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t total = h[0];
+    if ((uint32_t)h[1] != 0 || total > 0xffffffffull) return LYG_ERR_TOO_MANY_VERTICES;
+    if (n_points) *n_points = (uint32_t)total;
+    if (total > point_capacity || !out_points) return LYG_ERR_CAPACITY;
+    L.offsets = offsets;
+    L.out_points = out_points;
+    if (!check(c, lyg_launch_flatten_emit(&L, (uint32_t)total, s), "flatten emit") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+uint32_t lyg_flatten_noalloc_max_segments(void) { return lyg_noalloc_max_segments(); }
 
 static int32_t arcs_common(lyg_ctx* c, const float* arcs, uint32_t n, float tol, bool quads,
                            float* out, uint32_t cap, uint32_t* offsets, uint32_t* n_out) {
