@@ -4,6 +4,7 @@
 // extern "C" face of the CPU restatement: stroke, fill and the lyon_geom flattening
 // calls, each producing VertexBuffers<[f32;2],u32> / point lists in a result handle.
 #include <atomic>
+#include <cstring>
 #include <chrono>
 #include <cstdlib>
 #include <thread>
@@ -202,6 +203,82 @@ uint64_t orc_batch_triangles(const uint8_t* verbs, const float* points, const ui
     }
     return tot;
 }
+
+// Every path of a batch tessellated on n_threads host threads, each path's output kept
+// (path-local ids): orc_batch_counts gives the per-path statuses and sizes,
+// orc_batch_copy the outputs back to back in path order.
+struct BatchOut {
+    std::vector<int> status;
+    std::vector<std::vector<V2>> v;
+    std::vector<std::vector<uint32_t>> i;
+};
+
+BatchOut* orc_batch_run(const uint8_t* verbs, const float* points, const uint32_t* verb_off,
+                        const uint32_t* point_off, const float* opts, uint32_t n_paths,
+                        int is_fill, int n_threads) {
+    BatchOut* r = new BatchOut();
+    r->status.assign(n_paths, 0);
+    r->v.resize(n_paths);
+    r->i.resize(n_paths);
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const uint32_t a = next.fetch_add(8);
+            if (a >= n_paths) break;
+            const uint32_t b = a + 8 < n_paths ? a + 8 : n_paths;
+            for (uint32_t p = a; p < b; ++p) {
+                refpath::PathView pv{verbs + verb_off[p], verb_off[p + 1] - verb_off[p],
+                                     points + 2 * (size_t)point_off[p],
+                                     point_off[p + 1] - point_off[p]};
+                if (is_fill) {
+                    reffill::FillOptions o;
+                    o.tolerance = opts[2 * p];
+                    o.fill_rule = (int)opts[2 * p + 1];
+                    reffill::Output out{&r->v[p], &r->i[p], 0};
+                    r->status[p] = reffill::fill_path(pv, o, out);
+                } else {
+                    const float* q = opts + 6 * (size_t)p;
+                    refstroke::StrokeOptions o;
+                    o.line_width = q[0];
+                    o.miter_limit = q[1];
+                    o.tolerance = q[2];
+                    o.start_cap = (int)q[3];
+                    o.end_cap = (int)q[4];
+                    o.line_join = (int)q[5];
+                    refstroke::Output out{&r->v[p], &r->i[p], 0};
+                    r->status[p] = refstroke::stroke_path(pv, o, out);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < (n_threads > 1 ? n_threads : 1); ++t) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+    return r;
+}
+
+void orc_batch_counts(const BatchOut* r, int32_t* status, uint64_t* nv, uint64_t* ni) {
+    for (size_t p = 0; p < r->status.size(); ++p) {
+        status[p] = r->status[p];
+        nv[p] = r->v[p].size();
+        ni[p] = r->i[p].size();
+    }
+}
+
+void orc_batch_copy(const BatchOut* r, float* vertices, uint32_t* indices) {
+    size_t ov = 0, oi = 0;
+    for (size_t p = 0; p < r->status.size(); ++p) {
+        for (const V2& q : r->v[p]) {
+            vertices[2 * ov] = q.x;
+            vertices[2 * ov + 1] = q.y;
+            ++ov;
+        }
+        std::memcpy(indices + oi, r->i[p].data(), 4 * r->i[p].size());
+        oi += r->i[p].size();
+    }
+}
+
+void orc_batch_free(BatchOut* r) { delete r; }
 
 // stroke.rs:3178-3212 find_sharp_turn: returns 1 and the split t when the quadratic turns
 // sharply, else 0.

@@ -141,6 +141,42 @@ def batch_triangles(batch, opts, is_fill, n_threads=1):
                                      int(is_fill), int(n_threads))
 
 
+def batch_tessellate(batch, opts, is_fill, n_threads=None):
+    """Every path of a PathBatch on the host's cores (the parity reference for full-size
+    batches).  opts as batch_triangles.  Returns (status (n,), vertex offsets (n+1,),
+    index offsets (n+1,), vertices (N, 2) f32, indices (M,) path-local u32)."""
+    import os
+    L = lib()
+    L.orc_batch_run.restype = C.c_void_p
+    L.orc_batch_run.argtypes = [U8P, F32P, U32P, U32P, F32P, C.c_uint32, C.c_int, C.c_int]
+    L.orc_batch_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]
+    L.orc_batch_copy.argtypes = [C.c_void_p, F32P, U32P]
+    L.orc_batch_free.argtypes = [C.c_void_p]
+    n = len(batch)
+    if n_threads is None:
+        n_threads = min(16, len(os.sched_getaffinity(0)))
+    o = np.ascontiguousarray(opts, np.float32)
+    h = L.orc_batch_run(_ptr(batch.verbs, U8P), _ptr(batch.points, F32P),
+                        _ptr(batch.verb_offsets, U32P), _ptr(batch.point_offsets, U32P),
+                        _ptr(o, F32P), n, int(is_fill), int(n_threads))
+    try:
+        st = np.zeros(n, np.int32)
+        nv = np.zeros(n, np.uint64)
+        ni = np.zeros(n, np.uint64)
+        L.orc_batch_counts(h, st.ctypes.data_as(C.POINTER(C.c_int32)),
+                           nv.ctypes.data_as(C.POINTER(C.c_uint64)),
+                           ni.ctypes.data_as(C.POINTER(C.c_uint64)))
+        voff = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+        ioff = np.concatenate([[0], np.cumsum(ni)]).astype(np.int64)
+        v = np.empty((int(voff[-1]), 2), np.float32)
+        i = np.empty(int(ioff[-1]), np.uint32)
+        L.orc_batch_copy(h, _ptr(v, F32P), _ptr(i, U32P))
+    finally:
+        L.orc_batch_free(h)
+    return st, voff, ioff, v, i
+
+
 def _box(points, pad_frac=0.01):
     p = np.asarray(points, np.float64).reshape(-1, 2)
     p = p[np.isfinite(p).all(axis=1)]
