@@ -27,8 +27,10 @@ The same JSON line carries:
     stream), as a fraction of the spec peak and of this device's measured copy rate (a
     1 GiB device copy, read + write); traffic from rocprofv3 counters when given
     (--traffic-json);
-  * serving: the serving loop (continuous batching, 8 GiB device pool, 4 in-flight
-    batches) under seeded Poisson request streams, with p50/p99/max request latency;
+  * serving: the serving loop (continuous batching, 4 in-flight batches) under seeded
+    Poisson request streams, with p50/p99/max request latency: short strokes, a 4-256
+    segment mix, and the serving configuration (100k req/s offered, log-uniform 4-65536
+    segments, 60% strokes / 40% random-walk fills, 32 GiB pool);
   * cpu_baseline: the oracle (C++ restatement of lyon's tessellators) on the box's host
     cores over a bounded sample of the same workloads, single thread and all cores.
 """
@@ -56,19 +58,55 @@ def kernel_times(L, ctx):
     return a.value, b.value
 
 
-def plain_rate(tess, paths, opts, make_buffers):
-    """Triangles/s of one tessellate_path call per path (host buffers in and out)."""
-    for i in range(min(5, len(paths))):
-        tess.tessellate_path(paths[i], opts[i], make_buffers()[1])
+def plain_rate(fn, ctx, paths, copts, vcap, icap):
+    """Triangles/s of the plain per-path drop-in: one C ABI call per path (lyg_stroke_path /
+    lyg_fill_path), host buffers in and out, preallocated once (as a caller reusing its
+    VertexBuffers would)."""
+    import numpy as np
+    from lyon_amd import _lib as B
+    from lyon_amd.tessellation import _path_view
+    v = np.empty((vcap, 2), np.float32)
+    ix = np.empty(icap, np.uint32)
+    views = [_path_view(p) for p in paths]
+
+    def one(k):
+        out = B.OutputC(v.ctypes.data_as(B.F32P), ix.ctypes.data_as(B.U32P), vcap, icap, 0, 0, 0)
+        st = fn(ctx, C.byref(views[k]), C.byref(copts[k]), C.byref(out))
+        if st != 0:
+            raise RuntimeError(f"plain call: status {st}")
+        return out.n_indices // 3
+
+    for k in range(min(5, len(paths))):
+        one(k)
     tris = 0
     t0 = time.perf_counter()
-    for i in range(len(paths)):
-        vb, bb = make_buffers()
-        tess.tessellate_path(paths[i], opts[i], bb)
-        tris += len(vb.indices) // 3
-        if i % 16 == 15:
-            log(f"  plain call {i + 1}/{len(paths)}")
+    for k in range(len(paths)):
+        tris += one(k)
+        if k % 64 == 63:
+            log(f"  plain call {k + 1}/{len(paths)}")
     return tris / (time.perf_counter() - t0)
+
+
+def median_time(fn, reps=5):
+    """Median wall time of reps calls after one warm-up call; returns (seconds, result)."""
+    r = fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], r
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -88,12 +126,19 @@ def main():
     ap.add_argument("--no-flatten", action="store_true")
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000,
+                    help="requests of the stream at the serving configuration")
+    ap.add_argument("--large-vertices", type=int, default=1 << 20)
+    ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--traffic-json",
                     default=os.path.join(ROOT, "profiles", "stroke_emit_batched_stroke_traffic.json"),
                     help="HBM traffic per emit launch measured with rocprofv3 counters")
     ap.add_argument("--profile", action="store_true",
                     help="only run the batched stroke steps and a 1 GiB device copy (the "
                          "reference kernel for the traffic counters), for rocprofv3")
+    ap.add_argument("--profile-large", action="store_true",
+                    help="only run the single large fill (banded) steps, for rocprofv3")
     ap.add_argument("--profile-fill", action="store_true",
                     help="only run the batched fill steps and a 1 GiB device copy, for rocprofv3")
     args = ap.parse_args()
@@ -128,6 +173,18 @@ def main():
             synchronize()
             log(f"profile step {k} done")
         print(json.dumps({"profile": "stroke batch + 1 GiB copy", "steps": args.steps}))
+        return
+
+    if args.profile_large:
+        from lyon_amd.path import PathBatch
+        walk = workloads.random_walk_path(args.large_vertices, seed=0)
+        wdb = DeviceBatch(PathBatch.from_paths([walk]), workloads.one_fill_options(0.1, 1))
+        lb = FillBatcher(banded=True)
+        for k in range(args.steps):
+            lb.run(wdb)
+            synchronize()
+            log(f"profile large fill step {k} done")
+        print(json.dumps({"profile": "single large fill (banded, NonZero)", "steps": args.steps}))
         return
 
     if args.profile_fill:
@@ -191,15 +248,12 @@ def main():
     ns = min(args.plain_sample, args.paths)
     sos = [StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
                          tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
-                         end_cap=int(o["end_cap"]), line_join=int(o["line_join"]))
+                         end_cap=int(o["end_cap"]), line_join=int(o["line_join"])).to_c()
            for o in opts[:ns]]
-
-    def buffers():
-        vb = VertexBuffers()
-        return vb, BuffersBuilder(vb)
-
-    plain_value = plain_rate(StrokeTessellator(), [batch.path(i) for i in range(ns)], sos,
-                             buffers)
+    vmax = int(np.max(np.diff(res.vertex_offsets.astype(np.int64))[:ns]))
+    imax = int(np.max(np.diff(res.index_offsets.astype(np.int64))[:ns]))
+    plain_value = plain_rate(L.lyg_stroke_path, ctx, [batch.path(i) for i in range(ns)], sos,
+                             vmax, imax)
 
     # ---- batched flattening (lyon_geom segments) -----------------------------------------
     flat = None
@@ -335,11 +389,13 @@ def main():
         synchronize()
         fms = (time.perf_counter() - t0) * 1e3 / args.fill_steps
         fns = min(args.fill_plain_sample, args.fill_paths)
-        fos = [FillOptions(tolerance=0.05, fill_rule=FillRule(int(o["fill_rule"])))
+        fos = [FillOptions(tolerance=0.05, fill_rule=FillRule(int(o["fill_rule"]))).to_c()
                for o in fopts[:fns]]
         log("fill: plain per-path drop-in")
-        fplain = plain_rate(FillTessellator(), [fbatch.path(i) for i in range(fns)], fos,
-                            buffers)
+        fvmax = int(np.max(np.diff(fres.vertex_offsets.astype(np.int64))[:fns]))
+        fimax = int(np.max(np.diff(fres.index_offsets.astype(np.int64))[:fns]))
+        fplain = plain_rate(L.lyg_fill_path, ctx, [fbatch.path(i) for i in range(fns)], fos,
+                            fvmax, fimax)
         fill = {"workload": f"batched fill: {args.fill_paths} closed paths, half "
                             "self-intersecting star polygons {n/k} (5-512 points), half loops "
                             "of 4-256 cubics, EvenOdd/NonZero 50/50, tol 0.05, coords in "
@@ -354,32 +410,130 @@ def main():
                                    "sample_paths": fns},
                 "batched_over_plain": (ftris / (fms / 1e3)) / fplain}
 
+    # ---- single large fill (banded) -----------------------------------------------------------
+    large = None
+    if not args.no_large:
+        from lyon_amd.path import PathBatch
+        log("single large fill: generating the random walk")
+        walk = workloads.random_walk_path(args.large_vertices, seed=0)
+        wbatch = PathBatch.from_paths([walk])
+        large = {"workload": f"single large fill: one closed random-walk polyline of "
+                             f"{args.large_vertices} vertices (N(0,1) steps, f32), rescaled to "
+                             "[0,8192]^2, tolerance 0.1, banded fill (coverage of lyon's fill)",
+                 "vertices": args.large_vertices}
+        wdb = DeviceBatch(wbatch, workloads.one_fill_options(0.1, 1))
+        lb = FillBatcher(banded=True)
+        for rule, rname in ((1, "nonzero"), (0, "evenodd")):
+            wdb = DeviceBatch(wbatch, workloads.one_fill_options(0.1, rule))
+            wres = lb.run(wdb)
+            synchronize()
+            prep, sweep = [], []
+            t0 = time.perf_counter()
+            for _ in range(args.large_steps):
+                lb.run(wdb)
+                a, b = kernel_times(L, ctx)
+                prep.append(a)
+                sweep.append(b)
+            synchronize()
+            lms = (time.perf_counter() - t0) * 1e3 / args.large_steps
+            ltris = wres.n_indices // 3
+            lbytes = len(wbatch.verbs) + 8 * len(wbatch.points) + 8 * wres.n_vertices + \
+                4 * wres.n_indices
+            lach = lbytes / (lms / 1e3) / 1e9
+            large[rname] = {"ms_per_step": lms, "triangles": ltris,
+                            "vertices": wres.n_vertices, "value": ltris / (lms / 1e3),
+                            "unit": "triangles/s",
+                            "kernel_ms": {"band_prep_avg": float(np.mean(prep)),
+                                          "sweep_avg": float(np.mean(sweep))},
+                            "roofline": {"bound": "hbm", "achieved": lach, "peak": HBM_PEAK_GBS,
+                                         "unit": "GB/s", "frac": lach / HBM_PEAK_GBS,
+                                         "scope": "whole banded step (all kernels and host "
+                                                  "syncs)", "algorithmic_bytes": lbytes}}
+            log(f"  {rname}: {lms:.2f} ms, {ltris} triangles")
+        if not args.no_cpu:
+            from oracle import pyoracle as O
+            sample_n = 1 << 16
+            sw = workloads.random_walk_path(sample_n, seed=0)
+            secs, (st, rv, ri) = median_time(lambda: O.fill(sw, 0.1, 1), reps=3)
+            large["cpu_baseline"] = {
+                "value": (len(ri) // 3) / secs, "unit": "triangles/s", "cores": 1,
+                "kind": "port", "sample": f"the restated lyon sweep on a {sample_n}-vertex walk "
+                "of the same generator, NonZero (median of 3 after a warm-up; the full walk "
+                "takes ~65 s per rule)", "seconds": secs}
+
+    # ---- plain workload (one closed 64-segment path) ------------------------------------------
+    plain = None
+    if not args.no_cpu:
+        from oracle import pyoracle as O
+        log("plain workload: GPU drop-in and oracle")
+        pp = workloads.plain_path(0)
+        sopt = StrokeOptions(line_width=4.0, miter_limit=4.0, line_join=0, start_cap=0,
+                             end_cap=0).to_c()
+        fopt = FillOptions(tolerance=0.1).to_c()
+        from lyon_amd.tessellation import _path_view
+        view = _path_view(pp)
+        pvb = np.empty((1 << 16, 2), np.float32)
+        pib = np.empty(1 << 18, np.uint32)
+
+        def one_call(fn, copt):
+            out = B.OutputC(pvb.ctypes.data_as(B.F32P), pib.ctypes.data_as(B.U32P), 1 << 16,
+                            1 << 18, 0, 0, 0)
+            st = fn(ctx, C.byref(view), C.byref(copt), C.byref(out))
+            if st != 0:
+                raise RuntimeError(f"plain workload: status {st}")
+            return out.n_indices // 3
+
+        gs, _ = median_time(lambda: one_call(L.lyg_stroke_path, sopt), reps=9)
+        gf, _ = median_time(lambda: one_call(L.lyg_fill_path, fopt), reps=9)
+        cs, (_, _, sri) = median_time(lambda: O.stroke(pp, 4.0, 4.0, 0.1, 0, 0, 0), reps=9)
+        cf, (_, _, fri) = median_time(lambda: O.fill(pp, 0.1, 0), reps=9)
+        plain = {"workload": "one closed path of 64 segments (25% line / 25% quadratic / 50% "
+                             "cubic, points U[0,1024]^2, seed 0): stroke width 4, Miter limit 4, "
+                             "Butt, tolerance 0.1; fill EvenOdd, tolerance 0.1",
+                 "stroke": {"triangles": len(sri) // 3, "gpu_ms": gs * 1e3, "cpu_ms": cs * 1e3},
+                 "fill": {"triangles": len(fri) // 3, "gpu_ms": gf * 1e3, "cpu_ms": cf * 1e3},
+                 "note": "GPU: one lyg_*_path call with host buffers (PCIe round trip); CPU: "
+                         "the oracle on one core; medians of 9 after a warm-up"}
+
     # ---- serving loop: Poisson request streams ---------------------------------------------
     serving = None
     if not args.no_serve:
         from lyon_amd.serve import Server
         serving = []
+        # (name, offered req/s, requests, stroke fraction, segments min/max, pool GiB,
+        #  fills as random walks routed to the banded fill from 2,048 edges)
         streams = [
-            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64),
-            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256),
-            ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256),
+            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64, 8, False),
+            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256, 8, False),
+            ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256, 8, False),
+            ("serving config: 60% stroke / 40% fill, log-uniform 4-65536 segments "
+             "(lines/quads/cubics 25/25/50, points uniform in [0,4096]^2, fills closed, "
+             "EvenOdd/NonZero), 32 GiB pool", 100000.0, args.serve_requests, 0.6, 4, 65536, 32,
+             1),
+            ("60% stroke / 40% fill (fills: closed random walks), log-uniform 4-65536 "
+             "segments, 32 GiB pool", 100000.0, args.serve_requests, 0.6, 4, 65536, 32, 2),
         ]
-        for name, rate, nreq, sfrac, smin, smax in streams:
+        for name, rate, nreq, sfrac, smin, smax, pool_gib, fill_kind in streams:
             log(f"serving stream: {name} at {rate:.0f} req/s")
-            with Server(pool_bytes=8 << 30, max_batch_paths=1024, max_inflight=4) as srv:
+            with Server(pool_bytes=pool_gib << 30, max_batch_paths=1024, max_inflight=4,
+                        fill_banded_min_edges=2048 if fill_kind else 0) as srv:
                 # warm-up stream (first launches on the workers' streams, their buffers
                 # growing to size), then the measured one on the same server
                 srv.run_stream(n_requests=max(nreq // 4, 100), rate_per_s=rate,
                                stroke_frac=sfrac, min_segs=smin, max_segs=smax,
-                               tolerance=0.05, n_distinct=256, seed=2)
+                               tolerance=0.05, n_distinct=256, seed=2, fill_kind=fill_kind)
                 r = srv.run_stream(n_requests=nreq, rate_per_s=rate, stroke_frac=sfrac,
                                    min_segs=smin, max_segs=smax, tolerance=0.05,
-                                   n_distinct=256, seed=1)
+                                   n_distinct=256, seed=1, fill_kind=fill_kind)
             serving.append({"stream": name, "requests": nreq, "warmup_requests": max(nreq // 4, 100),
+                            "pool_gib": pool_gib,
                             "offered_rate": rate, "completed_rate": r["completed_rate"],
+                            "failed": r["failed"], "rejected_at_admission": r["rejected"],
                             "p50_us": r["p50_us"], "p99_us": r["p99_us"], "max_us": r["max_us"],
                             "triangles_per_s": r["triangles_per_s"],
-                            "mean_batch": r["mean_batch"]})
+                            "mean_batch": r["mean_batch"], "seconds": r["seconds"]})
+            log(f"  completed {r['completed_rate']:.0f} req/s, p50 {r['p50_us'] / 1e3:.2f} ms, "
+                f"p99 {r['p99_us'] / 1e3:.2f} ms, failed {r['failed']} (rejected {r['rejected']})")
 
     # ---- CPU baseline ---------------------------------------------------------------------
     cpu = None
@@ -394,29 +548,25 @@ def main():
         for j, k in enumerate(("line_width", "miter_limit", "tolerance", "start_cap", "end_cap",
                                "line_join")):
             so[:, j] = opts[k][:m]
-        O.batch_triangles(sub, so, False, 1)
-        c0 = time.perf_counter()
-        ctris = O.batch_triangles(sub, so, False, 1)
-        c1 = time.perf_counter()
-        atris = O.batch_triangles(sub, so, False, ncores)
-        a1 = time.perf_counter()
-        cpu = {"value": ctris / (c1 - c0), "unit": "triangles/s", "cores": 1, "kind": "port",
-               "sample": f"first {m} of the {args.paths} batched-stroke paths, single thread",
-               "all_cores_value": atris / (a1 - c1), "all_cores": ncores}
+        s1, ctris = median_time(lambda: O.batch_triangles(sub, so, False, 1))
+        sa, atris = median_time(lambda: O.batch_triangles(sub, so, False, ncores))
+        cpu = {"value": ctris / s1, "unit": "triangles/s", "cores": 1, "kind": "port",
+               "sample": f"first {m} of the {args.paths} batched-stroke paths, single thread "
+                         "(median of 5 after a warm-up)",
+               "all_cores_value": atris / sa, "all_cores": ncores, "nproc": os.cpu_count(),
+               "cpu_model": cpu_model()}
         if fbatch is not None:
             fm = min(args.cpu_sample, args.fill_paths)
             fsub = PathBatch.from_paths([fbatch.path(i) for i in range(fm)])
             fo = np.zeros((fm, 2), np.float32)  # tolerance, fill rule per path
             fo[:, 0] = 0.05
             fo[:, 1] = fopts["fill_rule"][:fm]
-            f0 = time.perf_counter()
-            fct = O.batch_triangles(fsub, fo, True, 1)
-            f1 = time.perf_counter()
-            fat = O.batch_triangles(fsub, fo, True, ncores)
-            f2 = time.perf_counter()
-            cpu["fill"] = {"value": fct / (f1 - f0), "unit": "triangles/s", "cores": 1,
-                           "sample": f"first {fm} of the {args.fill_paths} batched-fill paths",
-                           "all_cores_value": fat / (f2 - f1), "all_cores": ncores}
+            f1s, fct = median_time(lambda: O.batch_triangles(fsub, fo, True, 1))
+            fas, fat = median_time(lambda: O.batch_triangles(fsub, fo, True, ncores))
+            cpu["fill"] = {"value": fct / f1s, "unit": "triangles/s", "cores": 1,
+                           "sample": f"first {fm} of the {args.fill_paths} batched-fill paths "
+                                     "(median of 5 after a warm-up)",
+                           "all_cores_value": fat / fas, "all_cores": ncores}
 
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
@@ -453,6 +603,8 @@ def main():
         "validators": validators,
         "ingest": ingest,
         "fill": fill,
+        "single_large_fill": large,
+        "plain_workload": plain,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -226,6 +226,36 @@ int32_t lyg_stroke_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_
 int32_t lyg_fill_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
                        uint32_t n_opts, lyg_batch_out* out);
 
+/* ---- banded fill (no lyon counterpart for the method) ---------------------------------
+ * The fill of FillTessellator::tessellate_path (fill.rs:628-641) computed for large paths
+ * across the whole GPU: the flattened edges (as lyon's event queue builder flattens them,
+ * event_queue.rs:661-900) are cut into horizontal bands swept independently into
+ * trapezoids.  The covered region is lyon's (EvenOdd / NonZero over the same edges); the
+ * triangles and vertices are not lyon's (vertices are not shared, no monotone spans), and
+ * sweep_orientation / handle_intersections do not change the result.  Errors:
+ * LYG_ERR_TOLERANCE_IS_NAN, LYG_ERR_POSITION_IS_NAN, LYG_ERR_INTERNAL (a band denser than
+ * 4096 edges), LYG_ERR_CAPACITY (sizes set).  Host buffers as lyg_fill_path. */
+int32_t lyg_fill_path_banded(lyg_ctx* ctx, const lyg_path_view* path,
+                             const lyg_fill_options* opts, lyg_output* out);
+/* Batched form over device buffers (as lyg_fill_batch). */
+int32_t lyg_fill_batch_banded(lyg_ctx* ctx, const lyg_batch_in* in,
+                              const lyg_fill_options* opts, uint32_t n_opts,
+                              lyg_batch_out* out);
+/* No lyon counterpart: how far each path's flattened curves stray from the true curves.
+ * For every chord of every flattened curve (the f32 points as the tessellator computes
+ * them: stroke_rule 0 the fill's flattening, event_queue.rs:758-900, curves turned down the
+ * sweep; 1 the stroker's, stroke.rs:2755-2779, quadratics with a sharp turn split first),
+ * the largest distance from the curve, evaluated in f64 from its f32 control points over
+ * the chord's parameter range, to the chord; path_error[i] (device, n_paths) gets
+ * path i's largest (0 for paths without curves).  tolerance: device, 1 or n_paths values.
+ * Lyon's Wang count keeps this within the tolerance up to the truncation of its last
+ * segment (quadratic_bezier.rs:375-390). */
+int32_t lyg_flattening_errors(lyg_ctx* ctx, const lyg_batch_in* in, const float* tolerance,
+                              uint32_t n_tol, int stroke_rule, float* path_error);
+/* Band size of the banded fill: about edges_per_band edge starts per band (default 256),
+ * at most max_bands bands per path (default 16384). */
+int32_t lyg_ctx_set_band_size(lyg_ctx* ctx, uint32_t edges_per_band, uint32_t max_bands);
+
 /* ---- lyon_algorithms path validators ---------------------------------------------------
  * path_winding_number_at_position / hit_test_path (crates/algorithms/src/hit_test.rs:8-83)
  * and approximate_signed_area (crates/algorithms/src/area.rs:7-19), bit-exact with lyon's
@@ -350,6 +380,11 @@ typedef struct lyg_server_config {
     int32_t device;
     uint32_t stroke_sizing;    /* 0: one staged emit pass (as lyg_stroke_batch), 1: count pass
                                   then emit (lyg_ctx_set_stroke_staging(ctx, 0, ..)) */
+    uint32_t fill_banded_min_edges; /* fills flattening to at least this many edges run through
+                                       the banded fill (lyon's coverage, other triangles); 0:
+                                       never */
+    uint64_t max_request_bytes; /* a request whose worst-case output exceeds this completes at
+                                   once with LYG_ERR_NO_SPACE (0: a quarter of the pool) */
 } lyg_server_config;
 typedef struct lyg_server_result {
     int32_t status;            /* the request's lyg_status */
@@ -357,12 +392,15 @@ typedef struct lyg_server_result {
     const uint32_t* indices;   /* device pointer */
     uint32_t n_vertices, n_indices;
     double latency_us;         /* submit to completion */
+    int32_t banded;            /* 1: a fill that ran through the banded fill (lyon's
+                                  coverage, its own triangles); 0: lyon's triangles */
 } lyg_server_result;
 typedef struct lyg_server_stats {
     uint64_t completed, triangles, vertices, batches, pending, pool_high_water;
     double mean_batch;         /* requests per batch */
     double p50_us, p99_us, max_us;  /* submit-to-completion latency over completed requests */
     double elapsed_s;          /* first submit to last completion */
+    uint64_t rejected;         /* requests refused at admission (worst case above the limit) */
 } lyg_server_stats;
 lyg_server* lyg_server_create(const lyg_server_config* cfg);
 void lyg_server_destroy(lyg_server* server);
@@ -390,11 +428,24 @@ typedef struct lyg_stream_stats {
     uint64_t requests, triangles, batches, pool_high_water;
     double offered_rate, submit_rate, completed_rate;  /* requests/s */
     double triangles_per_s, p50_us, p99_us, max_us, mean_batch, seconds;
+    uint64_t failed;           /* requests that completed with an error status (rejected
+                                  ones included) */
+    uint64_t rejected;         /* of which refused at admission */
 } lyg_stream_stats;
 int32_t lyg_server_run_stream(lyg_server* server, uint64_t seed, uint32_t n_requests,
                               double rate_per_s, float stroke_frac, uint32_t min_segs,
                               uint32_t max_segs, float tolerance, uint32_t n_distinct,
                               lyg_stream_stats* out);
+/* The same with a choice of fill requests: fill_kind 0 = the batched fill workload's stars
+ * and cubic loops (as lyg_server_run_stream); 1 = closed paths from the stroke requests'
+ * generator (log-uniform segment counts in [min_segs, max_segs], lines / quadratics /
+ * cubics 25/25/50, points uniform in [0, 4096]^2); 2 = closed random walks (the single
+ * large fill's generator) of a log-uniform edge count in [min_segs, max_segs] rescaled
+ * onto [0, 4096]^2. */
+int32_t lyg_server_run_stream_mixed(lyg_server* server, uint64_t seed, uint32_t n_requests,
+                                    double rate_per_s, float stroke_frac, uint32_t min_segs,
+                                    uint32_t max_segs, float tolerance, uint32_t n_distinct,
+                                    int fill_kind, lyg_stream_stats* out);
 
 /* ---- synthetic inputs ----------------------------------------------------------------
  * Seeded generators for the benchmark workloads (SplitMix64 -> PCG32).  Host memory.
@@ -416,6 +467,11 @@ int32_t lyg_gen_stroke_options(uint64_t seed, uint32_t n_paths, float tolerance,
 int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, float* points,
                            uint32_t* verb_offsets, uint32_t* point_offsets, uint64_t* n_verbs,
                            uint64_t* n_points);
+/* The single large fill workload: a closed polyline of n vertices from a seeded 2D random
+ * walk (steps N(0,1) per axis, Box-Muller on PCG32, f32), rescaled per axis to
+ * [0, extent]^2.  verbs: n + 1 entries (Begin, Lines, Close), points: n + 1 pairs. */
+int32_t lyg_gen_random_walk(uint64_t seed, uint32_t n, float extent, uint8_t* verbs,
+                            float* points);
 /* Per-path FillOptions of the batched fill workload: EvenOdd/NonZero 50/50, vertical
  * sweep, intersections handled. */
 int32_t lyg_gen_fill_options(uint64_t seed, uint32_t n_paths, float tolerance,

@@ -57,13 +57,14 @@ class BatchOutC(C.Structure):
 class ServerConfigC(C.Structure):
     _fields_ = [("pool_bytes", C.c_uint64), ("max_batch_paths", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("linger_us", C.c_uint32), ("device", C.c_int32),
-                ("stroke_sizing", C.c_uint32)]
+                ("stroke_sizing", C.c_uint32), ("fill_banded_min_edges", C.c_uint32),
+                ("max_request_bytes", C.c_uint64)]
 
 
 class ServerResultC(C.Structure):
     _fields_ = [("status", C.c_int32), ("vertices", C.c_void_p), ("indices", C.c_void_p),
                 ("n_vertices", C.c_uint32), ("n_indices", C.c_uint32),
-                ("latency_us", C.c_double)]
+                ("latency_us", C.c_double), ("banded", C.c_int32)]
 
 
 class ServerStatsC(C.Structure):
@@ -71,7 +72,7 @@ class ServerStatsC(C.Structure):
                 ("batches", C.c_uint64), ("pending", C.c_uint64),
                 ("pool_high_water", C.c_uint64), ("mean_batch", C.c_double),
                 ("p50_us", C.c_double), ("p99_us", C.c_double), ("max_us", C.c_double),
-                ("elapsed_s", C.c_double)]
+                ("elapsed_s", C.c_double), ("rejected", C.c_uint64)]
 
 
 class StreamStatsC(C.Structure):
@@ -79,7 +80,8 @@ class StreamStatsC(C.Structure):
                 ("pool_high_water", C.c_uint64), ("offered_rate", C.c_double),
                 ("submit_rate", C.c_double), ("completed_rate", C.c_double),
                 ("triangles_per_s", C.c_double), ("p50_us", C.c_double), ("p99_us", C.c_double),
-                ("max_us", C.c_double), ("mean_batch", C.c_double), ("seconds", C.c_double)]
+                ("max_us", C.c_double), ("mean_batch", C.c_double), ("seconds", C.c_double),
+                ("failed", C.c_uint64), ("rejected", C.c_uint64)]
 
 
 class TuningC(C.Structure):
@@ -139,6 +141,14 @@ SIGNATURES = {
                           C.POINTER(BatchOutC)], C.c_int32),
     "lyg_fill_batch": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                         C.POINTER(BatchOutC)], C.c_int32),
+    "lyg_fill_batch_banded": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                               C.POINTER(BatchOutC)], C.c_int32),
+    "lyg_fill_path_banded": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),
+                              C.POINTER(OutputC)], C.c_int32),
+    "lyg_ctx_set_band_size": ([C.c_void_p, C.c_uint32, C.c_uint32], C.c_int32),
+    "lyg_flattening_errors": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32, C.c_int,
+                               C.c_void_p], C.c_int32),
+    "lyg_gen_random_walk": ([C.c_uint64, C.c_uint32, C.c_float, U8P, F32P], C.c_int32),
     "lyg_stroke_batch_count": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                                 C.c_void_p, C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_gen_stroke_batch": ([C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_int,
@@ -168,6 +178,9 @@ SIGNATURES = {
     "lyg_server_run_stream": ([C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_float,
                                C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
                                C.POINTER(StreamStatsC)], C.c_int32),
+    "lyg_server_run_stream_mixed": ([C.c_void_p, C.c_uint64, C.c_uint32, C.c_double, C.c_float,
+                                     C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.c_int,
+                                     C.POINTER(StreamStatsC)], C.c_int32),
     "lyg_gen_fill_batch": ([C.c_uint64, C.c_uint32, U8P, F32P, U32P, U32P, U64P, U64P],
                            C.c_int32),
     "lyg_gen_fill_options": ([C.c_uint64, C.c_uint32, C.c_float, C.POINTER(FillOptionsC)],

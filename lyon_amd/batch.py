@@ -197,9 +197,36 @@ class StrokeBatcher(_Batcher):
 
 
 class FillBatcher(_Batcher):
-    """Batched fill; with check=False a path's failure is reported in ``status`` only."""
+    """Batched fill; with check=False a path's failure is reported in ``status`` only.
+    ``banded=True``: every path through the banded fill (lyg_fill_batch_banded)."""
 
     _entry = "lyg_fill_batch"
+
+    def __init__(self, banded: bool = False):
+        super().__init__()
+        if banded:
+            self._entry = "lyg_fill_batch_banded"
+
+
+def flattening_errors(db: DeviceBatch, tolerance, stroke_rule: bool) -> np.ndarray:
+    """lyg_flattening_errors: per path, the largest distance (f64) between its flattened
+    curves' chords and the true curves, under the stroker's or the fill's flattening."""
+    tol = np.atleast_1d(np.asarray(tolerance, np.float32))
+    dt = DeviceBuffer.from_numpy(tol)
+    out = DeviceBuffer(4 * max(db.n, 1))
+    cin = db.c_in()
+    st = B.lib().lyg_flattening_errors(_Ctx.get(), C.byref(cin), dt.ptr, len(tol),
+                                       1 if stroke_rule else 0, out.ptr)
+    if st != 0:
+        raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+    return out.download(np.float32, db.n)
+
+
+def set_band_size(edges_per_band: int = 256, max_bands: int = 16384):
+    """Band size of the banded fill (lyg_ctx_set_band_size)."""
+    st = B.lib().lyg_ctx_set_band_size(_Ctx.get(), int(edges_per_band), int(max_bands))
+    if st != 0:
+        raise TessellationError(st)
 
 
 class StrokeGraph:

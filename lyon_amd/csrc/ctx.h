@@ -15,6 +15,8 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+struct BandBufs;  // the banded fill's buffers (band_api.cpp)
+
 struct lyg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -46,7 +48,13 @@ struct lyg_ctx {
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
     hipEvent_t class_ev[9] = {};
+    // banded fill: band size (edge starts per band) and the most bands of one path
+    BandBufs* band = nullptr;
+    uint32_t band_edges_per_band = 256;
+    uint32_t band_max_bands = 16384;
 };
+
+void lyg_band_free(lyg_ctx* c);
 
 inline void free_buf(DevBuf& b) {
     if (b.p) (void)hipFree(b.p);

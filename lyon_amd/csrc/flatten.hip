@@ -93,16 +93,7 @@ __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_runs_kernel(Flatt
     if (base >= total) return;  // wave-uniform
     const uint64_t wend = base + WAVE * FR_RUN < total ? base + WAVE * FR_RUN : total;
     // segment of the wave's first point: last segment whose offset <= base
-    uint32_t cur = 0;
-    {
-        uint32_t lo = 0, hi = L.n - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) >> 1;
-            if (L.offsets[mid] <= base) lo = mid;
-            else hi = mid - 1;
-        }
-        cur = lo;
-    }
+    const uint32_t cur = wave_search_last_le(L.offsets, L.n, base);
     // offsets of segments cur .. until one starts at or beyond the wave's end (L.offsets
     // has n + 1 entries; the last is the total)
     uint32_t nwin = 0;

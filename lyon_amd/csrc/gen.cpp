@@ -165,6 +165,58 @@ void fill_path(uint64_t seed, uint64_t i, std::vector<uint8_t>& verbs,
 
 }  // namespace lygen
 
+namespace lygen {
+// Random walk of n vertices, stream `stream` of `seed` (see lyg_gen_random_walk).
+void random_walk(uint64_t seed, uint64_t stream, uint32_t n, float extent, uint8_t* verbs,
+                 float* points) {
+    const double two_pi = 6.283185307179586;
+    Pcg32 rng(path_seed(seed ^ 0x7a1c3a1cull, stream));
+    float x = 0.0f, y = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        points[2 * i] = x;
+        points[2 * i + 1] = y;
+        const double u1 = ((double)(rng.next() >> 8) + 1.0) * (1.0 / 16777216.0);  // (0, 1]
+        const double u2 = (double)(rng.next() >> 8) * (1.0 / 16777216.0);
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        x += (float)(r * std::cos(two_pi * u2));
+        y += (float)(r * std::sin(two_pi * u2));
+    }
+    float mn[2] = {points[0], points[1]}, mx[2] = {points[0], points[1]};
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a) {
+            mn[a] = std::fmin(mn[a], points[2 * i + a]);
+            mx[a] = std::fmax(mx[a], points[2 * i + a]);
+        }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 2; ++a) {
+            const double span = (double)mx[a] - (double)mn[a];
+            const double t = span > 0 ? ((double)points[2 * i + a] - (double)mn[a]) / span : 0.0;
+            points[2 * i + a] = (float)(t * (double)extent);
+        }
+    verbs[0] = LYG_VERB_BEGIN;
+    for (uint32_t i = 1; i < n; ++i) verbs[i] = LYG_VERB_LINE;
+    verbs[n] = LYG_VERB_CLOSE;
+    points[2 * n] = points[0];
+    points[2 * n + 1] = points[1];
+}
+
+// Fill request i of the serving stream: a closed random walk whose edge count is drawn
+// log-uniformly from [min_segs, max_segs] (as stroke_path draws its segment count),
+// appended to `verbs` / `points`.
+void walk_path(uint64_t seed, uint64_t i, uint32_t min_segs, uint32_t max_segs, float extent,
+               std::vector<uint8_t>& verbs, std::vector<float>& points) {
+    Pcg32 rng(path_seed(seed ^ 0x3a1c3a1cull, i));
+    const double lmin = std::log((double)min_segs), lmax = std::log((double)max_segs);
+    uint32_t segs = (uint32_t)std::round(std::exp(lmin + rng.uniform() * (lmax - lmin)));
+    segs = segs < min_segs ? min_segs : (segs > max_segs ? max_segs : segs);
+    if (segs < 3) segs = 3;
+    const size_t v0 = verbs.size(), p0 = points.size();
+    verbs.resize(v0 + segs + 1);
+    points.resize(p0 + 2 * (segs + 1));
+    random_walk(seed, i + 1, segs, extent, verbs.data() + v0, points.data() + p0);
+}
+}  // namespace lygen
+
 extern "C" {
 
 // Open (or closed) paths with a log-uniform segment count in [min_segs, max_segs],
@@ -249,6 +301,18 @@ int32_t lyg_gen_fill_batch(uint64_t seed, uint32_t n_paths, uint8_t* verbs, floa
     if (point_offsets) point_offsets[n_paths] = (uint32_t)np;
     if (n_verbs) *n_verbs = nv;
     if (n_points) *n_points = np;
+    return LYG_OK;
+}
+
+// The single large fill workload: one closed polyline of n vertices from a 2D random walk
+// (each step N(0, 1) per axis by Box-Muller on PCG32, accumulated in f32), then rescaled
+// per axis (f64, rounded to f32) onto [0, extent]^2.  Begin, n - 1 Lines and Close (with
+// the stored copy of the first point): n + 1 verbs, n + 1 points.
+
+int32_t lyg_gen_random_walk(uint64_t seed, uint32_t n, float extent, uint8_t* verbs,
+                            float* points) {
+    if (n < 2 || !verbs || !points) return LYG_ERR_INVALID_ARGUMENT;
+    lygen::random_walk(seed, 0, n, extent, verbs, points);
     return LYG_OK;
 }
 

@@ -28,6 +28,9 @@ struct BatchSpec {
 int32_t stroke_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
                      uint64_t* ti);
 // Batched fill; `status` gets every path's status (a failed path produces nothing).
+// Banded fill of a batch (band_api.cpp): lyon's fill coverage, its own triangles.
+int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
+                          uint64_t* ti, std::vector<uint32_t>* status);
 int32_t fill_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
                    uint64_t* ti, std::vector<uint32_t>* status);
 

@@ -355,6 +355,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
                       &c->st_fallback})
         free_buf(*b);
     free_buf(c->f_boxes);
+    lyg_band_free(c);
     for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
         for (DevBuf& b : *v) free_buf(b);
     if (c->d_counters) (void)hipFree(c->d_counters);

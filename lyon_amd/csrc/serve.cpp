@@ -107,6 +107,7 @@ struct Request {
     lyg_stroke_options sopts{};
     uint64_t reserve = 0;        // worst-case output bytes (admission)
     uint32_t size_class = 0;     // log4 of the flattened-point bound: batches mix one class
+    bool banded = false;         // a fill large enough for the banded fill
     Clock::time_point t_submit, t_done;
     State state = S_QUEUED;
     int32_t status = LYG_OK;
@@ -153,11 +154,27 @@ uint64_t flattened_bound(const std::vector<uint8_t>& verbs, const std::vector<fl
     return f;
 }
 
+// Crossings between the path's segments, bounded by Bezout: curves of degrees m and n
+// cross at most m n times, and a cubic crosses itself at most once.  Their flattened
+// polylines follow the curves within the tolerance, so this is the admission estimate of
+// the intersections a fill can add.
+double crossing_bound(const std::vector<uint8_t>& verbs) {
+    double s1 = 0, s2 = 0, self = 0;
+    for (uint8_t v : verbs) {
+        const double d = v == LYG_VERB_QUADRATIC ? 2.0 : v == LYG_VERB_CUBIC ? 3.0 :
+                         (v == LYG_VERB_LINE || v == LYG_VERB_CLOSE) ? 1.0 : 0.0;
+        s1 += d;
+        s2 += d * d;
+        if (v == LYG_VERB_CUBIC) self += 1.0;
+    }
+    return 0.5 * (s1 * s1 - s2) + self;
+}
+
 // Worst-case output bytes of a request, for admission.  Stroke: per flattened point at
 // most 4 join vertices plus a round join's arc (2^k vertices for an arc of up to pi at the
-// circle flattening step of the half width) and as many triangles plus 2.  Fill: two
-// vertices and four triangles per event of the path without intersections (a path whose
-// intersections outgrow it still runs; its exact output is allocated when known).
+// circle flattening step of the half width) and as many triangles plus 2.  Fill: per event
+// of the path (edge ends and crossings) two vertices and four triangles of the sweep, or
+// the banded fill's three trapezoids of four vertices and two triangles.
 uint64_t worst_case_bytes(const Request& r) {
     if (r.kind == K_STROKE) {
         const float tol = r.sopts.tolerance > 0 ? r.sopts.tolerance : 0.1f;
@@ -175,8 +192,11 @@ uint64_t worst_case_bytes(const Request& r) {
         return (uint64_t)(8.0 * nv + 12.0 * nt);
     }
     const float tol = r.fopts.tolerance > 0 ? r.fopts.tolerance : 0.1f;
-    const uint64_t e = 2 * flattened_bound(r.verbs, r.points, tol) + 16;
-    return 8 * (2 * e + 256) + 12 * (4 * e + 512);
+    const double e = 2.0 * (double)flattened_bound(r.verbs, r.points, tol) + 16.0 +
+                     2.0 * crossing_bound(r.verbs);
+    const double per_event = r.banded ? 3.0 * (4.0 * 8.0 + 2.0 * 12.0) : 2.0 * 8.0 + 4.0 * 12.0;
+    const double b = per_event * e + 8.0 * 256 + 12.0 * 512;
+    return b < 1.8e19 ? (uint64_t)b : UINT64_MAX;
 }
 
 }  // namespace
@@ -191,9 +211,24 @@ struct lyg_server {
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
     bool stop = false;
-    // stats
-    std::vector<double> latencies_us;
+    // stats: the latencies of the last LAT_WINDOW completed requests, as a ring
+    static constexpr uint64_t LAT_WINDOW = 1ull << 20;
+    std::vector<double> lat_ring;
+    uint64_t lat_count = 0;
+    void record_latency(double us) {
+        if (lat_ring.size() < LAT_WINDOW) lat_ring.push_back(us);
+        else lat_ring[lat_count % LAT_WINDOW] = us;
+        ++lat_count;
+    }
+    // The latencies recorded after the first `since` ones (at most the last LAT_WINDOW).
+    std::vector<double> latencies_since(uint64_t since) const {
+        const uint64_t n = std::min<uint64_t>(lat_count - std::min(since, lat_count), lat_ring.size());
+        std::vector<double> l(n);
+        for (uint64_t k = 0; k < n; ++k) l[k] = lat_ring[(lat_count - n + k) % LAT_WINDOW];
+        return l;
+    }
     uint64_t completed = 0, triangles = 0, vertices = 0, batches = 0, batched_requests = 0;
+    uint64_t rejected = 0;
     Clock::time_point t_first_submit{}, t_last_done{};
     bool any_submit = false;
 
@@ -249,7 +284,7 @@ void lyg_server::run_dispatcher() {
         std::vector<Request*> batch;
         for (auto it = pending.begin(); it != pending.end() && batch.size() < cfg.max_batch_paths;) {
             Request* r = *it;
-            if (r->kind != kind || r->size_class != cls) {
+            if (r->kind != kind || r->size_class != cls || r->banded != pending.front()->banded) {
                 ++it;
                 continue;
             }
@@ -352,6 +387,7 @@ void lyg_server::run_batch(Worker* w) {
     std::vector<uint32_t> status;
     if (st == LYG_OK) {
         if (kind == K_STROKE) st = lygi::stroke_batch(c, b, alloc, &tv, &ti);
+        else if (B[0]->banded) st = lygi::fill_batch_banded(c, b, alloc, &tv, &ti, &status);
         else st = lygi::fill_batch(c, b, alloc, &tv, &ti, &status);
     }
     std::vector<uint32_t> offs(2 * (n + 1), 0);
@@ -381,7 +417,7 @@ void lyg_server::run_batch(Worker* w) {
             triangles += r->ni / 3;
             vertices += r->nv;
         }
-        latencies_us.push_back(std::chrono::duration<double, std::micro>(done - r->t_submit).count());
+        record_latency(std::chrono::duration<double, std::micro>(done - r->t_submit).count());
         completed++;
     }
     if (st != LYG_OK && out_off != SIZE_MAX) {
@@ -408,12 +444,16 @@ int32_t submit(lyg_server* s, Kind kind, const lyg_path_view* path, const void* 
     r->points.assign(path->points, path->points + 2ull * path->n_points);
     if (kind == K_FILL) r->fopts = *static_cast<const lyg_fill_options*>(opts);
     else r->sopts = *static_cast<const lyg_stroke_options*>(opts);
-    r->reserve = worst_case_bytes(*r);
     {
         const float tol = kind == K_FILL ? r->fopts.tolerance : r->sopts.tolerance;
         const uint64_t f = flattened_bound(r->verbs, r->points, tol > 0 ? tol : 0.1f);
         r->size_class = (uint32_t)(f > 1 ? (63 - __builtin_clzll(f)) / 2 : 0);
+        r->banded = kind == K_FILL && s->cfg.fill_banded_min_edges != 0 &&
+                    f >= s->cfg.fill_banded_min_edges;
     }
+    r->reserve = worst_case_bytes(*r);
+    const uint64_t limit = s->cfg.max_request_bytes ? s->cfg.max_request_bytes
+                                                    : s->cfg.pool_bytes / 4;
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return LYG_ERR_INVALID_ARGUMENT;
@@ -422,6 +462,17 @@ int32_t submit(lyg_server* s, Kind kind, const lyg_path_view* path, const void* 
     if (!s->any_submit) {
         s->any_submit = true;
         s->t_first_submit = r->t_submit;
+    }
+    if (r->reserve > limit) {
+        // refused at admission: complete at once, nothing reserved or allocated
+        r->reserve = 0;
+        r->status = LYG_ERR_NO_SPACE;
+        r->state = S_DONE;
+        r->t_done = r->t_submit;
+        s->rejected++;
+        s->reqs.emplace(r->ticket, std::move(r));
+        s->cv_done.notify_all();
+        return LYG_OK;
     }
     s->pending.push_back(r.get());
     s->reqs.emplace(r->ticket, std::move(r));
@@ -436,6 +487,7 @@ void fill_result(lyg_server* s, const Request& r, lyg_server_result* out) {
     out->n_vertices = r.status == LYG_OK ? r.nv : 0;
     out->n_indices = r.status == LYG_OK ? r.ni : 0;
     out->latency_us = std::chrono::duration<double, std::micro>(r.t_done - r.t_submit).count();
+    out->banded = r.banded ? 1 : 0;
 }
 
 }  // namespace
@@ -557,30 +609,39 @@ int32_t lyg_server_release(lyg_server* s, uint64_t ticket) {
     return LYG_OK;
 }
 
+// p50 / p99 / max of `l` (reordered), nearest rank.
+static void latency_summary(std::vector<double>& l, double* p50, double* p99, double* mx) {
+    *p50 = *p99 = *mx = 0;
+    if (l.empty()) return;
+    auto pct = [&](double q) {
+        size_t k = (size_t)std::min<double>((double)l.size() - 1, std::floor(q * (double)(l.size() - 1) + 0.5));
+        std::nth_element(l.begin(), l.begin() + k, l.end());
+        return l[k];
+    };
+    *p50 = pct(0.50);
+    *p99 = pct(0.99);
+    *mx = *std::max_element(l.begin(), l.end());
+}
+
 int32_t lyg_server_get_stats(lyg_server* s, lyg_server_stats* out) {
     if (!s || !out) return LYG_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(s->mu);
-    std::memset(out, 0, sizeof(*out));
-    out->completed = s->completed;
-    out->triangles = s->triangles;
-    out->vertices = s->vertices;
-    out->batches = s->batches;
-    out->mean_batch = s->batches ? (double)s->batched_requests / (double)s->batches : 0.0;
-    out->pool_high_water = s->pool.high_water();
-    out->pending = s->pending.size();
-    if (!s->latencies_us.empty()) {
-        std::vector<double> l = s->latencies_us;
-        auto pct = [&](double q) {
-            size_t k = (size_t)std::min<double>((double)l.size() - 1, std::floor(q * (double)(l.size() - 1) + 0.5));
-            std::nth_element(l.begin(), l.begin() + k, l.end());
-            return l[k];
-        };
-        out->p50_us = pct(0.50);
-        out->p99_us = pct(0.99);
-        out->max_us = *std::max_element(l.begin(), l.end());
+    std::vector<double> l;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        std::memset(out, 0, sizeof(*out));
+        out->completed = s->completed;
+        out->triangles = s->triangles;
+        out->vertices = s->vertices;
+        out->batches = s->batches;
+        out->mean_batch = s->batches ? (double)s->batched_requests / (double)s->batches : 0.0;
+        out->pool_high_water = s->pool.high_water();
+        out->pending = s->pending.size();
+        out->rejected = s->rejected;
+        if (s->any_submit && s->completed)
+            out->elapsed_s = std::chrono::duration<double>(s->t_last_done - s->t_first_submit).count();
+        l = s->latencies_since(0);
     }
-    if (s->any_submit && s->completed)
-        out->elapsed_s = std::chrono::duration<double>(s->t_last_done - s->t_first_submit).count();
+    latency_summary(l, &out->p50_us, &out->p99_us, &out->max_us);
     return LYG_OK;
 }
 
@@ -592,6 +653,8 @@ void stroke_path(uint64_t seed, uint64_t i, uint32_t min_segs, uint32_t max_segs
                  float coord_max, bool closed, std::vector<uint8_t>& verbs,
                  std::vector<float>& points);
 void fill_path(uint64_t seed, uint64_t i, std::vector<uint8_t>& verbs, std::vector<float>& pts);
+void walk_path(uint64_t seed, uint64_t i, uint32_t min_segs, uint32_t max_segs, float extent,
+               std::vector<uint8_t>& verbs, std::vector<float>& points);
 }  // namespace lygen
 
 extern "C" {
@@ -608,6 +671,14 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
                               double rate_per_s, float stroke_frac, uint32_t min_segs,
                               uint32_t max_segs, float tolerance, uint32_t n_distinct,
                               lyg_stream_stats* out) {
+    return lyg_server_run_stream_mixed(s, seed, n_requests, rate_per_s, stroke_frac, min_segs,
+                                       max_segs, tolerance, n_distinct, 0, out);
+}
+
+int32_t lyg_server_run_stream_mixed(lyg_server* s, uint64_t seed, uint32_t n_requests,
+                                    double rate_per_s, float stroke_frac, uint32_t min_segs,
+                                    uint32_t max_segs, float tolerance, uint32_t n_distinct,
+                                    int fill_kind, lyg_stream_stats* out) {
     if (!s || !out || n_requests == 0 || !(rate_per_s > 0) || n_distinct == 0 || min_segs < 1 ||
         max_segs < min_segs)
         return LYG_ERR_INVALID_ARGUMENT;
@@ -621,7 +692,13 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
     for (uint32_t i = 0; i < n_distinct; ++i) {
         lygen::stroke_path(seed ^ 0x5e7e5e7eull, i, min_segs, max_segs, 4096.0f, false,
                            strokes[i].v, strokes[i].p);
-        lygen::fill_path(seed, i, fills[i].v, fills[i].p);
+        if (fill_kind == 1)
+            lygen::stroke_path(seed ^ 0xf111f111ull, i, min_segs, max_segs, 4096.0f, true,
+                               fills[i].v, fills[i].p);
+        else if (fill_kind == 2)
+            lygen::walk_path(seed, i, min_segs, max_segs, 4096.0f, fills[i].v, fills[i].p);
+        else
+            lygen::fill_path(seed, i, fills[i].v, fills[i].p);
     }
     lyg_gen_stroke_options(seed, n_distinct, tolerance, 0.5f, 8.0f, sopts.data());
     lyg_gen_fill_options(seed, n_distinct, tolerance, fopts.data());
@@ -645,13 +722,15 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
     }
     std::vector<uint64_t> tickets(n_requests, 0);
     std::atomic<uint32_t> submitted{0};
+    std::atomic<uint64_t> failed{0};
     std::atomic<int32_t> err{LYG_OK};
     lyg_server_stats before{};
     lyg_server_get_stats(s, &before);
-    size_t lat0, batched0;
+    uint64_t lat0;
+    size_t batched0;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        lat0 = s->latencies_us.size();
+        lat0 = s->lat_count;
         batched0 = s->batched_requests;
     }
     const Clock::time_point t0 = Clock::now();
@@ -662,6 +741,7 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
             if (err.load() != LYG_OK) return;
             lyg_server_result r;
             lyg_server_wait(s, tickets[i], &r);
+            if (r.status != LYG_OK) failed.fetch_add(1);
             lyg_server_release(s, tickets[i]);
         }
     });
@@ -694,25 +774,19 @@ int32_t lyg_server_run_stream(lyg_server* s, uint64_t seed, uint32_t n_requests,
     out->triangles_per_s = (double)out->triangles / total_s;
     // latencies and batch sizes of this stream's requests only (a server that ran a warm-up
     // stream before reports its steady state here)
+    std::vector<double> l;
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        std::vector<double> l(s->latencies_us.begin() + (std::ptrdiff_t)lat0, s->latencies_us.end());
-        if (!l.empty()) {
-            auto pct = [&](double q) {
-                size_t k = (size_t)std::min<double>((double)l.size() - 1, std::floor(q * (double)(l.size() - 1) + 0.5));
-                std::nth_element(l.begin(), l.begin() + k, l.end());
-                return l[k];
-            };
-            out->p50_us = pct(0.50);
-            out->p99_us = pct(0.99);
-            out->max_us = *std::max_element(l.begin(), l.end());
-        }
+        l = s->latencies_since(lat0);
         const uint64_t nb = after.batches - before.batches;
         out->mean_batch = nb ? (double)(s->batched_requests - batched0) / (double)nb : 0.0;
     }
+    latency_summary(l, &out->p50_us, &out->p99_us, &out->max_us);
     out->batches = after.batches - before.batches;
     out->seconds = total_s;
     out->pool_high_water = after.pool_high_water;
+    out->failed = failed.load();
+    out->rejected = after.rejected - before.rejected;
     return LYG_OK;
 }
 

@@ -91,6 +91,26 @@ __device__ __forceinline__ uint32_t incl_max_u32(uint32_t v) {
     return x;
 }
 __device__ __forceinline__ int highest_bit(uint64_t m) { return m ? 63 - __builtin_clzll(m) : -1; }
+// Last index i in [0, n) with a[i] <= key, for a non-decreasing `a` with a[0] <= key (0
+// when n == 0), searched by the whole wave: each round probes 64 evenly spaced entries at
+// once, so ceil(log64(n)) dependent loads instead of log2(n).  Wave-uniform arguments.
+__device__ __forceinline__ uint32_t wave_search_last_le(const uint32_t* a, uint32_t n,
+                                                        uint64_t key) {
+    uint32_t lo = 0, hi = n ? n - 1 : 0;
+    while (hi > lo) {
+        const uint32_t span = hi - lo + 1;
+        const uint32_t step = (span + 63u) / 64u;
+        const uint32_t idx = lo + (uint32_t)__lane_id() * step;
+        const bool ok = idx <= hi && (uint64_t)a[idx] <= key;
+        const int k = 63 - __builtin_clzll(__ballot(ok) | 1ull);  // lane 0 probes lo: true
+        const uint32_t nlo = lo + (uint32_t)k * step;
+        const uint32_t nhi_c = nlo + step - 1u;
+        lo = nlo;
+        hi = nhi_c < hi ? nhi_c : hi;
+        if (step == 1u) break;
+    }
+    return lo;
+}
 __device__ __forceinline__ uint64_t mask_lt(int l) { return l >= 64 ? ~0ull : ((1ull << l) - 1ull); }
 __device__ __forceinline__ uint64_t mask_le(int l) { return l >= 63 ? ~0ull : ((1ull << (l + 1)) - 1ull); }
 __device__ __forceinline__ void wave_sync() {

@@ -17,9 +17,9 @@ LYG_PENDING = 21
 
 class Server:
     def __init__(self, pool_bytes=1 << 30, max_batch_paths=1024, max_inflight=4, linger_us=0,
-                 device=0, stroke_sizing=0):
+                 device=0, stroke_sizing=0, fill_banded_min_edges=0, max_request_bytes=0):
         cfg = B.ServerConfigC(pool_bytes, max_batch_paths, max_inflight, linger_us, device,
-                              stroke_sizing)
+                              stroke_sizing, fill_banded_min_edges, max_request_bytes)
         self._h = B.lib().lyg_server_create(C.byref(cfg))
         if not self._h:
             raise RuntimeError("lyg_server_create failed (no usable HIP device or pool)")
@@ -74,6 +74,10 @@ class Server:
         r = B.ServerResultC()
         return B.lib().lyg_server_poll(self._h, ticket, C.byref(r)) != LYG_PENDING
 
+    def banded(self, ticket: int) -> bool:
+        """Whether the request ran through the banded fill (its triangles are not lyon's)."""
+        return bool(self.wait(ticket).banded)
+
     def result(self, ticket: int):
         """(status, vertices (n, 2) f32, indices u32) on the host; waits if needed."""
         r = self.wait(ticket)
@@ -98,12 +102,15 @@ class Server:
         return {k: getattr(s, k) for k, _ in s._fields_}
 
     def run_stream(self, n_requests, rate_per_s, stroke_frac=0.6, min_segs=4, max_segs=65536,
-                   tolerance=0.05, n_distinct=256, seed=0) -> dict:
-        """Poisson request stream (lyg_server_run_stream)."""
+                   tolerance=0.05, n_distinct=256, seed=0, fill_kind=0) -> dict:
+        """Poisson request stream (lyg_server_run_stream_mixed).  fill_kind 0: the batched
+        fill workload's shapes; 1: closed random paths from the strokes' generator (the
+        same segment counts, kinds and coordinates); 2: closed random walks with the
+        strokes' segment-count distribution."""
         s = B.StreamStatsC()
-        st = B.lib().lyg_server_run_stream(self._h, seed, n_requests, rate_per_s, stroke_frac,
-                                           min_segs, max_segs, tolerance, n_distinct,
-                                           C.byref(s))
+        st = B.lib().lyg_server_run_stream_mixed(self._h, seed, n_requests, rate_per_s,
+                                                 stroke_frac, min_segs, max_segs, tolerance,
+                                                 n_distinct, int(fill_kind), C.byref(s))
         if st != 0:
             raise TessellationError(st)
         return {k: getattr(s, k) for k, _ in s._fields_}

@@ -318,12 +318,22 @@ class StrokeTessellator:
 
 
 class FillTessellator:
-    """FillTessellator (fill.rs:523-821)."""
+    """FillTessellator (fill.rs:523-821).
+
+    ``method="sweep"`` (default) is lyon's sweep, one path per wave: lyon's vertices and
+    triangles.  ``method="banded"`` spreads one large path over the whole GPU
+    (lyg_fill_path_banded): the same covered region, different triangles."""
+
+    def __init__(self, method: str = "sweep"):
+        if method not in ("sweep", "banded"):
+            raise ValueError("method is 'sweep' or 'banded'")
+        self.method = method
 
     def tessellate_path(self, path: Path, options: FillOptions, builder: BuffersBuilder):
         if not (options.tolerance > 0.0):
             raise TessellationError(4)
-        _run(B.lib().lyg_fill_path, path, options.to_c(), builder)
+        fn = B.lib().lyg_fill_path_banded if self.method == "banded" else B.lib().lyg_fill_path
+        _run(fn, path, options.to_c(), builder)
 
     tessellate = tessellate_path
 

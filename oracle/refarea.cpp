@@ -13,8 +13,9 @@
 // its first piece's midpoint and carried along it: crossing another edge changes both
 // sides by that edge's winding; where another edge ends on it (a T-junction) it is counted
 // again.  Each piece is straight, so its integral is its mid x times its height, exact up
-// to f64 rounding.  Collinear overlapping edges are not resolved (the check is for inputs
-// without them).
+// to f64 rounding.  Collinear overlapping edges (a spike, a two-point sub-path) meet the
+// ray at the piece's own point: the lowest-numbered of them accounts for the boundary of the
+// whole bundle, with the bundle's windings summed on its right.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -179,19 +180,31 @@ int orc_fill_area(const uint8_t* verbs, uint32_t n_verbs, const float* points, u
         }
         for (auto& x : th) x.join();
     }
-    auto winding_left = [&](size_t i, double mx, double my) {
-        // signed crossings of the horizontal ray from (mx, my) towards -x (half-open in y),
-        // edge i excluded: the winding just left of edge i there
-        int w = 0;
+    // signed crossings of the horizontal ray from (mx, my) towards -x (half-open in y):
+    // *wl the edges strictly left, *wb the other edges through the point itself (collinear
+    // overlaps, which meet the ray exactly there); *lead: no such edge comes before edge i,
+    // so edge i accounts for the whole bundle's boundary
+    auto winding_left = [&](size_t i, double mx, double my, int* wl, int* wb, bool* lead) {
+        int w = 0, bsum = 0;
+        bool ld = true;
+        const double tol = 1e-12 * std::max(1.0, std::fabs(mx));
         const int r = cy(my);
         for (uint32_t k = rc[r]; k < rc[r + 1]; ++k) {
             const uint32_t j = rows[k];
             if (j == i) continue;
             const E& f = es[j];
             if (!(f.y0 <= my && my < f.y1)) continue;
-            if (xat(f, my) < mx) w += f.s;
+            const double x = xat(f, my);
+            if (x < mx - tol) {
+                w += f.s;
+            } else if (x <= mx + tol) {
+                bsum += f.s;
+                if (j < i) ld = false;
+            }
         }
-        return w;
+        *wl = w;
+        *wb = bsum;
+        *lead = ld;
     };
     // each piece: the winding on its left is counted at the first piece's midpoint (and
     // again after a T-junction), and carried across proper crossings, where both sides
@@ -209,21 +222,21 @@ int orc_fill_area(const uint8_t* verbs, uint32_t n_verbs, const float* points, u
                 const auto& v = xs[i];
                 const double dex = e.x1 - e.x0, dey = e.y1 - e.y0;
                 double tp = 0.0;
-                bool need = true;
-                int wl = 0;
+                bool need = true, lead = true;
+                int wl = 0, wb = 0;
                 for (size_t q = 0; q <= v.size(); ++q) {
                     const double tn = q < v.size() ? v[q].first : 1.0;
                     if (tn > tp) {
                         const double tm = 0.5 * (tp + tn);
                         const double my = e.y0 + tm * dey, mx = e.x0 + tm * dex;
                         if (need) {
-                            wl = winding_left(i, mx, my);
+                            winding_left(i, mx, my, &wl, &wb, &lead);
                             need = false;
                         }
-                        const int wr = wl + e.s;
+                        const int wr = wl + e.s + wb;
                         const double xdy = mx * ((tn - tp) * dey);
                         // region on the left of the piece: + x dy; on the right: - x dy
-                        for (int rule = 0; rule < 2; ++rule) {
+                        for (int rule = 0; rule < 2 && lead; ++rule) {
                             const bool il = in_rule(rule, wl), ir = in_rule(rule, wr);
                             if (il != ir) (rule ? a_nz : a_eo) += (il ? 1.0 : -1.0) * xdy;
                         }

@@ -1,0 +1,170 @@
+"""The banded fill (lyg_fill_path_banded / lyg_fill_batch_banded) through the C ABI.
+
+Its triangles are not lyon's, so parity is the covered region:
+  * per-sample coverage equal to the textbook winding number (crossing count under
+    EvenOdd / NonZero) of the path's flattened edges — the edges lyon's event queue builder
+    makes (event_queue.rs:661-900) — on a grid over the path, samples within `eps` of an
+    edge excluded;
+  * total area within 1e-6 of the exact area of that region (oracle/refarea.cpp, pinned by
+    tests/test_area_cpu.py);
+  * on inputs the restated lyon sweep finishes quickly, the same checks against lyon's own
+    triangles.
+`eps` is 1e-4, widened to one f32 ulp of the largest coordinate (4.9e-4 in [4096, 8192)):
+a crossing vertex is an f32 point, so no f32 triangulation (lyon's included) places it
+closer than half an ulp to the true crossing.
+"""
+import numpy as np
+import pytest
+
+from helpers import fill_corpus
+from lyon_amd import (BuffersBuilder, FillOptions, FillRule, FillTessellator, VertexBuffers,
+                      workloads)
+from lyon_amd.tessellation import TessellationError
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def banded(path, rule, tol=0.1):
+    vb = VertexBuffers()
+    FillTessellator("banded").tessellate_path(
+        path, FillOptions(tolerance=tol, fill_rule=FillRule(rule)), BuffersBuilder(vb))
+    return np.asarray(vb.vertices, np.float32).reshape(-1, 2), np.asarray(vb.indices, np.uint32)
+
+
+def tri_area(v, i):
+    if len(i) == 0:
+        return 0.0
+    v = np.asarray(v, np.float64)
+    t = np.asarray(i, np.int64).reshape(-1, 3)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    e1, e2 = b - a, c - a
+    return float(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]).sum() / 2)
+
+
+def eps_for(points):
+    m = float(np.max(np.abs(np.asarray(points, np.float64)))) if len(points) else 1.0
+    return max(1e-4, float(np.spacing(np.float32(max(m, 1.0)))))
+
+
+def test_corpus_coverage_and_area():
+    """Every corpus path (KAT polygons, fuzz and earcut fixtures, the plain path, stars)
+    under both rules: coverage equal to the winding number of its flattened edges, and on
+    inputs in general position (no spikes or collinear overlaps) the exact area."""
+    checked = 0
+    for name, path in fill_corpus():
+        for rule in (0, 1):
+            print(f"banded corpus: {name} rule {rule}", flush=True)
+            v, i = banded(path, rule, 0.05)
+            r = O.winding_compare(path, 0.05, rule, v, i, n=1024, eps=eps_for(path.points))
+            assert r["mismatch"] == 0, (name, rule, r)
+            assert r["overlap_a"] == 0, (name, rule, r)
+            if not name.startswith(("fuzz", "earcut")):
+                a, _ = O.fill_area(path, 0.05, rule)
+                assert tri_area(v, i) == pytest.approx(a, rel=1e-6, abs=1e-6), (name, rule)
+            checked += 1
+    assert checked > 100
+
+
+def test_walk_prefix_against_lyon():
+    """A 65,536-vertex random walk (the large fill workload's generator), closed: the banded
+    fill covers exactly the winding-number region and its area.  The restated lyon sweep's
+    triangles snap vertices onto edges within tolerance / 2 (fill.rs:1095, 1360-1368) and
+    step around near-coincident crossings, which moves its area by ~1e-5 of the total on
+    this input: its coverage equals the banded fill's except on the few samples (more than
+    tolerance / 2 from any edge) where lyon's own triangles leave the winding region."""
+    path = workloads.random_walk_path(1 << 16, seed=3)
+    eps = eps_for(path.points)
+    eo, nz, k = O.fill_areas(path, 0.1)
+    assert k > 10000
+    for rule, exact in ((0, eo), (1, nz)):
+        print(f"walk prefix rule {rule}", flush=True)
+        v, i = banded(path, rule)
+        r = O.winding_compare(path, 0.1, rule, v, i, box=(0, 0, 8192, 8192), n=4096, eps=eps)
+        assert r["mismatch"] == 0 and r["overlap_a"] == 0, r
+        assert tri_area(v, i) == pytest.approx(exact, rel=1e-6)
+        st, rv, ri = O.fill(path, 0.1, rule)
+        assert st == 0
+        c = O.coverage_compare(rv, ri, v, i, box=(0, 0, 8192, 8192), n=4096, eps=0.05)
+        lw = O.winding_compare(path, 0.1, rule, rv, ri, box=(0, 0, 8192, 8192), n=4096,
+                               eps=0.05)
+        print(f"  lyon vs banded: {c['mismatch']} of {c['covered_b']} samples differ (more than "
+              f"0.05 from an edge); lyon vs the winding number: {lw['mismatch']}; areas lyon "
+              f"{tri_area(rv, ri):.6f} banded {tri_area(v, i):.6f} exact {exact:.6f}", flush=True)
+        # every sample where the two differ is one where lyon's triangles leave the path's
+        # winding-number region (its snapping and crossing recovery), not the banded fill
+        assert c["mismatch"] == lw["mismatch"], (c, lw)
+        assert c["mismatch"] <= 1e-5 * c["covered_b"], c
+        assert c["overlap_b"] == 0, c
+
+
+@pytest.fixture(scope="module")
+def walk():
+    p = workloads.random_walk_path()
+    eo, nz, k = O.fill_areas(p, 0.1)
+    return p, {0: eo, 1: nz}, k
+
+
+@pytest.mark.parametrize("rule", [1, 0], ids=["nonzero", "evenodd"])
+def test_single_large_fill(walk, rule):
+    """The single large fill workload at full size: 1,048,576 vertices, ~2.1 M crossings."""
+    path, exact, k = walk
+    assert k > 2_000_000
+    print(f"single large fill rule {rule}", flush=True)
+    v, i = banded(path, rule)
+    eps = eps_for(path.points)
+    r = O.winding_compare(path, 0.1, rule, v, i, box=(0, 0, 8192, 8192), n=4096, eps=eps)
+    print(f"  {len(i) // 3} triangles, coverage {r}", flush=True)
+    assert r["mismatch"] == 0, r
+    assert r["overlap_a"] == 0, r
+    assert tri_area(v, i) == pytest.approx(exact[rule], rel=1e-6)
+
+
+def test_band_sizes_agree():
+    """Other band sizes cut the same region (only the cuts move)."""
+    from lyon_amd.batch import set_band_size
+    path = workloads.random_walk_path(1 << 15, seed=5)
+    ref_v, ref_i = banded(path, 1)
+    try:
+        for epb, mb in ((64, 16384), (1024, 16384), (256, 16)):
+            set_band_size(epb, mb)
+            v, i = banded(path, 1)
+            c = O.coverage_compare(ref_v, ref_i, v, i, n=2048, eps=eps_for(path.points))
+            assert c["mismatch"] == 0, (epb, mb, c)
+            assert tri_area(v, i) == pytest.approx(tri_area(ref_v, ref_i), rel=2e-7)
+    finally:
+        set_band_size(256, 16384)
+
+
+def test_batch_matches_single_paths():
+    """The batched form over the fill workload's paths (per-path options) gives every path
+    the triangles the single-path call gives it."""
+    from lyon_amd.batch import DeviceBatch, FillBatcher
+    fb = workloads.fill_batch(48, seed=9)
+    opts = workloads.fill_options(48, 0.05, seed=9)
+    db = DeviceBatch(fb, opts)
+    res = FillBatcher(banded=True).run(db, vertex_offset=7)
+    assert np.all(res.status == 0)
+    for k in range(48):
+        v, idx = res.path(k)
+        rule = int(opts[k]["fill_rule"])
+        sv, si = banded(fb.path(k), rule, 0.05)
+        assert np.array_equal(v.view(np.uint32), sv.view(np.uint32)), k
+        base = int(res.vertex_offsets[k])
+        assert np.array_equal(idx - 7 - base, si), k
+
+
+def test_errors():
+    from lyon_amd.path import PathBuilder
+    b = PathBuilder()
+    b.add_polygon([(0.0, 0.0), (float("nan"), 1.0), (2.0, 0.0)], True)
+    with pytest.raises(TessellationError) as e:
+        banded(b.build(), 0)
+    assert e.value.code == 3
+    b = PathBuilder()
+    b.add_polygon([(0.0, 0.0), (1.0, 1.0), (2.0, 0.0)], True)
+    with pytest.raises(TessellationError) as e:
+        banded(b.build(), 0, tol=0.0)
+    assert e.value.code == 4
+    v, i = banded(PathBuilder().build(), 0)
+    assert len(i) == 0

@@ -138,3 +138,104 @@ def test_poisson_stream():
         assert r["requests"] == 4000
         assert r["triangles"] > 0
         assert 0 < r["p50_us"] <= r["p99_us"] <= r["max_us"]
+
+
+def c5_fill_sizes(n, seed):
+    """Edge counts drawn log-uniformly from [4, 65536], as the serving stream's fills."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(4), np.log(65536), n))).astype(int)
+
+
+def test_c5_requests_match_oracle():
+    """Requests drawn as the 4-65536-segment serving stream draws them, both fill rules:
+    strokes equal to the oracle's; fills below the banded threshold equal to lyon's
+    (vertices bit for bit, the same triangles); banded fills with coverage equal to the
+    winding number of the flattened edges and the exact area (relative 1e-6)."""
+    sb = workloads.stroke_batch(6, 4, 65536, 4096.0, seed=21)
+    so = workloads.stroke_options(6, tolerance=0.05, seed=21)
+    sizes = list(c5_fill_sizes(6, 5)) + [12, 1500, 40000]
+    with Server(pool_bytes=8 << 30, max_batch_paths=64, max_inflight=2,
+                fill_banded_min_edges=2048) as s:
+        reqs = []
+        for i in range(6):
+            o = so[i]
+            opts = StrokeOptions(line_width=float(o["line_width"]),
+                                 miter_limit=float(o["miter_limit"]), tolerance=0.05,
+                                 start_cap=int(o["start_cap"]), end_cap=int(o["end_cap"]),
+                                 line_join=int(o["line_join"]))
+            reqs.append(("stroke", sb.path(i), opts))
+        for k, n in enumerate(sizes):
+            p = workloads.random_walk_path(int(max(n, 3)), seed=100 + k, extent=4096.0)
+            reqs.append(("fill", p, FillOptions(tolerance=0.05, fill_rule=FillRule(k & 1))))
+        # fills as the serving configuration draws them (closed random curves), small
+        # enough for the oracle's sweep
+        fb = workloads.stroke_batch(6, 4, 160, 4096.0, seed=9, closed=True)
+        for k in range(6):
+            reqs.append(("fill", fb.path(k), FillOptions(tolerance=0.05,
+                                                         fill_rule=FillRule(k & 1))))
+        tickets = [submit(s, kind, p, o) for kind, p, o in reqs]
+        n_banded = 0
+        for k, ((kind, p, o), t) in enumerate(zip(reqs, tickets)):
+            st, v, i = s.result(t)
+            was_banded = s.banded(t)
+            s.release(t)
+            n_banded += was_banded
+            print(f"c5 {kind} request {k}: {len(p.verbs)} verbs, status {st}, "
+                  f"banded {was_banded}", flush=True)
+            if kind == "stroke":
+                rst, rv, ri = O.stroke(p, o.line_width, o.miter_limit, o.tolerance,
+                                       int(o.start_cap), int(o.end_cap), int(o.line_join))
+                assert st == rst
+                assert_same_geometry((v, i), (rv, ri), max(bbox_diag(p.points), o.line_width),
+                                     what=f"served c5 stroke {k}")
+            elif not was_banded:
+                rst, rv, ri = O.fill(p, o.tolerance, int(o.fill_rule))
+                assert st == rst
+                if st == 0:
+                    assert np.array_equal(v.view(np.uint32), rv.view(np.uint32))
+                    assert np.array_equal(canon(i), canon(ri))
+            else:
+                assert st == 0
+                rule = int(o.fill_rule)
+                r = O.winding_compare(p, 0.05, rule, v, i, n=1024,
+                                      eps=float(np.spacing(np.float32(4096.0))))
+                assert r["mismatch"] == 0, (k, r)
+                exact, _ = O.fill_area(p, 0.05, rule)
+                t3 = np.asarray(i, np.int64).reshape(-1, 3)
+                vv = np.asarray(v, np.float64)
+                a, b, c = vv[t3[:, 0]], vv[t3[:, 1]], vv[t3[:, 2]]
+                area = float(np.abs((b - a)[:, 0] * (c - a)[:, 1]
+                                    - (b - a)[:, 1] * (c - a)[:, 0]).sum() / 2)
+                assert abs(area - exact) <= 1e-6 * exact, (k, area, exact)
+        assert n_banded >= 3
+
+
+def test_c5_stream_random_walk_fills():
+    """A short stream at the serving configuration's sizes (log-uniform 4-65536 segments,
+    60/40 stroke/fill, 100k req/s offered, 32 GiB pool) with random-walk fills: every
+    request completes without error."""
+    with Server(pool_bytes=32 << 30, max_batch_paths=256, max_inflight=4,
+                fill_banded_min_edges=2048) as s:
+        r = s.run_stream(n_requests=2000, rate_per_s=100000, stroke_frac=0.6, min_segs=4,
+                         max_segs=65536, tolerance=0.05, n_distinct=128, seed=3, fill_kind=2)
+        print("c5 stream (walk fills)", r, flush=True)
+        assert r["requests"] == 2000 and r["failed"] == 0
+        assert r["triangles"] > 0
+        assert 0 < r["p50_us"] <= r["p99_us"] <= r["max_us"]
+
+
+def test_admission_rejects_beyond_the_worst_case_limit():
+    """A fill whose worst-case output (Bezout crossing bound) exceeds the request limit
+    completes at once with NO_SPACE and is counted; a small one beside it runs."""
+    big = workloads.stroke_batch(1, 60000, 65536, 4096.0, seed=4, closed=True).path(0)
+    small = workloads.fill_batch(2, seed=4).path(0)
+    with Server(pool_bytes=1 << 30, max_batch_paths=16, max_inflight=1,
+                max_request_bytes=64 << 20) as s:
+        tb = s.submit_fill(big, FillOptions(tolerance=0.05))
+        ts = s.submit_fill(small, FillOptions(tolerance=0.05))
+        assert s.result(tb)[0] == 20  # LYG_ERR_NO_SPACE
+        st, v, i = s.result(ts)
+        assert st == 0 and len(i) > 0
+        s.release(tb)
+        s.release(ts)
+        assert s.stats()["rejected"] == 1
