@@ -127,8 +127,10 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-large", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=10000,
+    ap.add_argument("--serve-requests", type=int, default=4000,
                     help="requests of the stream at the serving configuration")
+    ap.add_argument("--serve-split", type=int, default=4,
+                    help="windows per part of split strokes in the serving streams' second run")
     ap.add_argument("--large-vertices", type=int, default=1 << 20)
     ap.add_argument("--large-steps", type=int, default=3)
     ap.add_argument("--traffic-json",
@@ -231,6 +233,23 @@ def main():
     alg_bytes = len(batch.verbs) + 8 * len(batch.points) + 8 * n_vertices + 4 * n_indices
     emit_avg = float(np.mean(emit_ms))
     achieved = alg_bytes / (emit_avg / 1e3) / 1e9
+
+    # the same step with long paths split into parts walked by different waves
+    from lyon_amd.batch import set_stroke_split
+    split_runs = []
+    for w in (0, 4, 16):
+        set_stroke_split(w)
+        batcher.run(db)
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batcher.run(db)
+        synchronize()
+        sms = (time.perf_counter() - t0) * 1e3 / args.steps
+        split_runs.append({"windows_per_part": w, "ms_per_step": sms,
+                           "value": tris_per_step / (sms / 1e3)})
+        log(f"  stroke split into parts of {w} windows (0: whole paths): {sms:.2f} ms")
+    set_stroke_split(64)
 
     # measured copy rate of this device (1 GiB device-to-device copy kernel, read + write)
     from lyon_amd.batch import DeviceBuffer as _DB
@@ -409,6 +428,52 @@ def main():
                 "plain_per_path": {"value": fplain, "unit": "triangles/s",
                                    "sample_paths": fns},
                 "batched_over_plain": (ftris / (fms / 1e3)) / fplain}
+        fbytes = len(fbatch.verbs) + 8 * len(fbatch.points) + 8 * fres.n_vertices + \
+            4 * fres.n_indices
+        fach = fbytes / (fms / 1e3) / 1e9
+        fill["roofline"] = {"bound": "hbm", "achieved": fach, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": fach / HBM_PEAK_GBS,
+                            "scope": "whole batched step (sweep launches, scans, copy)",
+                            "algorithmic_bytes": fbytes}
+        # the same workload through the banded fill (lyon's coverage, its own triangles),
+        # at each band size; the fastest is reported
+        from lyon_amd.batch import set_band_size
+        bb = FillBatcher(banded=True)
+        banded_runs = []
+        for epb in (32, 64, 128, 256):
+            set_band_size(epb, 16384)
+            bres = bb.run(fdb, check=False)
+            synchronize()
+            ts = []
+            for _ in range(args.fill_steps):
+                t0 = time.perf_counter()
+                bb.run(fdb, check=False)
+                synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            bms = float(np.median(ts))
+            btris = bres.n_indices // 3
+            bbytes = len(fbatch.verbs) + 8 * len(fbatch.points) + 8 * bres.n_vertices + \
+                4 * bres.n_indices
+            banded_runs.append({"edges_per_band": epb, "ms_per_step": bms,
+                                "triangles_per_step": btris, "value": btris / (bms / 1e3),
+                                "paths_per_s": args.fill_paths / (bms / 1e3),
+                                "failed_paths": int(np.count_nonzero(bres.status)),
+                                "algorithmic_bytes": bbytes})
+            log(f"  banded fill, {epb} edges per band: {bms:.2f} ms")
+        set_band_size(256, 16384)
+        best = min(banded_runs, key=lambda r: r["ms_per_step"])
+        bach = best["algorithmic_bytes"] / (best["ms_per_step"] / 1e3) / 1e9
+        fill["banded"] = {
+            "note": "the same paths and options through lyg_fill_batch_banded: the covered "
+                    "region of lyon's fill, trapezoid triangles (not lyon's); median of "
+                    f"{args.fill_steps} steps per band size",
+            "best": best, "by_band_size": banded_runs,
+            "paths_per_s": best["paths_per_s"],
+            "over_lyon_triangles_form": fms / best["ms_per_step"],
+            "roofline": {"bound": "hbm", "achieved": bach, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": bach / HBM_PEAK_GBS,
+                         "scope": "whole banded step", "algorithmic_bytes":
+                             best["algorithmic_bytes"]}}
 
     # ---- single large fill (banded) -----------------------------------------------------------
     large = None
@@ -501,22 +566,32 @@ def main():
         from lyon_amd.serve import Server
         serving = []
         # (name, offered req/s, requests, stroke fraction, segments min/max, pool GiB,
-        #  fills as random walks routed to the banded fill from 2,048 edges)
+        #  fill kind (0 the fill workload's shapes, 1 random paths as the strokes, 2 random
+        #  walks), windows per part of split strokes (0: whole-path walks))
+        W = args.serve_split
         streams = [
-            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64, 8, False),
-            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256, 8, False),
-            ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256, 8, False),
+            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64, 8, 0, 0),
+            ("stroke, 4-64 segments", 100000.0, 100000, 1.0, 4, 64, 8, 0, W),
+            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256, 8, 0, 0),
+            ("stroke, 4-256 segments", 100000.0, 100000, 1.0, 4, 256, 8, 0, W),
+            ("60% stroke / 40% fill, 4-256 segments", 5000.0, 5000, 0.6, 4, 256, 8, 0, W),
             ("serving config: 60% stroke / 40% fill, log-uniform 4-65536 segments "
              "(lines/quads/cubics 25/25/50, points uniform in [0,4096]^2, fills closed, "
              "EvenOdd/NonZero), 32 GiB pool", 100000.0, args.serve_requests, 0.6, 4, 65536, 32,
-             1),
+             1, W),
             ("60% stroke / 40% fill (fills: closed random walks), log-uniform 4-65536 "
-             "segments, 32 GiB pool", 100000.0, args.serve_requests, 0.6, 4, 65536, 32, 2),
+             "segments, 32 GiB pool", 100000.0, args.serve_requests, 0.6, 4, 65536, 32, 2, W),
         ]
-        for name, rate, nreq, sfrac, smin, smax, pool_gib, fill_kind in streams:
-            log(f"serving stream: {name} at {rate:.0f} req/s")
+        os.environ["LYG_STREAM_PROGRESS"] = "1"
+        for name, rate, nreq, sfrac, smin, smax, pool_gib, fill_kind, split in streams:
+            log(f"serving stream: {name} at {rate:.0f} req/s, stroke split {split}")
+            # fills whose estimated crossings exceed 4e6 are refused at admission: their
+            # output grows with the square of their segments (random segments across the
+            # whole box) and one of them would hold the device for seconds
             with Server(pool_bytes=pool_gib << 30, max_batch_paths=1024, max_inflight=4,
-                        fill_banded_min_edges=2048 if fill_kind else 0) as srv:
+                        fill_banded_min_edges=2048 if fill_kind else 0,
+                        stroke_split_windows=split,
+                        max_fill_crossings=4_000_000 if fill_kind else 0) as srv:
                 # warm-up stream (first launches on the workers' streams, their buffers
                 # growing to size), then the measured one on the same server
                 srv.run_stream(n_requests=max(nreq // 4, 100), rate_per_s=rate,
@@ -526,7 +601,7 @@ def main():
                                    min_segs=smin, max_segs=smax, tolerance=0.05,
                                    n_distinct=256, seed=1, fill_kind=fill_kind)
             serving.append({"stream": name, "requests": nreq, "warmup_requests": max(nreq // 4, 100),
-                            "pool_gib": pool_gib,
+                            "pool_gib": pool_gib, "stroke_split_windows": split,
                             "offered_rate": rate, "completed_rate": r["completed_rate"],
                             "failed": r["failed"], "rejected_at_admission": r["rejected"],
                             "p50_us": r["p50_us"], "p99_us": r["p99_us"], "max_us": r["max_us"],
@@ -595,6 +670,7 @@ def main():
         "plain_per_path": {"value": plain_value, "unit": "triangles/s",
                            "sample_paths": ns, "batched_over_plain": value / plain_value},
         "kernel_ms": {"sizing_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
+        "stroke_split": split_runs,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,

@@ -358,6 +358,15 @@ int32_t lyg_ctx_set_tuning(lyg_ctx* ctx, const lyg_tuning* stroke, const lyg_tun
  * outgrows it is emitted again at its final place), then a compaction copy; enabled = 0:
  * a count pass and an emit pass.  Both give the same output. */
 int32_t lyg_ctx_set_stroke_staging(lyg_ctx* ctx, int enabled, float room_scale);
+/* Intra-path parallelism of the staged stroke (no lyon counterpart).  windows_per_part > 0:
+ * a path of one sub-path with at least 2 * windows_per_part windows of 64 steps (sub-path
+ * start, line ends, flattened curve points) is cut into parts of that many windows, walked
+ * by different waves at once; each part first re-walks the window before it to rebuild
+ * the walk's state, and a check per path confirms it matches the state the previous part
+ * ended in (else the path is walked again whole).  The output is the same bytes as the
+ * whole-path walk.  Default 64 (parts of 4,096 steps); 0: every path is walked by one
+ * wave. */
+int32_t lyg_ctx_set_stroke_split(lyg_ctx* ctx, uint32_t windows_per_part);
 int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
@@ -384,7 +393,12 @@ typedef struct lyg_server_config {
                                        the banded fill (lyon's coverage, other triangles); 0:
                                        never */
     uint64_t max_request_bytes; /* a request whose worst-case output exceeds this completes at
-                                   once with LYG_ERR_NO_SPACE (0: a quarter of the pool) */
+                                   once with LYG_ERR_NO_SPACE (0: the pool's size); a fill's
+                                   worst case counts its estimated crossings */
+    uint32_t stroke_split_windows; /* lyg_ctx_set_stroke_split of every worker (0: the
+                                      library's default) */
+    uint64_t max_fill_crossings;   /* a fill whose estimated crossings exceed this completes at
+                                      once with LYG_ERR_NO_SPACE (0: no limit) */
 } lyg_server_config;
 typedef struct lyg_server_result {
     int32_t status;            /* the request's lyg_status */

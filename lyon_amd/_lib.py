@@ -58,7 +58,8 @@ class ServerConfigC(C.Structure):
     _fields_ = [("pool_bytes", C.c_uint64), ("max_batch_paths", C.c_uint32),
                 ("max_inflight", C.c_uint32), ("linger_us", C.c_uint32), ("device", C.c_int32),
                 ("stroke_sizing", C.c_uint32), ("fill_banded_min_edges", C.c_uint32),
-                ("max_request_bytes", C.c_uint64)]
+                ("max_request_bytes", C.c_uint64), ("stroke_split_windows", C.c_uint32),
+                ("max_fill_crossings", C.c_uint64)]
 
 
 class ServerResultC(C.Structure):
@@ -160,6 +161,7 @@ SIGNATURES = {
     "lyg_graph_destroy": ([C.c_void_p], None),
     "lyg_ctx_set_tuning": ([C.c_void_p, C.POINTER(TuningC), C.POINTER(TuningC)], C.c_int32),
     "lyg_ctx_set_stroke_staging": ([C.c_void_p, C.c_int, C.c_float], C.c_int32),
+    "lyg_ctx_set_stroke_split": ([C.c_void_p, C.c_uint32], C.c_int32),
     "lyg_autotune_stroke": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                              C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
     "lyg_autotune_fill": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,

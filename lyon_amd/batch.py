@@ -305,6 +305,16 @@ def set_tuning(stroke_blocks_per_cu=None, fill_waves_per_block=None, fill_lds_by
         raise TessellationError(st)
 
 
+def set_stroke_split(windows_per_part: int = 64):
+    """Intra-path parallelism of the staged batched stroke (lyg_ctx_set_stroke_split): a
+    long one-sub-path path is cut into parts of this many 64-step windows walked by
+    different waves (default 64; 0: one wave per path).  The output is the same either
+    way."""
+    st = B.lib().lyg_ctx_set_stroke_split(_Ctx.get(), int(windows_per_part))
+    if st != 0:
+        raise TessellationError(st)
+
+
 def set_stroke_staging(enabled: bool = True, room_scale: float = 1.0):
     """How batched strokes size their output (lyg_ctx_set_stroke_staging): one emit pass
     into per-path staging room then a compaction copy (default), or count + emit passes.

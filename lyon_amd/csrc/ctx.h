@@ -45,6 +45,9 @@ struct lyg_ctx {
     DevBuf nsub_thr;                 // round-join subdivision thresholds (31 floats)
     DevBuf scan_tiles;               // tile sums of scans over more than 32768 counts
     DevBuf st_cap_v, st_cap_i, st_off_v, st_off_i, st_v, st_i, st_fallback;
+    // stroke: long paths split into parts of this many 64-step windows (0: whole paths)
+    uint32_t stroke_split_windows = 64;
+    DevBuf st_items, st_heads, st_spec, st_end, st_parts, st_parts_off;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
     hipEvent_t class_ev[9] = {};

@@ -166,6 +166,12 @@ int32_t lyg_ctx_set_stroke_staging(lyg_ctx* c, int enabled, float room_scale) {
     return LYG_OK;
 }
 
+int32_t lyg_ctx_set_stroke_split(lyg_ctx* c, uint32_t windows_per_part) {
+    if (!c || windows_per_part > (1u << 20)) return LYG_ERR_INVALID_ARGUMENT;
+    c->stroke_split_windows = windows_per_part;
+    return LYG_OK;
+}
+
 int32_t lyg_autotune_stroke(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen) {
     if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;

@@ -21,6 +21,22 @@ hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hip
 hipError_t lyg_launch_stroke_stage_caps(const lyg::StrokeLaunch* L, uint32_t n, float room_scale,
                                         uint32_t* cap_v, uint32_t* cap_i, hipStream_t s);
 hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipStream_t s);
+hipError_t lyg_launch_stroke_plan(const lyg::StrokeLaunch* L, uint32_t n_paths,
+                                  uint32_t split_windows, float room_scale, uint32_t* extra,
+                                  uint32_t* rv, uint32_t* ri, int grid, hipStream_t s);
+hipError_t lyg_launch_stroke_cut(const lyg::StrokeLaunch* L, uint32_t n_paths,
+                                 uint32_t split_windows, float room_scale, const uint32_t* extra,
+                                 const uint32_t* extra_off, const uint32_t* rv,
+                                 const uint32_t* ri, const uint32_t* v_off, const uint32_t* i_off,
+                                 int grid, hipStream_t s);
+hipError_t lyg_launch_stroke_items_verify(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
+                                          hipStream_t s);
+hipError_t lyg_launch_stroke_heads(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
+                                   hipStream_t s);
+hipError_t lyg_launch_stroke_items_compact(const lyg::StrokeCompact* C,
+                                           const lyg::StrokeItem* items,
+                                           const lyg::StrokeHeads* heads, uint32_t n_items,
+                                           int grid, hipStream_t s);
 hipError_t lyg_launch_nsub_thresholds(float* thr, hipStream_t s);
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
@@ -246,6 +262,175 @@ int32_t stroke_batch_staged(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Ou
     return LYG_OK;
 }
 
+// Staged single pass over items: every path is one item, except paths of one sub-path
+// with at least 2 * stroke_split_windows windows, which are cut into parts walked by
+// different waves (stroke_items_kernel).  Each item is emitted into its own staging room
+// with item-local ids; a check per path confirms that each part's prologue rebuilt the
+// carry the previous part ended with (stroke_items_verify_kernel: else the path is
+// counted whole and emitted again whole at its final place); the compaction places every
+// item in its path.  The output is the same as the per-path staged pass.  Returns 1 when
+// the batch does not fit the staging budget.
+int32_t stroke_batch_items(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
+                           lyg::StrokeLaunch& L, uint64_t* tv_out, uint64_t* ti_out) {
+    const uint32_t n = b.in.n_paths;
+    hipStream_t s = c->stream;
+    if (!ensure(c, c->st_heads, sizeof(lyg::StrokeHeads) * (size_t)n) ||
+        !ensure(c, c->st_cap_v, 4ull * n) || !ensure(c, c->st_cap_i, 4ull * n) ||
+        !ensure(c, c->st_off_v, 4ull * (n + 1)) || !ensure(c, c->st_off_i, 4ull * (n + 1)) ||
+        !ensure(c, c->st_parts, 4ull * n) || !ensure(c, c->st_parts_off, 4ull * (n + 1)) ||
+        !ensure(c, c->st_fallback, 4ull * n + 4))
+        return LYG_ERR_DEVICE;
+    L.heads = as<lyg::StrokeHeads>(c->st_heads);
+    uint64_t* h = reinterpret_cast<uint64_t*>(c->h_pinned);
+    const int pgrid = (int)std::min<uint32_t>(n, (uint32_t)std::max(1, c->cu_count) * 32u);
+    uint32_t* parts = as<uint32_t>(c->st_parts);
+    uint32_t* parts_off = as<uint32_t>(c->st_parts_off);
+    uint32_t* rv = as<uint32_t>(c->st_cap_v);
+    uint32_t* ri = as<uint32_t>(c->st_cap_i);
+    uint32_t* voff = as<uint32_t>(c->st_off_v);
+    uint32_t* ioff = as<uint32_t>(c->st_off_i);
+    // 1. the plan of every path (parts beyond the first, staging rooms), its offsets
+    if (c->timing) (void)hipEventRecord(c->ev[0], s);
+    if (!check(c, lyg_launch_fill_u32(c->d_counters, 8, 0, s), "reset") ||
+        !check(c, lyg_launch_stroke_plan(&L, n, c->stroke_split_windows, c->stroke_room_scale,
+                                         parts, rv, ri, pgrid, s),
+               "stroke plan") ||
+        !check(c, ctx_scan(c, parts, n, parts_off, c->d_totals + 2, c->d_counters + 4, s), "scan") ||
+        !check(c, ctx_scan(c, rv, n, voff, c->d_totals + 0, c->d_counters + 4, s), "scan") ||
+        !check(c, ctx_scan(c, ri, n, ioff, c->d_totals + 1, c->d_counters + 4, s), "scan") ||
+        !check(c, hipMemcpyAsync(h, c->d_totals, 24, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipMemcpyAsync(h + 3, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t sv = h[0], si = h[1], n_extra = h[2];
+    const uint64_t n_items = (uint64_t)n + n_extra;
+    if ((uint32_t)h[3] != 0 || sv >= 0x7fffffffull || si >= 0x7fffffffull ||
+        n_items >= (1ull << 26) || 8 * sv + 4 * si > STROKE_STAGE_BUDGET)
+        return 1;
+    if (!ensure(c, c->st_items, sizeof(lyg::StrokeItem) * n_items) ||
+        !ensure(c, c->st_spec, sizeof(lyg::CarryRec) * n_items) ||
+        !ensure(c, c->st_end, sizeof(lyg::CarryRec) * n_items))
+        return LYG_ERR_DEVICE;
+    if (!ensure(c, c->st_v, 8 * sv + 8) || !ensure(c, c->st_i, 4 * si + 4)) return 1;
+    L.items = as<lyg::StrokeItem>(c->st_items);
+    L.item_spec = as<lyg::CarryRec>(c->st_spec);
+    L.item_end = as<lyg::CarryRec>(c->st_end);
+    // the items, and the first points of split paths
+    if (!check(c, lyg_launch_stroke_cut(&L, n, c->stroke_split_windows, c->stroke_room_scale,
+                                        parts, parts_off, rv, ri, voff, ioff, pgrid, s),
+               "stroke cut") ||
+        !check(c, lyg_launch_stroke_heads(&L, n, pgrid, s), "stroke heads"))
+        return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[1], s);
+    // 2. one emit pass over the items into their staging rooms
+    L.n_items = (uint32_t)n_items;
+    L.out_vertices = as<float>(c->st_v);
+    L.out_indices = as<uint32_t>(c->st_i);
+    L.vertex_capacity = sv;
+    L.index_capacity = si;
+    L.vertex_offset = lyg::ITEM_ID_BIAS;  // item-local ids count from the bias
+    L.local_ids = 1;
+    L.store_counts = 0;
+    L.work_counter = c->d_counters + 1;
+    L.err = c->d_counters + 5;  // dropped writes are expected here: the counts decide
+    if (c->timing) (void)hipEventRecord(c->ev[2], s);
+    if (!check(c, lyg_launch_stroke(&L, 1, persistent_grid(c, (uint32_t)std::min<uint64_t>(n_items, 0xffffffffu)), s),
+               "stroke emit (items)"))
+        return LYG_ERR_DEVICE;
+    if (c->timing) (void)hipEventRecord(c->ev[3], s);
+    // 3. check the parts, place them in their paths, the paths' counts and final offsets
+    if (!check(c, lyg_launch_stroke_items_verify(&L, n, pgrid, s), "stroke items verify") ||
+        !check(c, lyg_launch_fill_u32(c->d_counters + 4, 1, 0, s), "reset") ||
+        !check(c, ctx_scan(c, L.path_nv, n, b.vtx_off, c->d_totals + 0, c->d_counters + 4, s),
+               "scan") ||
+        !check(c, ctx_scan(c, L.path_ni, n, b.idx_off, c->d_totals + 1, c->d_counters + 4, s),
+               "scan") ||
+        !check(c, hipMemcpyAsync(h, c->d_totals, 16, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipMemcpyAsync(h + 2, c->d_counters + 4, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    const uint64_t tv = h[0], ti = h[1];
+    *tv_out = tv;
+    *ti_out = ti;
+    if ((uint32_t)h[2] != 0) return LYG_ERR_TOO_MANY_VERTICES;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
+    if (!b.local_ids && tv > 0 && (uint64_t)b.vertex_offset + tv - 1 > 0xfffffffeull)
+        return LYG_ERR_TOO_MANY_VERTICES;
+    float* ov = nullptr;
+    uint32_t* oi = nullptr;
+    int32_t st = alloc(tv, ti, &ov, &oi);
+    if (st != LYG_OK) return st;
+    // 4. compaction; paths marked for a whole walk are listed
+    uint32_t* n_fb = as<uint32_t>(c->st_fallback) + n;
+    lyg::StrokeCompact C{};
+    C.stage_v = as<float>(c->st_v);
+    C.stage_i = as<uint32_t>(c->st_i);
+    C.vtx_off = b.vtx_off;
+    C.idx_off = b.idx_off;
+    C.out_vertices = ov;
+    C.out_indices = oi;
+    C.vertex_offset = b.vertex_offset;
+    C.local_ids = b.local_ids ? 1u : 0u;
+    C.n_paths = n;
+    C.fallback = as<uint32_t>(c->st_fallback);
+    C.n_fallback = n_fb;
+    uint32_t* hf = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
+    if (!check(c, lyg_launch_fill_u32(n_fb, 1, 0, s), "reset") ||
+        !check(c, lyg_launch_stroke_items_compact(&C, L.items, L.heads, L.n_items,
+                                                  (int)std::min<uint64_t>(n_items, 1u << 20), s),
+               "stroke items compact") ||
+        !check(c, hipMemcpyAsync(hf, n_fb, 4, hipMemcpyDeviceToHost, s), "copy") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    // 5. listed paths: emitted again whole at their final offsets
+    if (*hf > 0) {
+        lyg::StrokeLaunch F = L;
+        F.items = nullptr;
+        F.in.n_paths = *hf;
+        F.path_list = as<uint32_t>(c->st_fallback);
+        F.vtx_off = b.vtx_off;
+        F.idx_off = b.idx_off;
+        F.out_vertices = ov;
+        F.out_indices = oi;
+        F.vertex_capacity = tv;
+        F.index_capacity = ti;
+        F.vertex_offset = b.vertex_offset;
+        F.local_ids = b.local_ids ? 1u : 0u;
+        F.store_counts = 0;
+        F.work_counter = c->d_counters + 2;
+        F.err = c->d_counters + 6;
+        uint32_t* he = hf + 1;
+        if (!check(c, lyg_launch_fill_u32(c->d_counters + 2, 1, 0, s), "reset") ||
+            !check(c, lyg_launch_fill_u32(c->d_counters + 6, 1, 0, s), "reset") ||
+            !check(c, lyg_launch_stroke(&F, 1, persistent_grid(c, *hf), s), "stroke emit (again)") ||
+            !check(c, hipMemcpyAsync(he, c->d_counters + 6, 4, hipMemcpyDeviceToHost, s), "copy") ||
+            !check(c, hipStreamSynchronize(s), "sync"))
+            return LYG_ERR_DEVICE;
+        if (*he) {
+            c->last_error = "emit pass wrote outside its path range (staged/final mismatch)";
+            return LYG_ERR_INTERNAL;
+        }
+    }
+    if (trace_on()) {
+        std::vector<lyg::StrokeHeads> hh(n);
+        (void)hipMemcpy(hh.data(), L.heads, sizeof(lyg::StrokeHeads) * n, hipMemcpyDeviceToHost);
+        uint32_t split = 0, mism = 0, over = 0;
+        for (const auto& x : hh) {
+            split += x.n_items > 1;
+            mism += x.redo == 1 || x.redo == 2;
+            over += x.redo == 3;
+        }
+        fprintf(stderr, "[lyg] stroke items: %u paths (%u split), %llu items, staging %llu v / "
+                        "%llu i, out %llu v / %llu i, %u emitted again (%u state mismatches, "
+                        "%u rooms outgrown)\n",
+                n, split, (unsigned long long)n_items, (unsigned long long)sv,
+                (unsigned long long)si, (unsigned long long)tv, (unsigned long long)ti, *hf, mism,
+                over);
+        fflush(stderr);
+    }
+    return LYG_OK;
+}
+
 }  // namespace
 
 int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAlloc& alloc,
@@ -264,7 +449,9 @@ int32_t lygi::stroke_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     L.local_ids = b.local_ids ? 1u : 0u;
     if (c->stroke_staged && n > 0) {
         lyg::StrokeLaunch Ls = L;
-        const int32_t r = stroke_batch_staged(c, b, alloc, Ls, tv_out, ti_out);
+        const int32_t r = c->stroke_split_windows > 0
+                              ? stroke_batch_items(c, b, alloc, Ls, tv_out, ti_out)
+                              : stroke_batch_staged(c, b, alloc, Ls, tv_out, ti_out);
         if (r != 1) return r;
     }
     uint64_t tv = 0, ti = 0;
@@ -352,7 +539,8 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     free_buf(c->nsub_thr);
     free_buf(c->scan_tiles);
     for (DevBuf* b : {&c->st_cap_v, &c->st_cap_i, &c->st_off_v, &c->st_off_i, &c->st_v, &c->st_i,
-                      &c->st_fallback})
+                      &c->st_fallback, &c->st_items, &c->st_heads, &c->st_spec, &c->st_end,
+                      &c->st_parts, &c->st_parts_off})
         free_buf(*b);
     free_buf(c->f_boxes);
     lyg_band_free(c);

@@ -17,6 +17,55 @@ struct PathBatchDev {
     uint32_t first_path;  // index of the first path of this launch within the batch
 };
 
+// ---- long strokes split into parts walked at once --------------------------------------
+// A part of a path's walk: windows [ke, we) .. (k1, w1) of the path's steps, after a
+// prologue from (k0, w0) that rebuilds the carry.  Parts of one path are walked by
+// different waves; a check afterwards confirms that every part's prologue rebuilt the
+// state the previous part ended in, else the path is walked again whole.
+struct StrokeItem {
+    uint32_t path;
+    uint32_t flags;                  // ITEM_* below
+    uint32_t k0, w0, ke, we, k1, w1; // verb chunk / step of the prologue, start and end
+    uint32_t pt0;                    // points consumed before chunk k0
+    uint32_t v_off, v_room;          // staging room (vertices), item-local ids
+    uint32_t i_off, i_room;          // staging room (index slots)
+    uint32_t nv, ni;                 // exact counts (vertices, index slots)
+    uint32_t vbase, ibase;           // the item's first vertex / index slot in its path
+};
+enum : uint32_t { ITEM_FIRST = 1u, ITEM_SPLIT = 2u };
+// Ids with this bit pattern (0b110 in the top three bits) refer to the first vertices of
+// the path, not of the item: the last part of a split path writes them at the sub-path's
+// end.  The range is apart from the item-local ids below 0x80000000 and from the stand-in
+// ids the walk counts new vertices with (0x80000000 + lane << 24 + k) before their real
+// ids are known: the walk compares ids for equality.
+static constexpr uint32_t ID_HEAD_MARK = 0xc0000000u;
+static constexpr uint32_t ID_HEAD_MASK = 0xe0000000u;
+// Item-local ids start here, so the ids a part refers to in the part before it (a few
+// vertices back) stay below it and far from INVALID_ID: the walk compares ids for
+// equality, and an id that wrapped onto 0xffffffff would read as "no vertex".
+static constexpr uint32_t ITEM_ID_BIAS = 0x40000000u;
+
+struct F2 {
+    float x, y;
+};
+
+// Per path: its first points (the first window walked alone), its items, the check.
+struct StrokeHeads {
+    F2 f0, f1, f2;
+    uint32_t ff1, f1_base;
+    uint32_t n_items, extra_first;  // item 0 is item `path`; items 1.. at extra_first..
+    uint32_t redo;                  // 1: the parts disagreed or outgrew their room
+};
+
+// The carry state that decides the rest of a walk (see carry_store in stroke.hip).
+struct CarryRec {
+    F2 pos[3];
+    F2 next[3][2];
+    F2 last_kept;
+    uint32_t flags[3], isp[3], code[3], base[3];
+    uint32_t sp, prev_end, mnec, pad;
+};
+
 struct StrokeLaunch {
     PathBatchDev in;
     const lyg_stroke_options* opts;  // per path (opts_stride 1) or shared (stride 0)
@@ -39,6 +88,12 @@ struct StrokeLaunch {
     const uint32_t* path_list;  // optional: the launch walks path_list[0 .. n_paths)
     uint32_t store_counts;      // emit pass also stores per-path counts (staged emit)
     const float* nsub_thr;      // round-join subdivision thresholds (31 floats) or null
+    // split walks (null items: one wave walks each whole path)
+    StrokeItem* items;          // n_paths + extra items; the work queue runs over n_items
+    uint32_t n_items;
+    StrokeHeads* heads;         // per path
+    CarryRec* item_spec;        // per item: the carry its prologue rebuilt
+    CarryRec* item_end;         // per item: the carry at its end
 };
 
 // Staged single-pass stroke: copy of every path's staged output to its final place.

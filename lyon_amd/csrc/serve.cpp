@@ -17,6 +17,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -108,6 +110,7 @@ struct Request {
     uint64_t reserve = 0;        // worst-case output bytes (admission)
     uint32_t size_class = 0;     // log4 of the flattened-point bound: batches mix one class
     bool banded = false;         // a fill large enough for the banded fill
+    double crossings = 0;        // fill: estimated crossings of its edges (admission)
     Clock::time_point t_submit, t_done;
     State state = S_QUEUED;
     int32_t status = LYG_OK;
@@ -154,28 +157,91 @@ uint64_t flattened_bound(const std::vector<uint8_t>& verbs, const std::vector<fl
     return f;
 }
 
-// Crossings between the path's segments, bounded by Bezout: curves of degrees m and n
-// cross at most m n times, and a cubic crosses itself at most once.  Their flattened
-// polylines follow the curves within the tolerance, so this is the admission estimate of
-// the intersections a fill can add.
-double crossing_bound(const std::vector<uint8_t>& verbs) {
-    double s1 = 0, s2 = 0, self = 0;
+// Estimated crossings between the path's segments: over the pairs of segments whose
+// control-point boxes overlap (the only pairs that can cross), Bezout's count (curves of
+// degrees m and n cross at most m n times), plus one per cubic (a cubic crosses itself at
+// most once).  Pairs are counted exactly up to 4,096 of them; beyond, 4,096 pairs drawn
+// with a fixed seed stand for all.  The flattened polylines follow the curves within the
+// tolerance, so this is the crossing count the fill's admission reserves room for.
+double crossing_estimate(const std::vector<uint8_t>& verbs, const std::vector<float>& pts) {
+    struct Seg {
+        float x0, y0, x1, y1;
+        float deg;
+    };
+    std::vector<Seg> seg;
+    seg.reserve(verbs.size());
+    size_t p = 0;
+    float cx = 0, cy = 0, fx = 0, fy = 0;
+    double self = 0;
+    auto at = [&](size_t i, float* x, float* y) {
+        *x = i < pts.size() / 2 ? pts[2 * i] : 0.0f;
+        *y = i < pts.size() / 2 ? pts[2 * i + 1] : 0.0f;
+    };
     for (uint8_t v : verbs) {
-        const double d = v == LYG_VERB_QUADRATIC ? 2.0 : v == LYG_VERB_CUBIC ? 3.0 :
-                         (v == LYG_VERB_LINE || v == LYG_VERB_CLOSE) ? 1.0 : 0.0;
-        s1 += d;
-        s2 += d * d;
+        int np = v == LYG_VERB_LINE ? 1 : v == LYG_VERB_QUADRATIC ? 2 : v == LYG_VERB_CUBIC ? 3 : 0;
+        if (v == LYG_VERB_BEGIN) {
+            at(p++, &cx, &cy);
+            fx = cx;
+            fy = cy;
+            continue;
+        }
+        Seg s{cx, cy, cx, cy, (float)np};
+        if (v == LYG_VERB_CLOSE || v == LYG_VERB_END) {
+            // the closing edge back to the sub-path's first point
+            s = Seg{std::min(cx, fx), std::min(cy, fy), std::max(cx, fx), std::max(cy, fy), 1.0f};
+            if (v == LYG_VERB_CLOSE) p++;
+            seg.push_back(s);
+            cx = fx;
+            cy = fy;
+            continue;
+        }
+        for (int k = 0; k < np; ++k) {
+            float x, y;
+            at(p++, &x, &y);
+            s.x0 = std::min(s.x0, x);
+            s.y0 = std::min(s.y0, y);
+            s.x1 = std::max(s.x1, x);
+            s.y1 = std::max(s.y1, y);
+            cx = x;
+            cy = y;
+        }
         if (v == LYG_VERB_CUBIC) self += 1.0;
+        seg.push_back(s);
     }
-    return 0.5 * (s1 * s1 - s2) + self;
+    const size_t n = seg.size();
+    if (n < 2) return self;
+    auto term = [&](size_t i, size_t j) {
+        const Seg& a = seg[i];
+        const Seg& b = seg[j];
+        const bool hit = a.x0 <= b.x1 && b.x0 <= a.x1 && a.y0 <= b.y1 && b.y0 <= a.y1;
+        return hit ? (double)a.deg * (double)b.deg : 0.0;
+    };
+    const double pairs = 0.5 * (double)n * (double)(n - 1);
+    double sum = 0;
+    if (pairs <= 4096.0) {
+        for (size_t i = 0; i < n; ++i)
+            for (size_t j = i + 1; j < n; ++j) sum += term(i, j);
+        return sum + self;
+    }
+    uint64_t st = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+    auto next = [&]() {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (uint32_t)(st >> 33);
+    };
+    for (int k = 0; k < 4096; ++k) {
+        size_t i = next() % n, j = next() % (n - 1);
+        if (j >= i) ++j;
+        sum += term(i, j);
+    }
+    return sum / 4096.0 * pairs + self;
 }
 
 // Worst-case output bytes of a request, for admission.  Stroke: per flattened point at
 // most 4 join vertices plus a round join's arc (2^k vertices for an arc of up to pi at the
 // circle flattening step of the half width) and as many triangles plus 2.  Fill: per event
-// of the path (edge ends and crossings) two vertices and four triangles of the sweep, or
-// the banded fill's three trapezoids of four vertices and two triangles.
-uint64_t worst_case_bytes(const Request& r) {
+// of the path (edge ends and the estimated crossings) two vertices and four triangles of
+// the sweep, or the banded fill's three trapezoids of four vertices and two triangles.
+uint64_t worst_case_bytes(Request& r) {
     if (r.kind == K_STROKE) {
         const float tol = r.sopts.tolerance > 0 ? r.sopts.tolerance : 0.1f;
         const uint64_t f = flattened_bound(r.verbs, r.points, tol);
@@ -193,7 +259,7 @@ uint64_t worst_case_bytes(const Request& r) {
     }
     const float tol = r.fopts.tolerance > 0 ? r.fopts.tolerance : 0.1f;
     const double e = 2.0 * (double)flattened_bound(r.verbs, r.points, tol) + 16.0 +
-                     2.0 * crossing_bound(r.verbs);
+                     2.0 * r.crossings;
     const double per_event = r.banded ? 3.0 * (4.0 * 8.0 + 2.0 * 12.0) : 2.0 * 8.0 + 4.0 * 12.0;
     const double b = per_event * e + 8.0 * 256 + 12.0 * 512;
     return b < 1.8e19 ? (uint64_t)b : UINT64_MAX;
@@ -451,9 +517,12 @@ int32_t submit(lyg_server* s, Kind kind, const lyg_path_view* path, const void* 
         r->banded = kind == K_FILL && s->cfg.fill_banded_min_edges != 0 &&
                     f >= s->cfg.fill_banded_min_edges;
     }
+    if (kind == K_FILL) r->crossings = crossing_estimate(r->verbs, r->points);
     r->reserve = worst_case_bytes(*r);
     const uint64_t limit = s->cfg.max_request_bytes ? s->cfg.max_request_bytes
-                                                    : s->cfg.pool_bytes / 4;
+                                                    : s->cfg.pool_bytes;
+    const bool too_many_crossings = kind == K_FILL && s->cfg.max_fill_crossings != 0 &&
+                                    r->crossings > (double)s->cfg.max_fill_crossings;
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return LYG_ERR_INVALID_ARGUMENT;
@@ -463,7 +532,7 @@ int32_t submit(lyg_server* s, Kind kind, const lyg_path_view* path, const void* 
         s->any_submit = true;
         s->t_first_submit = r->t_submit;
     }
-    if (r->reserve > limit) {
+    if (r->reserve > limit || too_many_crossings) {
         // refused at admission: complete at once, nothing reserved or allocated
         r->reserve = 0;
         r->status = LYG_ERR_NO_SPACE;
@@ -505,6 +574,8 @@ lyg_server* lyg_server_create(const lyg_server_config* cfg) {
         auto w = std::make_unique<lyg_server::Worker>();
         w->ctx = lyg_ctx_create(cfg->device);
         if (w->ctx && cfg->stroke_sizing == 1) lyg_ctx_set_stroke_staging(w->ctx, 0, 1.0f);
+        if (w->ctx && cfg->stroke_split_windows)
+            lyg_ctx_set_stroke_split(w->ctx, cfg->stroke_split_windows);
         if (!w->ctx) {
             for (auto& x : s->workers)
                 if (x->ctx) lyg_ctx_destroy(x->ctx);
@@ -723,6 +794,10 @@ int32_t lyg_server_run_stream_mixed(lyg_server* s, uint64_t seed, uint32_t n_req
     std::vector<uint64_t> tickets(n_requests, 0);
     std::atomic<uint32_t> submitted{0};
     std::atomic<uint64_t> failed{0};
+    // LYG_STREAM_PROGRESS=1: a line on stderr for every twentieth of the stream completed
+    const char* pe = getenv("LYG_STREAM_PROGRESS");
+    const bool progress = pe && pe[0] == '1';
+    const uint32_t progress_every = std::max(1u, n_requests / 20);
     std::atomic<int32_t> err{LYG_OK};
     lyg_server_stats before{};
     lyg_server_get_stats(s, &before);
@@ -743,6 +818,10 @@ int32_t lyg_server_run_stream_mixed(lyg_server* s, uint64_t seed, uint32_t n_req
             lyg_server_wait(s, tickets[i], &r);
             if (r.status != LYG_OK) failed.fetch_add(1);
             lyg_server_release(s, tickets[i]);
+            if (progress && (i + 1) % progress_every == 0) {
+                fprintf(stderr, "[lyg stream] %u of %u requests done\n", i + 1, n_requests);
+                fflush(stderr);
+            }
         }
     });
     for (uint32_t i = 0; i < n_requests; ++i) {

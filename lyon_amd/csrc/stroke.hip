@@ -235,35 +235,8 @@ __device__ __forceinline__ uint32_t verb_points(uint8_t v) {
     }
 }
 
-// Walks one path with the whole wave.  Returns (vertices, triangles) in the count pass.
-template <bool EMIT>
-__device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs,
-                                ArcJob* arc_jobs, uint32_t* arc_first, uint32_t* arc_c0,
-                                uint32_t* out_nv, uint32_t* out_ni) {
-    const int l = lane_id();
-    const uint32_t gpath = path;
-    const uint32_t vb = L.in.verb_off[gpath], ve = L.in.verb_off[gpath + 1];
-    const uint32_t pb = L.in.point_off[gpath], pe = L.in.point_off[gpath + 1];
-    const uint8_t* verbs = L.in.verbs + vb;
-    const float* pts = L.in.points + 2ull * pb;
-    const uint32_t nverbs = ve - vb, npts = pe - pb;
-    const lyg_stroke_options o = L.opts[L.opts_stride ? gpath : 0];
-    const SParams P = make_params(o, L.nsub_thr);
-
-    uint32_t path_v0 = 0, path_i0 = 0, path_v_end = 0, path_i_end = 0;
-    if (EMIT) {
-        path_v0 = L.vtx_off[gpath];
-        path_i0 = L.idx_off[gpath];
-        path_v_end = L.vtx_off[gpath + 1];
-        path_i_end = L.idx_off[gpath + 1];
-        // a path that does not fit the output arrays is skipped and reported
-        if (path_v_end > L.vertex_capacity || path_i_end > L.index_capacity) {
-            if (lane_id() == 0 && L.err) atomicOr(L.err, 2u);
-            return;
-        }
-    }
-
-    Carry c;
+// The carry at the start of a path.
+__device__ __forceinline__ void carry_init(Carry& c) {
     const float nan = __builtin_nanf("");
     for (int k = 0; k < 3; ++k) {
         c.pos[k] = mk(nan, nan);
@@ -282,9 +255,80 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
     c.f1_base = INVALID_ID;
     c.run_v = 0;
     c.run_i = 0;
+}
 
-    uint32_t pt_carry = 0;
-    for (uint32_t v0 = 0; v0 < nverbs; v0 += WAVE) {
+// The carry a part of a split path starts its prologue from: inside the sub-path (three
+// or more points kept before it), nothing known about the last items, the sub-path's
+// first points from the path's head walk.  Ids that refer to the head's vertices carry
+// the marker bit so the compaction resolves them against the path instead of the part.
+__device__ __forceinline__ void carry_mid_path(Carry& c, const StrokeHeads& H) {
+    carry_init(c);
+    for (int k = 0; k < 3; ++k) c.isp[k] = 3;
+    c.sp_cnt = 3;
+    c.prev_end = false;
+    c.f0 = mk(H.f0.x, H.f0.y);
+    c.f1 = mk(H.f1.x, H.f1.y);
+    c.f2 = mk(H.f2.x, H.f2.y);
+    c.ff1 = H.ff1 != 0;
+    c.f1_base = H.f1_base == INVALID_ID ? INVALID_ID : (ID_HEAD_MARK | H.f1_base);
+}
+
+// The part of the carry the rest of a walk depends on, for checking that a part's
+// prologue rebuilt the state the previous part ended in.  Counts of kept points matter
+// only up to 3 (joins need two points before them, the sub-path end distinguishes 1, 2
+// and more), so they are stored clamped; ids are relative to the part's first vertex.
+__device__ __forceinline__ void carry_store(const Carry& c, CarryRec* r) {
+    CarryRec x;
+    for (int k = 0; k < 3; ++k) {
+        x.pos[k] = F2{c.pos[k].x, c.pos[k].y};
+        x.next[k][0] = F2{c.pub[k].next[0].x, c.pub[k].next[0].y};
+        x.next[k][1] = F2{c.pub[k].next[1].x, c.pub[k].next[1].y};
+        x.flags[k] = c.flags[k];
+        x.isp[k] = min(c.isp[k], 3u);
+        x.code[k] = c.pub[k].code;
+        x.base[k] = c.pub[k].base;
+    }
+    x.last_kept = F2{c.last_kept.x, c.last_kept.y};
+    x.sp = min(c.sp_cnt, 3u);
+    x.prev_end = c.prev_end ? 1u : 0u;
+    x.mnec = c.mnec ? 1u : 0u;
+    x.pad = 0;
+    *r = x;
+}
+
+// A range of one path's walk.  Steps are addressed as (verb chunk k, step w within the
+// chunk's steps), w a multiple of the window.  Windows before (ke, we) are a prologue:
+// walked to rebuild the carry, nothing emitted and their counts dropped.  The range
+// ends before (k1, w1); k1 = the number of chunks ends it with the path.
+struct WalkRange {
+    uint32_t k0, w0, ke, we, k1, w1;
+    uint32_t pt0;  // points consumed by the verbs before chunk k0
+};
+
+// Walks [R.k0, R.k1) of one path with the whole wave from the carry `c`, which it
+// updates; c.run_v / c.run_i count the vertices / triangles from the range's emitted
+// start.  EMIT: output to [v0, v_end) / [i0, i_end) of the output arrays.  `spec`: where
+// the carry is stored when the prologue ends (null when there is none).
+template <bool EMIT, bool RANGED = true>
+__device__ __forceinline__ void stroke_walk(const StrokeLaunch& L, uint32_t gpath, const WalkRange& R,
+                            Carry& c, uint32_t path_v0, uint32_t path_v_end, uint32_t path_i0,
+                            uint32_t path_i_end, CarryRec* spec, VRec* recs, ArcJob* arc_jobs,
+                            uint32_t* arc_first, uint32_t* arc_c0) {
+    const int l = lane_id();
+    const uint32_t vb = L.in.verb_off[gpath], ve = L.in.verb_off[gpath + 1];
+    const uint32_t pb = L.in.point_off[gpath], pe = L.in.point_off[gpath + 1];
+    const uint8_t* verbs = L.in.verbs + vb;
+    const float* pts = L.in.points + 2ull * pb;
+    const uint32_t nverbs = ve - vb, npts = pe - pb;
+    const lyg_stroke_options o = L.opts[L.opts_stride ? gpath : 0];
+    const SParams P = make_params(o, L.nsub_thr);
+    const float nan = __builtin_nanf("");
+    bool started = !RANGED || (R.k0 == R.ke && R.w0 == R.we);
+
+    uint32_t pt_carry = RANGED ? R.pt0 : 0u;
+    for (uint32_t v0 = RANGED ? R.k0 * WAVE : 0u; v0 < nverbs; v0 += WAVE) {
+        const uint32_t kch = v0 / WAVE;
+        if (RANGED && (kch > R.k1 || (kch == R.k1 && R.w1 == 0))) break;
         // ---- 1. decode a chunk of verbs ------------------------------------------------
         const uint32_t nch = min((uint32_t)WAVE, nverbs - v0);
         uint8_t verb = LYG_VERB_END;
@@ -313,7 +357,18 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
         // t chain of the curve half the previous window ended in (key, steps, units)
         uint32_t ch_key = NO_CHAIN, ch_j = 0;
         uint64_t ch_T = 0;
-        for (uint32_t w0 = 0; w0 < nsteps; w0 += WAVE) {
+        for (uint32_t w0 = RANGED && kch == R.k0 ? R.w0 : 0u; w0 < nsteps; w0 += WAVE) {
+            if (RANGED && kch == R.k1 && w0 >= R.w1) break;
+            const bool silent = RANGED && (kch < R.ke || (kch == R.ke && w0 < R.we));
+            if (!silent && !started) {
+                // the prologue ends: ids from here on count from the range's start
+                started = true;
+                for (int k = 0; k < 3; ++k)
+                    if (c.pub[k].base != INVALID_ID) c.pub[k].base -= c.run_v;
+                c.run_v = 0;
+                c.run_i = 0;
+                if (spec && l == 0) carry_store(c, spec);
+            }
             const uint32_t g = w0 + (uint32_t)l;
             const bool valid = g < nsteps;
             vec2 pos = mk(nan, nan);
@@ -570,7 +625,7 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
             }
             ein.pm1_base = prv.base;
 
-            if (EMIT) {
+            if (EMIT && !silent) {
                 // vertex id -> slot: id - (vertex_offset + id_v0 - path_v0), modulo 2^32
                 Sink se{real_base, 0, 0, L.out_vertices, L.vertex_offset + id_v0 - path_v0, nullptr, true,
                         path_v_end, 0, L.err};
@@ -715,8 +770,59 @@ __device__ void stroke_one_path(const StrokeLaunch& L, uint32_t path, VRec* recs
         }
         wave_sync();
     }
+}
+
+// Walks one whole path with the whole wave.  Returns (vertices, triangles).
+template <bool EMIT>
+__device__ void stroke_one_path(const StrokeLaunch& L, uint32_t gpath, VRec* recs,
+                                ArcJob* arc_jobs, uint32_t* arc_first, uint32_t* arc_c0,
+                                uint32_t* out_nv, uint32_t* out_ni) {
+    uint32_t path_v0 = 0, path_i0 = 0, path_v_end = 0, path_i_end = 0;
+    if (EMIT) {
+        path_v0 = L.vtx_off[gpath];
+        path_i0 = L.idx_off[gpath];
+        path_v_end = L.vtx_off[gpath + 1];
+        path_i_end = L.idx_off[gpath + 1];
+        // a path that does not fit the output arrays is skipped and reported
+        if (path_v_end > L.vertex_capacity || path_i_end > L.index_capacity) {
+            if (lane_id() == 0 && L.err) atomicOr(L.err, 2u);
+            return;
+        }
+    }
+    Carry c;
+    carry_init(c);
+    const WalkRange R{0, 0, 0, 0, 0xffffffffu, 0, 0};
+    stroke_walk<EMIT, false>(L, gpath, R, c, path_v0, path_v_end, path_i0, path_i_end, nullptr,
+                             recs, arc_jobs, arc_first, arc_c0);
     *out_nv = c.run_v;
     *out_ni = c.run_i;
+}
+
+// One item of the split walk: a whole path, or a part of a long one (after its prologue),
+// into its own staging room with item-local ids; stores its exact counts and, for parts,
+// the carries the check compares.
+template <bool EMIT>
+__device__ void stroke_item(const StrokeLaunch& L, uint32_t item, VRec* recs, ArcJob* arc_jobs,
+                            uint32_t* arc_first, uint32_t* arc_c0) {
+    const StrokeItem it = L.items[item];
+    const WalkRange R{it.k0, it.w0, it.ke, it.we, it.k1, it.w1, it.pt0};
+    const bool first = (it.flags & ITEM_FIRST) != 0;
+    if (EMIT && ((uint64_t)it.v_off + it.v_room > L.vertex_capacity ||
+                 (uint64_t)it.i_off + it.i_room > L.index_capacity)) {
+        if (lane_id() == 0 && L.err) atomicOr(L.err, 2u);
+        return;
+    }
+    Carry c;
+    if (first) carry_init(c);
+    else carry_mid_path(c, L.heads[it.path]);
+    stroke_walk<EMIT>(L, it.path, R, c, it.v_off, it.v_off + it.v_room, it.i_off,
+                      it.i_off + it.i_room, first ? nullptr : L.item_spec + item, recs, arc_jobs,
+                      arc_first, arc_c0);
+    if (lane_id() == 0) {
+        L.items[item].nv = c.run_v;
+        L.items[item].ni = 3u * c.run_i;
+        if (it.flags & ITEM_SPLIT) carry_store(c, L.item_end + item);
+    }
 }
 
 // Fetches the next path id for the whole wave.  Every lane executes the atomic (lane 0
@@ -732,7 +838,7 @@ __device__ __forceinline__ uint32_t next_path(uint32_t* counter) {
 // chain of shuffles, LDS reads and scans), so occupancy beats spill-free code.  Measured
 // on the 10,000-path stroke workload: count pass 11.4 ms at 1 wave/SIMD, 7.2 at 2, 6.0 at
 // 4; emit pass 16.0 ms at 1, 11.1 at 2, 15.7 at 4 (its larger state spills too much).
-template <bool EMIT>
+template <bool EMIT, bool ITEMS>
 __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     LYG_WAVES_PER_EU(EMIT ? 2 : 4) stroke_kernel(StrokeLaunch L) {
     __shared__ VRec recs_all[WAVES_PER_BLOCK][WAVE];
@@ -747,6 +853,14 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     uint32_t* arc_c0 = arc_c0_all[EMIT ? w : 0];
     const int l = lane_id();
     uint32_t pid = next_path(L.work_counter);
+    if constexpr (ITEMS) {
+        // parts of split paths and whole paths, as listed by stroke_items_kernel
+        while (pid < L.n_items) {
+            stroke_item<EMIT>(L, pid, recs, arc_jobs, arc_first, arc_c0);
+            pid = next_path(L.work_counter);
+        }
+        return;
+    }
     while (pid < L.in.n_paths) {
         const uint32_t gp = L.path_list ? L.path_list[pid] : L.in.first_path + pid;
         uint32_t nv = 0, ni = 0;
@@ -843,6 +957,386 @@ __global__ void __launch_bounds__(WAVE) stroke_stage_caps_kernel(StrokeLaunch L,
     cap_i[path] = (uint32_t)(3 * ct < 0x7fffffffull ? 3 * ct : 0x7fffffffull);
 }
 
+// ---- split walks: plan, check, compaction ------------------------------------------------
+// Steps of one verb chunk (64 verbs from v0): every lane decodes its verb as the walk does;
+// *my_off / *my_cnt: the lane's verb's first step in the chunk and its steps.
+__device__ __forceinline__ uint32_t chunk_steps(const uint8_t* verbs, uint32_t nverbs,
+                                                uint32_t v0, const float* pts, uint32_t npts,
+                                                uint32_t* pt_carry, float tol, uint8_t* my_verb,
+                                                uint32_t* my_off, uint32_t* my_cnt) {
+    const int l = lane_id();
+    const uint32_t nch = min((uint32_t)WAVE, nverbs - v0);
+    uint8_t verb = LYG_VERB_END;
+    uint32_t np = 0;
+    if ((uint32_t)l < nch) {
+        verb = verbs[v0 + l];
+        np = verb_points(verb);
+    }
+    const uint32_t np_incl = incl_scan(np);
+    const uint32_t first_pt = *pt_carry + np_incl - np;
+    *pt_carry += readlane_u(np_incl, WAVE - 1);
+    uint32_t cnt = 0;
+    if ((uint32_t)l < nch) {
+        VRec r;
+        decode_verb(verb, pts, npts, first_pt, tol, r);
+        cnt = r.nA + r.nB;
+    }
+    *my_verb = (uint32_t)l < nch ? verb : (uint8_t)0xff;
+    const uint32_t incl = incl_scan(cnt);
+    *my_off = incl - cnt;
+    *my_cnt = cnt;
+    return readlane_u(incl, WAVE - 1);
+}
+
+// Staging room of `steps` steps with `verbs` verb boundaries (joins), as the bound of
+// stroke_stage_caps_kernel: two vertices and triangles per step (the fast path), a full
+// join with its round arcs per boundary, caps per sub-path.
+__device__ __forceinline__ void item_room(uint64_t steps, uint64_t verbs, uint64_t subs,
+                                          uint64_t A, float room_scale, uint32_t* rv,
+                                          uint32_t* ri) {
+    const double f = room_scale > 0.0f ? (double)room_scale : 1.0;
+    const uint64_t cv = (uint64_t)(f * (double)(2 * steps + verbs * (4 + 2 * A) +
+                                                subs * (8 + 4 * A) + 64));
+    const uint64_t ct = (uint64_t)(f * (double)(2 * steps + verbs * (6 + 2 * A) +
+                                                subs * (10 + 4 * A) + 64));
+    *rv = (uint32_t)(cv < 0x7fffffffull ? cv : 0x7fffffffull);
+    *ri = (uint32_t)(3 * ct < 0x7fffffffull ? 3 * ct : 0x7fffffffull);
+}
+
+// The cutting of one path into items.  A path of one sub-path with at least
+// 2 * split_windows windows is cut into parts of split_windows windows (part j starts at
+// window j * split_windows of the path; its prologue is the window before it); every
+// other path is one item.  PLAN: counts the parts beyond the first and the staging room
+// of all of them (*extra, *rv, *ri).  Otherwise: writes the items, item 0 at index
+// `path`, parts 1.. from extra_first, staging rooms from v_base / i_base on (the rooms
+// are recomputed exactly as planned).  One wave.
+template <bool PLAN>
+__device__ void cut_path(const StrokeLaunch& L, uint32_t path, uint32_t split_windows,
+                         float room_scale, uint32_t* extra, uint32_t* rv, uint32_t* ri,
+                         uint32_t n_extra_planned, uint32_t extra_first, uint32_t v_base,
+                         uint32_t i_base) {
+    const int l = lane_id();
+    const uint32_t vb = L.in.verb_off[path], ve = L.in.verb_off[path + 1];
+    const uint32_t pb = L.in.point_off[path], pe = L.in.point_off[path + 1];
+    const uint8_t* verbs = L.in.verbs + vb;
+    const float* pts = L.in.points + 2ull * pb;
+    const uint32_t nverbs = ve - vb, npts = pe - pb;
+    const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
+    const float tol = o.tolerance;
+    const bool round = o.line_join == LYG_JOIN_ROUND || o.start_cap == LYG_CAP_ROUND ||
+                       o.end_cap == LYG_CAP_ROUND;
+    const float hw = o.line_width * 0.5f;
+    uint64_t A = 0;
+    if (round && tol > 0.0f && hw > 0.0f)
+        A = arc_count(arc_subdivisions(2.0f * 3.14159265358979323846f, hw, tol));
+    // parts of split_windows windows; with PLAN the eligibility is decided at the end
+    const bool cut = PLAN ? split_windows > 0 : n_extra_planned > 0;
+    uint64_t steps = 0, windows = 0, sum_v = 0, sum_i = 0;
+    uint32_t n_begin = 0, n_end = 0, pt_carry = 0, open_j = 0;
+    bool end_last = false;
+    // the last four windows seen (chunk, first step, points before the chunk, steps), for
+    // prologues: a prologue goes back over whole windows until it holds PRO_STEPS steps,
+    // so the last items it carries over are all computed from points it walked
+    constexpr uint32_t PRO_STEPS = 32;
+    uint32_t rk[4] = {0, 0, 0, 0}, rw[4] = {0, 0, 0, 0}, rp[4] = {0, 0, 0, 0},
+             rn[4] = {0, 0, 0, 0};
+    uint32_t n_seen = 0;
+    StrokeItem cur{};
+    cur.path = path;
+    cur.flags = ITEM_FIRST | ITEM_SPLIT;
+    uint64_t cur_steps = 0, cur_verbs = 0;
+    auto close_item = [&]() {
+        // joins: the verbs that start in the part, the one in progress at its start, and
+        // room for one more (a cusp inside a curve) per verb
+        uint32_t pv, pi;
+        item_room(cur_steps, 2 * (cur_verbs + 1), 1, A, room_scale, &pv, &pi);
+        if (!PLAN && l == 0) {
+            cur.v_room = pv;
+            cur.i_room = pi;
+            cur.v_off = v_base + (uint32_t)sum_v;
+            cur.i_off = i_base + (uint32_t)sum_i;
+            L.items[open_j == 0 ? path : extra_first + open_j - 1] = cur;
+        }
+        sum_v += pv;
+        sum_i += pi;
+    };
+    if (PLAN || cut) {
+        for (uint32_t v0 = 0; v0 < nverbs; v0 += WAVE) {
+            const uint32_t k = v0 / WAVE;
+            const uint32_t pt_k = pt_carry;
+            uint8_t mv;
+            uint32_t off, cnt;
+            const uint32_t ns =
+                chunk_steps(verbs, nverbs, v0, pts, npts, &pt_carry, tol, &mv, &off, &cnt);
+            const bool has = mv != (uint8_t)0xff;
+            steps += ns;
+            n_begin += __popcll(ballot(mv == LYG_VERB_BEGIN));
+            const uint64_t em = ballot(mv == LYG_VERB_END || mv == LYG_VERB_CLOSE);
+            n_end += __popcll(em);
+            if (em && v0 + (uint32_t)highest_bit(em) == nverbs - 1) end_last = true;
+            if (!cut) continue;
+            const uint32_t nw = (ns + WAVE - 1) / WAVE;
+            uint32_t part_s = 0;  // first step of the open part in this chunk
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint64_t gw = windows + w;
+                if (gw > 0 && gw % split_windows == 0) {
+                    // the open part ends here; the next starts with its prologue
+                    const uint32_t s0 = w * WAVE;
+                    cur_verbs += __popcll(ballot(has && off >= part_s && off < s0));
+                    cur.k1 = k;
+                    cur.w1 = s0;
+                    close_item();
+                    ++open_j;
+                    // the prologue: back over the windows seen until PRO_STEPS steps
+                    uint32_t q = 0, got = 0;
+                    const uint32_t avail = n_seen < 4 ? n_seen : 4u;
+                    for (; q + 1 < avail && got + rn[q] < PRO_STEPS; ++q) got += rn[q];
+                    cur = StrokeItem{};
+                    cur.path = path;
+                    cur.flags = ITEM_SPLIT;
+                    cur.k0 = rk[q];
+                    cur.w0 = rw[q];
+                    cur.pt0 = rp[q];
+                    cur.ke = k;
+                    cur.we = s0;
+                    cur_steps = 0;
+                    cur_verbs = 0;
+                    part_s = s0;
+                }
+                const uint32_t wn = min((uint32_t)WAVE, ns - w * WAVE);
+                cur_steps += wn;
+                // newest window first in the ring
+                for (int r = 3; r > 0; --r) {
+                    rk[r] = rk[r - 1];
+                    rw[r] = rw[r - 1];
+                    rp[r] = rp[r - 1];
+                    rn[r] = rn[r - 1];
+                }
+                rk[0] = k;
+                rw[0] = w * WAVE;
+                rp[0] = pt_k;
+                rn[0] = wn;
+                ++n_seen;
+            }
+            cur_verbs += __popcll(ballot(has && off >= part_s));
+            windows += nw;
+        }
+        if (cut) {
+            cur.k1 = 0xffffffffu;
+            cur.w1 = 0;
+            close_item();
+        }
+    }
+    if (PLAN) {
+        const bool one_sub = nverbs > 0 && verbs[0] == LYG_VERB_BEGIN && n_begin == 1 &&
+                             n_end == 1 && end_last;
+        const bool split = cut && one_sub && tol > 0.0f && windows >= 2ull * split_windows;
+        uint32_t wv, wi;
+        item_room(steps, nverbs, n_begin, A, room_scale, &wv, &wi);
+        *extra = split ? open_j : 0u;
+        *rv = split ? (uint32_t)min<uint64_t>(sum_v, 0x7fffffffull) : wv;
+        *ri = split ? (uint32_t)min<uint64_t>(sum_i, 0x7fffffffull) : wi;
+    } else if (!cut && l == 0) {
+        StrokeItem it{};
+        it.path = path;
+        it.flags = ITEM_FIRST;
+        it.k1 = 0xffffffffu;
+        it.v_room = *rv;
+        it.i_room = *ri;
+        it.v_off = v_base;
+        it.i_off = i_base;
+        L.items[path] = it;
+    }
+}
+
+// One wave per path: the plan (parts beyond the first, staging rooms) into per-path
+// arrays, which scans turn into item and staging offsets.
+__global__ void __launch_bounds__(WAVE) stroke_plan_kernel(StrokeLaunch L, uint32_t n_paths,
+                                                           uint32_t split_windows,
+                                                           float room_scale, uint32_t* extra,
+                                                           uint32_t* rv, uint32_t* ri) {
+    for (uint32_t path = blockIdx.x; path < n_paths; path += gridDim.x) {
+        uint32_t e, v, i;
+        cut_path<true>(L, path, split_windows, room_scale, &e, &v, &i, 0, 0, 0, 0);
+        if (lane_id() == 0) {
+            extra[path] = e;
+            rv[path] = v;
+            ri[path] = i;
+        }
+    }
+}
+
+// One wave per path: the items, at the offsets the scans gave (extra_off: parts beyond
+// the first of the paths before; v_off / i_off: staging), and the path's item range.
+__global__ void __launch_bounds__(WAVE) stroke_cut_kernel(StrokeLaunch L, uint32_t n_paths,
+                                                          uint32_t split_windows,
+                                                          float room_scale,
+                                                          const uint32_t* extra,
+                                                          const uint32_t* extra_off,
+                                                          const uint32_t* rv, const uint32_t* ri,
+                                                          const uint32_t* v_off,
+                                                          const uint32_t* i_off) {
+    for (uint32_t path = blockIdx.x; path < n_paths; path += gridDim.x) {
+        const uint32_t e = extra[path];
+        const uint32_t ef = n_paths + extra_off[path];
+        uint32_t v = rv[path], i = ri[path];
+        cut_path<false>(L, path, split_windows, room_scale, nullptr, &v, &i, e, ef, v_off[path],
+                        i_off[path]);
+        if (lane_id() == 0) {
+            StrokeHeads H;
+            H.f0 = H.f1 = H.f2 = F2{0.0f, 0.0f};
+            H.ff1 = 0;
+            H.f1_base = INVALID_ID;
+            H.n_items = e + 1;
+            H.extra_first = e ? ef : 0u;
+            H.redo = 0;
+            L.heads[path] = H;
+        }
+    }
+}
+
+// One wave per split path: the sub-path's first points, from its first window walked
+// alone (a path whose first window does not keep three points is marked to be walked
+// again whole).
+__global__ void __launch_bounds__(WAVE) stroke_heads_kernel(StrokeLaunch L, uint32_t n_paths) {
+    __shared__ VRec recs[WAVE];
+    for (uint32_t path = blockIdx.x; path < n_paths; path += gridDim.x) {
+        if (L.heads[path].n_items < 2) continue;
+        Carry c;
+        carry_init(c);
+        const WalkRange R{0, 0, 0, 0, 0, (uint32_t)WAVE, 0};
+        stroke_walk<false>(L, path, R, c, 0, 0, 0, 0, nullptr, recs, nullptr, nullptr, nullptr);
+        if (lane_id() == 0) {
+            StrokeHeads* H = L.heads + path;
+            if (c.sp_cnt >= 3 && !c.prev_end) {
+                H->f0 = F2{c.f0.x, c.f0.y};
+                H->f1 = F2{c.f1.x, c.f1.y};
+                H->f2 = F2{c.f2.x, c.f2.y};
+                H->ff1 = c.ff1 ? 1u : 0u;
+                H->f1_base = c.f1_base;
+            } else {
+                H->redo = 2;  // the parts cannot be trusted
+            }
+        }
+    }
+}
+
+// Whether the carry the previous part ended with (ids relative to that part, which made
+// nv_prev vertices) is the carry the next part's prologue rebuilt.
+__device__ __forceinline__ bool carry_match(const CarryRec& e, uint32_t nv_prev,
+                                            const CarryRec& s) {
+    auto same = [](F2 a, F2 b) {
+        return __float_as_uint(a.x) == __float_as_uint(b.x) &&
+               __float_as_uint(a.y) == __float_as_uint(b.y);
+    };
+    bool ok = same(e.last_kept, s.last_kept) && e.sp == s.sp && e.prev_end == s.prev_end &&
+              (e.sp >= 2 || e.mnec == s.mnec);
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t eb = e.base[k] == INVALID_ID ? INVALID_ID : e.base[k] - nv_prev;
+        ok = ok && same(e.pos[k], s.pos[k]) && same(e.next[k][0], s.next[k][0]) &&
+             same(e.next[k][1], s.next[k][1]) && e.flags[k] == s.flags[k] &&
+             e.isp[k] == s.isp[k] && e.code[k] == s.code[k] && eb == s.base[k];
+    }
+    return ok;
+}
+
+// One wave per path: checks its parts' carries, places the parts in the path (vertex and
+// index bases), stores the path's counts, and marks it to be emitted again whole when a
+// part's prologue went wrong (its counts then come from a whole walk here) or a part
+// outgrew its staging room.
+__global__ void __launch_bounds__(WAVE) stroke_items_verify_kernel(StrokeLaunch L,
+                                                                   uint32_t n_paths) {
+    __shared__ VRec recs[WAVE];
+    const int l = lane_id();
+    for (uint32_t path = blockIdx.x; path < n_paths; path += gridDim.x) {
+        const uint32_t n = L.heads[path].n_items;
+        const uint32_t ef = L.heads[path].extra_first;
+        uint32_t vrun = 0, irun = 0;
+        bool mism = L.heads[path].redo != 0, over = false;
+        for (uint32_t j0 = 0; j0 < n; j0 += WAVE) {
+            const uint32_t j = j0 + (uint32_t)l;
+            const bool valid = j < n;
+            const uint32_t q = j == 0 ? path : ef + j - 1;
+            uint32_t nv = 0, ni = 0;
+            bool o = false, m = false;
+            if (valid) {
+                const StrokeItem& it = L.items[q];
+                nv = it.nv;
+                ni = it.ni;
+                o = nv > it.v_room || ni > it.i_room;
+                if (j > 0) {
+                    const uint32_t qp = j == 1 ? path : ef + j - 2;
+                    m = !carry_match(L.item_end[qp], L.items[qp].nv, L.item_spec[q]);
+                }
+            }
+            const uint32_t vi = incl_scan(nv), ii = incl_scan(ni);
+            if (valid) {
+                L.items[q].vbase = vrun + vi - nv;
+                L.items[q].ibase = irun + ii - ni;
+            }
+            vrun += readlane_u(vi, WAVE - 1);
+            irun += readlane_u(ii, WAVE - 1);
+            mism = mism || ballot(m) != 0;
+            over = over || ballot(o) != 0;
+        }
+        if (mism) {
+            Carry c;
+            carry_init(c);
+            const WalkRange R{0, 0, 0, 0, 0xffffffffu, 0, 0};
+            stroke_walk<false, false>(L, path, R, c, 0, 0, 0, 0, nullptr, recs, nullptr, nullptr,
+                                      nullptr);
+            vrun = c.run_v;
+            irun = 3u * c.run_i;
+        }
+        if (l == 0) {
+            L.path_nv[path] = vrun;
+            L.path_ni[path] = irun;
+            // 1: a part's prologue did not rebuild the state; 3: a part outgrew its room
+            L.heads[path].redo = mism ? 1u : (over ? 3u : 0u);
+        }
+    }
+}
+
+// One block of 256 threads per item (grid-stride): copies the item's staged vertices to
+// its place in its path's final range and its indices with ids made final (item-local
+// less ITEM_ID_BIAS + the item's base in the path + the path's id base; head-marked ids +
+// the path's id base); a path marked for a whole walk is listed once instead.
+__global__ void __launch_bounds__(256) stroke_items_compact_kernel(StrokeCompact C,
+                                                                   const StrokeItem* items,
+                                                                   const StrokeHeads* heads,
+                                                                   uint32_t n_items) {
+    for (uint32_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+        const StrokeItem it = items[q];
+        const uint32_t p = it.path;
+        if (heads[p].redo) {
+            if (threadIdx.x == 0 && (it.flags & ITEM_FIRST))
+                C.fallback[atomicAdd(C.n_fallback, 1u)] = p;
+            continue;
+        }
+        const uint32_t v0 = C.vtx_off[p] + it.vbase, i0 = C.idx_off[p] + it.ibase;
+        const uint2* __restrict__ sv = reinterpret_cast<const uint2*>(C.stage_v) + it.v_off;
+        uint2* __restrict__ dv = reinterpret_cast<uint2*>(C.out_vertices) + v0;
+        const uint32_t nv = it.nv, ni = it.ni;
+        uint32_t j = threadIdx.x;
+        for (; j + 768 < nv; j += 1024) {
+            const uint2 a = sv[j], b = sv[j + 256], c = sv[j + 512], d = sv[j + 768];
+            dv[j] = a;
+            dv[j + 256] = b;
+            dv[j + 512] = c;
+            dv[j + 768] = d;
+        }
+        for (; j < nv; j += 256) dv[j] = sv[j];
+        const uint32_t pbase = C.vertex_offset + (C.local_ids ? 0u : C.vtx_off[p]);
+        const uint32_t add = pbase + it.vbase - ITEM_ID_BIAS;
+        const uint32_t* __restrict__ si = C.stage_i + it.i_off;
+        uint32_t* __restrict__ di = C.out_indices + i0;
+        for (j = threadIdx.x; j < ni; j += 256) {
+            const uint32_t x = si[j];
+            di[j] = (x & ID_HEAD_MASK) == ID_HEAD_MARK ? (x - ID_HEAD_MARK) + pbase : x + add;
+        }
+    }
+}
+
 // One block of 256 threads per path (grid-stride over paths): copies the path's staged vertices and
 // indices (adding its id base) to the final offsets, or lists the path for a second emit
 // when its exact counts exceed its staging room.
@@ -903,12 +1397,56 @@ hipError_t lyg_launch_stroke_compact(const lyg::StrokeCompact* C, int grid, hipS
     hipLaunchKernelGGL(lyg::stroke_compact_kernel, dim3(grid), dim3(256), 0, s, *C);
     return hipGetLastError();
 }
+hipError_t lyg_launch_stroke_plan(const lyg::StrokeLaunch* L, uint32_t n_paths,
+                                  uint32_t split_windows, float room_scale, uint32_t* extra,
+                                  uint32_t* rv, uint32_t* ri, int grid, hipStream_t s) {
+    if (n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_plan_kernel, dim3(grid), dim3(lyg::WAVE), 0, s, *L, n_paths,
+                       split_windows, room_scale, extra, rv, ri);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_stroke_cut(const lyg::StrokeLaunch* L, uint32_t n_paths,
+                                 uint32_t split_windows, float room_scale, const uint32_t* extra,
+                                 const uint32_t* extra_off, const uint32_t* rv,
+                                 const uint32_t* ri, const uint32_t* v_off, const uint32_t* i_off,
+                                 int grid, hipStream_t s) {
+    if (n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_cut_kernel, dim3(grid), dim3(lyg::WAVE), 0, s, *L, n_paths,
+                       split_windows, room_scale, extra, extra_off, rv, ri, v_off, i_off);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_stroke_heads(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
+                                   hipStream_t s) {
+    if (n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_heads_kernel, dim3(grid), dim3(lyg::WAVE), 0, s, *L, n_paths);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_stroke_items_verify(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
+                                          hipStream_t s) {
+    if (n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_items_verify_kernel, dim3(grid), dim3(lyg::WAVE), 0, s, *L,
+                       n_paths);
+    return hipGetLastError();
+}
+hipError_t lyg_launch_stroke_items_compact(const lyg::StrokeCompact* C,
+                                           const lyg::StrokeItem* items,
+                                           const lyg::StrokeHeads* heads, uint32_t n_items,
+                                           int grid, hipStream_t s) {
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(lyg::stroke_items_compact_kernel, dim3(grid), dim3(256), 0, s, *C, items,
+                       heads, n_items);
+    return hipGetLastError();
+}
 hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hipStream_t stream) {
     dim3 block(lyg::WAVE * lyg::WAVES_PER_BLOCK);
-    if (emit)
-        hipLaunchKernelGGL(lyg::stroke_kernel<true>, dim3(grid), block, 0, stream, *L);
-    else
-        hipLaunchKernelGGL(lyg::stroke_kernel<false>, dim3(grid), block, 0, stream, *L);
+    if (L->items) {
+        if (!emit) return hipErrorInvalidValue;  // items are emitted only
+        hipLaunchKernelGGL((lyg::stroke_kernel<true, true>), dim3(grid), block, 0, stream, *L);
+    } else if (emit) {
+        hipLaunchKernelGGL((lyg::stroke_kernel<true, false>), dim3(grid), block, 0, stream, *L);
+    } else {
+        hipLaunchKernelGGL((lyg::stroke_kernel<false, false>), dim3(grid), block, 0, stream, *L);
+    }
     return hipGetLastError();
 }
 

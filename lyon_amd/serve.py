@@ -17,9 +17,11 @@ LYG_PENDING = 21
 
 class Server:
     def __init__(self, pool_bytes=1 << 30, max_batch_paths=1024, max_inflight=4, linger_us=0,
-                 device=0, stroke_sizing=0, fill_banded_min_edges=0, max_request_bytes=0):
+                 device=0, stroke_sizing=0, fill_banded_min_edges=0, max_request_bytes=0,
+                 stroke_split_windows=0, max_fill_crossings=0):
         cfg = B.ServerConfigC(pool_bytes, max_batch_paths, max_inflight, linger_us, device,
-                              stroke_sizing, fill_banded_min_edges, max_request_bytes)
+                              stroke_sizing, fill_banded_min_edges, max_request_bytes,
+                              stroke_split_windows, max_fill_crossings)
         self._h = B.lib().lyg_server_create(C.byref(cfg))
         if not self._h:
             raise RuntimeError("lyg_server_create failed (no usable HIP device or pool)")

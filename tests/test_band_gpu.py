@@ -42,6 +42,16 @@ def tri_area(v, i):
     return float(np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]).sum() / 2)
 
 
+def area_tol(path, exact):
+    """1e-6 of the area, or what rounding the region's boundary to f32 can move: its
+    length (at most the control polygon's) times half an ulp of the largest coordinate.
+    Shapes a few units across at coordinates in the thousands fall under the second."""
+    pts = np.asarray(path.points, np.float64)
+    perim = float(np.sum(np.hypot(*np.diff(pts, axis=0).T))) if len(pts) > 1 else 0.0
+    m = float(np.max(np.abs(pts))) if len(pts) else 1.0
+    return max(1e-6 * abs(exact), perim * 0.5 * float(np.spacing(np.float32(max(m, 1.0)))))
+
+
 def eps_for(points):
     m = float(np.max(np.abs(np.asarray(points, np.float64)))) if len(points) else 1.0
     return max(1e-4, float(np.spacing(np.float32(max(m, 1.0)))))
@@ -152,6 +162,51 @@ def test_batch_matches_single_paths():
         assert np.array_equal(v.view(np.uint32), sv.view(np.uint32)), k
         base = int(res.vertex_offsets[k])
         assert np.array_equal(idx - 7 - base, si), k
+
+
+@pytest.mark.parametrize("epb", [32, 256])
+def test_fill_workload_matches_lyon_coverage(epb):
+    """The batched fill workload's paths (stars with thousands of crossings, cubic loops)
+    through the batched banded fill at two band sizes: per path, coverage equal to the
+    winding number of the flattened edges on a 1024^2 grid (the first three on 4096^2),
+    area within 1e-6 of the exact area of the region (or, for small shapes far from the
+    origin, within what f32 vertices allow: area_tol), and coverage equal to lyon's
+    triangles wherever lyon's own triangles follow the winding number."""
+    from lyon_amd.batch import DeviceBatch, FillBatcher, set_band_size
+    fb = workloads.fill_batch(40, seed=0)
+    opts = workloads.fill_options(40, 0.05)
+    db = DeviceBatch(fb, opts)
+    set_band_size(epb, 16384)
+    try:
+        res = FillBatcher(banded=True).run(db)
+    finally:
+        set_band_size(256, 16384)
+    assert np.all(res.status == 0)
+    for k in range(40):
+        v, idx = res.path(k)
+        idx = idx - int(res.vertex_offsets[k])
+        p = fb.path(k)
+        rule = int(opts[k]["fill_rule"])
+        print(f"fill workload path {k}: {len(p.verbs)} verbs, rule {rule}, band size {epb}",
+              flush=True)
+        pts = np.asarray(p.points, np.float64)
+        lo, hi = pts.min(axis=0) - 1, pts.max(axis=0) + 1
+        box = (lo[0], lo[1], hi[0], hi[1])
+        n = 4096 if k < 3 else 1024
+        r = O.winding_compare(p, 0.05, rule, v, idx, box=box, n=n, eps=eps_for(p.points))
+        assert r["mismatch"] == 0 and r["overlap_a"] == 0, (k, r)
+        exact, _ = O.fill_area(p, 0.05, rule)
+        a = tri_area(v, idx)
+        tol = area_tol(p, exact)
+        assert abs(a - exact) <= tol, (k, a, exact, tol)
+        st, rv, ri = O.fill(p, 0.05, rule)
+        assert st == 0
+        # lyon's own f32 triangles are held to the same bound
+        assert abs(tri_area(rv, ri) - exact) <= tol, (k, tri_area(rv, ri), exact, tol)
+        c = O.coverage_compare(rv, ri, v, idx, box=box, n=n, eps=0.025)
+        lw = O.winding_compare(p, 0.05, rule, rv, ri, box=box, n=n, eps=0.025)
+        print(f"  lyon vs banded {c['mismatch']}, lyon vs winding {lw['mismatch']}", flush=True)
+        assert c["mismatch"] <= lw["mismatch"], (k, c, lw)
 
 
 def test_errors():
