@@ -733,24 +733,38 @@ struct Gap {
 // only when the triangles are written.  Two values stand for the band's top and bottom.
 static constexpr uint32_t TOP_EV = 0xffffffffu, BOT_EV = 0xfffffffeu;
 
+// Events are read through a window of EV_WIN consecutive events staged in LDS (one
+// coalesced load per lane), moved forward when the sweep nears its end.
+static constexpr uint32_t EV_WIN = 256;
+// Bands are swept by the variant with the smallest state that holds their edges, so more
+// waves fit a CU's LDS: up to M_SMALL, M_MID, M_MAX edges.
+static constexpr uint32_t M_SMALL = 512;
+static constexpr uint32_t M_MID = 2048;
+
+template <uint32_t M>
 struct SweepState {
-    uint16_t act[M_MAX];
-    uint16_t pos[M_MAX];
-    int16_t wsum[M_MAX];  // winding of the gap right of act[i]
-    uint32_t top[M_MAX];  // the event that gap opened at
+    uint16_t act[M];
+    uint16_t pos[M];
+    int16_t wsum[M];  // winding of the gap right of act[i]
+    uint32_t top[M];  // the event that gap opened at
     // one event's rebuild
     uint32_t nb[BLK_MAX];
     double nk[BLK_MAX];
     Gap og[BLK_MAX + 1];
+    // the event window
+    double ey[EV_WIN];
+    float ex[EV_WIN];
+    uint32_t ek[EV_WIN];
 };
 
-// One wave per band.
+// One wave per band; the variant M takes the bands of (M_lo, M] edges.
+template <uint32_t M, uint32_t M_LO>
 __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
     const uint32_t b = blockIdx.x;
     if (P.b_status[b] != 0) return;
     const uint32_t m = P.b_nedge[b];
-    if (m == 0) return;
-    __shared__ SweepState S;
+    if (m == 0 || m > M || m <= M_LO) return;
+    __shared__ SweepState<M> S;
     const int lane = lane_id();
     const uint32_t off = P.b_edge_off[b];
     const float4* G = P.bgeo + off;
@@ -764,8 +778,32 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
     bool overflow = false, block_err = false;
 
     auto wnd = [&](uint32_t local) -> int32_t { return (int32_t)W8[local]; };
+    // the event window: events [wb, wb + EV_WIN) in LDS (wave-uniform wb)
+    uint32_t wb = 0;
+    auto load_window = [&](uint32_t from) {
+        wave_sync();
+        wb = from;
+        for (uint32_t i = lane; i < EV_WIN; i += 64) {
+            const uint32_t e = from + i;
+            if (e < ne) {
+                S.ey[i] = P.ev_y[eoff + e];
+                S.ex[i] = P.ev_x[eoff + e];
+                S.ek[i] = P.ev_k[eoff + e];
+            }
+        }
+        wave_sync();
+    };
+    auto evy = [&](uint32_t e) -> double {
+        return e - wb < EV_WIN ? S.ey[e - wb] : P.ev_y[eoff + e];
+    };
+    auto evx = [&](uint32_t e) -> float {
+        return e - wb < EV_WIN ? S.ex[e - wb] : P.ev_x[eoff + e];
+    };
+    auto evk = [&](uint32_t e) -> uint32_t {
+        return e - wb < EV_WIN ? S.ek[e - wb] : P.ev_k[eoff + e];
+    };
     auto height = [&](uint32_t ev) -> double {
-        return ev == TOP_EV ? (double)ytop : (ev == BOT_EV ? (double)ybot : P.ev_y[eoff + ev]);
+        return ev == TOP_EV ? (double)ytop : (ev == BOT_EV ? (double)ybot : evy(ev));
     };
     auto emit = [&](uint32_t l, uint32_t r, uint32_t t, uint32_t bt) {
         // uniform call: lane 0 writes
@@ -803,22 +841,24 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
     wave_sync();
 
     uint32_t ei = 0;
+    if (ne) load_window(0);
     while (ei < ne && !block_err) {
+        // keep the window ahead of the sweep (uniform)
+        if (ei - wb > EV_WIN - 64 && wb + EV_WIN < ne) load_window(ei);
         // the event (a vertex: the ends and starts at one point; or one crossing)
-        const double y = P.ev_y[eoff + ei];
-        const float x = P.ev_x[eoff + ei];
-        const uint32_t k0 = P.ev_k[eoff + ei];
+        const double y = evy(ei);
+        const float x = evx(ei);
+        const uint32_t k0 = evk(ei);
         uint32_t ej = ei + 1;
         const bool is_cross = (k0 >> 30) == EV_CROSS;
         if (!is_cross) {
-            while (ej < ne && P.ev_y[eoff + ej] == y && P.ev_x[eoff + ej] == x &&
-                   (P.ev_k[eoff + ej] >> 30) != EV_CROSS)
+            while (ej < ne && evy(ej) == y && evx(ej) == x && (evk(ej) >> 30) != EV_CROSS)
                 ++ej;
         }
         // the next height strictly below
         double ynext = (double)ybot;
         for (uint32_t q = ej; q < ne; ++q) {
-            const double yq = P.ev_y[eoff + q];
+            const double yq = evy(q);
             if (yq > y) {
                 ynext = yq;
                 break;
@@ -842,7 +882,7 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
         } else {
             int32_t pmin = 0x7fffffff, pmax = -1;
             for (uint32_t q = ei; q < ej; ++q) {
-                const uint32_t k = P.ev_k[eoff + q];
+                const uint32_t k = evk(q);
                 if ((k >> 30) == EV_END) {
                     const uint32_t p = S.pos[k & 0xffffu];
                     if (p != NO_POS) {
@@ -903,7 +943,7 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
                 bool ends = false;
                 if (!is_cross)
                     for (uint32_t q = ei; q < ej; ++q) {
-                        const uint32_t k = P.ev_k[eoff + q];
+                        const uint32_t k = evk(q);
                         if ((k >> 30) == EV_END && (k & 0xffffu) == id) ends = true;
                     }
                 if (ends) continue;
@@ -912,7 +952,7 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
             }
             if (!is_cross)
                 for (uint32_t q = ei; q < ej; ++q) {
-                    const uint32_t k = P.ev_k[eoff + q];
+                    const uint32_t k = evk(q);
                     if ((k >> 30) == EV_START) {
                         if (nn < BLK_MAX && lane == 0) S.nb[nn] = k & 0xffffu;
                         ++nn;
@@ -1026,7 +1066,7 @@ __global__ void __launch_bounds__(64) sweep_kernel(Params P, const int8_t* bw) {
             // ends leave the active set
             if (!is_cross && lane == 0)
                 for (uint32_t q = ei; q < ej; ++q) {
-                    const uint32_t k = P.ev_k[eoff + q];
+                    const uint32_t k = evk(q);
                     if ((k >> 30) == EV_END) S.pos[k & 0xffffu] = NO_POS;
                 }
             wave_sync();
@@ -1460,9 +1500,19 @@ hipError_t lyg_band_sort(const Params* P, int8_t* bw, hipStream_t s) {
         hipLaunchKernelGGL(lyg::band::band_sort_kernel, dim3(P->n_bands), dim3(256), 0, s, *P, bw);
     return hipGetLastError();
 }
-hipError_t lyg_band_sweep(const Params* P, const int8_t* bw, hipStream_t s) {
-    if (P->n_bands)
-        hipLaunchKernelGGL(lyg::band::sweep_kernel, dim3(P->n_bands), dim3(64), 0, s, *P, bw);
+// Each band by the variant of its size class (the others return at once); the three
+// launches go to three streams so the classes run at once.
+hipError_t lyg_band_sweep(const Params* P, const int8_t* bw, hipStream_t s_small,
+                          hipStream_t s_mid, hipStream_t s_big) {
+    using namespace lyg::band;
+    if (P->n_bands) {
+        hipLaunchKernelGGL((sweep_kernel<M_MAX, M_MID>), dim3(P->n_bands), dim3(64), 0, s_big,
+                           *P, bw);
+        hipLaunchKernelGGL((sweep_kernel<M_MID, M_SMALL>), dim3(P->n_bands), dim3(64), 0, s_mid,
+                           *P, bw);
+        hipLaunchKernelGGL((sweep_kernel<M_SMALL, 0>), dim3(P->n_bands), dim3(64), 0, s_small,
+                           *P, bw);
+    }
     return hipGetLastError();
 }
 hipError_t lyg_band_path_offsets(const Params* P, uint32_t* path_edge_off, uint32_t* p_hbins,

@@ -28,7 +28,8 @@ hipError_t lyg_band_sort(const BandParams* P, int8_t* bw, hipStream_t s);
 hipError_t lyg_band_cross(const BandParams* P, int emit, hipStream_t s);
 hipError_t lyg_band_event_count(const BandParams* P, hipStream_t s);
 hipError_t lyg_band_event_sort(const BandParams* P, void* scratch, hipStream_t s);
-hipError_t lyg_band_sweep(const BandParams* P, const int8_t* bw, hipStream_t s);
+hipError_t lyg_band_sweep(const BandParams* P, const int8_t* bw, hipStream_t s_small,
+                          hipStream_t s_mid, hipStream_t s_big);
 hipError_t lyg_band_expand_count(const BandParams* P, hipStream_t s);
 hipError_t lyg_band_expand_write(const BandParams* P, hipStream_t s);
 hipError_t lyg_band_path_offsets(const BandParams* P, uint32_t* path_edge_off, uint32_t* p_hbins,
@@ -287,7 +288,28 @@ int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc,
         P.n_records = (uint32_t)nrec;
         if (!zero(c, B.b_nrec, nb)) return LYG_ERR_DEVICE;
         if (c->timing) (void)hipEventRecord(c->ev[2], s);
-        if (!check(c, lyg_band_sweep(&P, as<int8_t>(B.bw), s), "band sweep")) return LYG_ERR_DEVICE;
+        // the sweep's three size classes on the context's stream and two side streams
+        // (default priority), joined by events
+        for (int k = 0; k < 2; ++k)
+            if (!c->class_stream[6 + k] &&
+                !check(c, hipStreamCreateWithFlags(&c->class_stream[6 + k], hipStreamNonBlocking),
+                       "hipStreamCreate"))
+                return LYG_ERR_DEVICE;
+        for (int k = 0; k < 3; ++k)
+            if (!c->class_ev[6 + k] &&
+                !check(c, hipEventCreateWithFlags(&c->class_ev[6 + k], hipEventDisableTiming),
+                       "hipEventCreate"))
+                return LYG_ERR_DEVICE;
+        hipStream_t s1 = c->class_stream[6], s2 = c->class_stream[7];
+        if (!check(c, hipEventRecord(c->class_ev[6], s), "event") ||
+            !check(c, hipStreamWaitEvent(s1, c->class_ev[6], 0), "wait") ||
+            !check(c, hipStreamWaitEvent(s2, c->class_ev[6], 0), "wait") ||
+            !check(c, lyg_band_sweep(&P, as<int8_t>(B.bw), s, s1, s2), "band sweep") ||
+            !check(c, hipEventRecord(c->class_ev[7], s1), "event") ||
+            !check(c, hipEventRecord(c->class_ev[8], s2), "event") ||
+            !check(c, hipStreamWaitEvent(s, c->class_ev[7], 0), "wait") ||
+            !check(c, hipStreamWaitEvent(s, c->class_ev[8], 0), "wait"))
+            return LYG_ERR_DEVICE;
         if (c->timing) (void)hipEventRecord(c->ev[3], s);
         // any band out of room?
         std::vector<uint32_t> bst(nb);
