@@ -342,13 +342,16 @@ void lyg_graph_destroy(lyg_graph* graph);
 
 /* ---- autotuner (no lyon counterpart) -----------------------------------------------------
  * Runs the batch under each candidate launch shape (stroke: resident blocks per CU of the
- * persistent grid; fill: waves per block and LDS bytes per wave for the sweep's hot
- * arrays), keeps the fastest in the context for later calls and reports it. */
+ * persistent grid, then windows per part of split paths; fill: waves per block and LDS
+ * bytes per wave for the sweep's hot arrays; banded fill: edges per band), keeps the
+ * fastest in the context for later calls and reports it. */
 typedef struct lyg_tuning {
     uint32_t blocks_per_cu;
     uint32_t waves_per_block;
     uint32_t lds_bytes_per_wave;
     float best_ms;
+    uint32_t split_windows;    /* stroke: windows per part (lyg_ctx_set_stroke_split) */
+    uint32_t edges_per_band;   /* banded fill (lyg_ctx_set_band_size) */
 } lyg_tuning;
 /* Sets the launch shapes explicitly (either pointer may be null), e.g. a tuning kept from an
  * earlier run. */
@@ -371,6 +374,8 @@ int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stro
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
                           uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
+int32_t lyg_autotune_banded(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
+                            uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 
 /* ---- serving loop (no lyon counterpart) -------------------------------------------------
  * Continuous batching of single-path requests on one GPU.  Requests (host paths, copied at

@@ -87,7 +87,8 @@ class StreamStatsC(C.Structure):
 
 class TuningC(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_uint32), ("waves_per_block", C.c_uint32),
-                ("lds_bytes_per_wave", C.c_uint32), ("best_ms", C.c_float)]
+                ("lds_bytes_per_wave", C.c_uint32), ("best_ms", C.c_float),
+                ("split_windows", C.c_uint32), ("edges_per_band", C.c_uint32)]
 
 
 # Every symbol include/lyg.h declares, with its ctypes signature.
@@ -166,6 +167,8 @@ SIGNATURES = {
                              C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
     "lyg_autotune_fill": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                            C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
+    "lyg_autotune_banded": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                             C.POINTER(BatchOutC), C.POINTER(TuningC)], C.c_int32),
     "lyg_server_create": ([C.POINTER(ServerConfigC)], C.c_void_p),
     "lyg_server_destroy": ([C.c_void_p], None),
     "lyg_server_submit_fill": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),

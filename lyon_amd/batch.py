@@ -204,6 +204,7 @@ class FillBatcher(_Batcher):
 
     def __init__(self, banded: bool = False):
         super().__init__()
+        self.banded = banded
         if banded:
             self._entry = "lyg_fill_batch_banded"
 
@@ -276,16 +277,25 @@ class StrokeGraph:
 
 
 def autotune(batcher: _Batcher, db: DeviceBatch) -> dict:
-    """Times db under each candidate launch shape (lyg_autotune_stroke / _fill) and keeps
-    the fastest in the context; returns the chosen shape."""
+    """Times db under each candidate launch shape (lyg_autotune_stroke / _fill / _banded)
+    and keeps the fastest in the context; returns the chosen shape (stroke: blocks per CU
+    and windows per part of split paths; fill: waves per block and LDS bytes per wave;
+    banded fill: edges per band)."""
+    if getattr(batcher, "banded", False):
+        # the smallest band size makes the most triangles: size the arena for it
+        B.lib().lyg_ctx_set_band_size(_Ctx.get(), 16, 16384)
     res = batcher.run(db, check=False)  # sizes the output arena
     out = B.BatchOutC(batcher.vertices.ptr, batcher.indices.ptr, batcher.vertices.nbytes // 8,
                       batcher.indices.nbytes // 4, batcher.voffs.ptr, batcher.ioffs.ptr, 0, 0, 0,
                       batcher.status.ptr)
     cin = db.c_in()
     t = B.TuningC()
-    fn = B.lib().lyg_autotune_stroke if isinstance(batcher, StrokeBatcher) else \
-        B.lib().lyg_autotune_fill
+    if isinstance(batcher, StrokeBatcher):
+        fn = B.lib().lyg_autotune_stroke
+    elif getattr(batcher, "banded", False):
+        fn = B.lib().lyg_autotune_banded
+    else:
+        fn = B.lib().lyg_autotune_fill
     st = fn(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts, C.byref(out), C.byref(t))
     if st != 0:
         raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())

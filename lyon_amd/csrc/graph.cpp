@@ -11,6 +11,8 @@
 // and keeps the fastest in the context.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <vector>
 
 #include "ctx.h"
@@ -199,14 +201,88 @@ int32_t lyg_autotune_stroke(lyg_ctx* c, const lyg_batch_in* in, const lyg_stroke
             best_b = b;
         }
     }
+    c->stroke_blocks_per_cu = st == LYG_OK ? best_b : keep;
+    // then the parts long paths are cut into (0: whole paths)
+    static const uint32_t split_cand[] = {0, 16, 32, 64, 128};
+    const uint32_t keep_w = c->stroke_split_windows;
+    uint32_t best_w = keep_w;
+    float best_split = 1e30f;
+    for (uint32_t w : split_cand) {
+        if (st != LYG_OK) break;
+        c->stroke_split_windows = w;
+        float ms = 0;
+        for (int rep = 0; rep < 2 && st == LYG_OK; ++rep) {
+            (void)hipEventRecord(e0, c->stream);
+            st = lyg_stroke_batch(c, in, opts, n_opts, out);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (st == LYG_OK && ms < best_split) {
+            best_split = ms;
+            best_w = w;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->stroke_blocks_per_cu = st == LYG_OK ? best_b : keep;
+    c->stroke_split_windows = st == LYG_OK ? best_w : keep_w;
     if (chosen) {
         chosen->blocks_per_cu = c->stroke_blocks_per_cu;
         chosen->waves_per_block = 4;
         chosen->lds_bytes_per_wave = 0;
+        chosen->best_ms = std::min(best, best_split);
+        chosen->split_windows = c->stroke_split_windows;
+        chosen->edges_per_band = c->band_edges_per_band;
+    }
+    return st;
+}
+
+int32_t lyg_autotune_banded(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_options* opts,
+                            uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen) {
+    if (!c || !in || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
+    static const uint32_t cand[] = {16, 32, 64, 128, 256};
+    hipEvent_t e0, e1;
+    if (!check(c, hipEventCreate(&e0), "event") || !check(c, hipEventCreate(&e1), "event"))
+        return LYG_ERR_DEVICE;
+    const uint32_t keep = c->band_edges_per_band;
+    float best = 1e30f;
+    uint32_t best_e = keep;
+    int32_t st = LYG_OK;
+    for (uint32_t e : cand) {
+        c->band_edges_per_band = e;
+        float ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {  // first run warms up
+            (void)hipEventRecord(e0, c->stream);
+            st = lyg_fill_batch_banded(c, in, opts, n_opts, out);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        // per-path failures (a path's own status) do not disqualify a band size; one
+        // whose output does not fit the caller's arrays is skipped
+        if (st == LYG_ERR_CAPACITY) {
+            st = LYG_OK;
+            continue;
+        }
+        if (st != LYG_OK && st != LYG_ERR_INTERNAL && st != LYG_ERR_POSITION_IS_NAN &&
+            st != LYG_ERR_TOLERANCE_IS_NAN)
+            break;
+        st = LYG_OK;
+        if (ms < best) {
+            best = ms;
+            best_e = e;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->band_edges_per_band = st == LYG_OK ? best_e : keep;
+    if (chosen) {
+        chosen->blocks_per_cu = 0;
+        chosen->waves_per_block = 1;
+        chosen->lds_bytes_per_wave = 0;
         chosen->best_ms = best;
+        chosen->split_windows = c->stroke_split_windows;
+        chosen->edges_per_band = c->band_edges_per_band;
     }
     return st;
 }
@@ -256,6 +332,8 @@ int32_t lyg_autotune_fill(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_opt
         chosen->waves_per_block = c->fill_waves_per_block;
         chosen->lds_bytes_per_wave = c->fill_hot_bytes;
         chosen->best_ms = best;
+        chosen->split_windows = c->stroke_split_windows;
+        chosen->edges_per_band = c->band_edges_per_band;
     }
     return st;
 }

@@ -55,6 +55,7 @@ def test_autotune_keeps_results():
     shape = autotune(b, db)
     print("stroke shape", shape, flush=True)
     assert shape["blocks_per_cu"] in (2, 4, 6, 8, 12, 16)
+    assert shape["split_windows"] in (0, 16, 32, 64, 128)
     assert same(snapshot(b.run(db)), ref)
 
     fbatch = workloads.fill_batch(64, seed=7)
@@ -65,3 +66,37 @@ def test_autotune_keeps_results():
     print("fill shape", fshape, flush=True)
     assert fshape["waves_per_block"] in (1, 2, 4)
     assert same(snapshot(fb.run(fdb, check=False)), fref)
+
+
+def test_autotune_band_size_keeps_the_region():
+    """The banded fill's band size changes its triangles, never the region: after the
+    autotuner picks one, every path covers the same area (relative 1e-6 of the exact
+    area, or the f32 bound for small shapes far from the origin)."""
+    import numpy as np
+    from lyon_amd.batch import set_band_size
+    from oracle import pyoracle as O
+    fbatch = workloads.fill_batch(48, seed=11)
+    fopts = workloads.fill_options(48, seed=11)
+    fdb = DeviceBatch(fbatch, fopts)
+    fb = FillBatcher(banded=True)
+    try:
+        shape = autotune(fb, fdb)
+        print("banded shape", shape, flush=True)
+        assert shape["edges_per_band"] in (16, 32, 64, 128, 256)
+        res = fb.run(fdb, check=False)
+        assert np.all(res.status == 0)
+        for k in range(48):
+            v, idx = res.path(k)
+            idx = idx - int(res.vertex_offsets[k])
+            t = np.asarray(idx, np.int64).reshape(-1, 3)
+            vv = np.asarray(v, np.float64)
+            a, b, c = vv[t[:, 0]], vv[t[:, 1]], vv[t[:, 2]]
+            area = float(np.abs((b - a)[:, 0] * (c - a)[:, 1] - (b - a)[:, 1] * (c - a)[:, 0]).sum() / 2)
+            p = fbatch.path(k)
+            exact, _ = O.fill_area(p, 0.05, int(fopts[k]["fill_rule"]))
+            pts = np.asarray(p.points, np.float64)
+            perim = float(np.sum(np.hypot(*np.diff(pts, axis=0).T)))
+            tol = max(1e-6 * exact, perim * 0.5 * float(np.spacing(np.float32(pts.max()))))
+            assert abs(area - exact) <= tol, (k, area, exact, tol)
+    finally:
+        set_band_size(256, 16384)
