@@ -370,6 +370,10 @@ int32_t lyg_ctx_set_stroke_staging(lyg_ctx* ctx, int enabled, float room_scale);
  * whole-path walk.  Default 64 (parts of 4,096 steps); 0: every path is walked by one
  * wave. */
 int32_t lyg_ctx_set_stroke_split(lyg_ctx* ctx, uint32_t windows_per_part);
+/* Of the last split batched stroke on this context: the items walked (whole paths and
+ * parts) and the paths walked again whole because a part's prologue did not rebuild the
+ * state or a part outgrew its room (no lyon counterpart). */
+int32_t lyg_ctx_stroke_split_stats(lyg_ctx* ctx, uint64_t* items, uint32_t* walked_again);
 int32_t lyg_autotune_stroke(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_options* opts,
                             uint32_t n_opts, lyg_batch_out* out, lyg_tuning* chosen);
 int32_t lyg_autotune_fill(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,

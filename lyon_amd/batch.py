@@ -325,6 +325,14 @@ def set_stroke_split(windows_per_part: int = 64):
         raise TessellationError(st)
 
 
+def stroke_split_stats() -> dict:
+    """Of the last split batched stroke: items walked and paths walked again whole."""
+    items = C.c_uint64(0)
+    again = C.c_uint32(0)
+    B.lib().lyg_ctx_stroke_split_stats(_Ctx.get(), C.byref(items), C.byref(again))
+    return {"items": items.value, "walked_again": again.value}
+
+
 def set_stroke_staging(enabled: bool = True, room_scale: float = 1.0):
     """How batched strokes size their output (lyg_ctx_set_stroke_staging): one emit pass
     into per-path staging room then a compaction copy (default), or count + emit passes.

@@ -47,6 +47,8 @@ struct lyg_ctx {
     DevBuf st_cap_v, st_cap_i, st_off_v, st_off_i, st_v, st_i, st_fallback;
     // stroke: long paths split into parts of this many 64-step windows (0: whole paths)
     uint32_t stroke_split_windows = 64;
+    uint64_t last_stroke_items = 0;       // items of the last split stroke batch
+    uint32_t last_stroke_again = 0;       // its paths walked again whole
     DevBuf st_items, st_heads, st_spec, st_end, st_parts, st_parts_off;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};

@@ -28,7 +28,7 @@ hipError_t lyg_launch_stroke_cut(const lyg::StrokeLaunch* L, uint32_t n_paths,
                                  uint32_t split_windows, float room_scale, const uint32_t* extra,
                                  const uint32_t* extra_off, const uint32_t* rv,
                                  const uint32_t* ri, const uint32_t* v_off, const uint32_t* i_off,
-                                 int grid, hipStream_t s);
+                                 uint32_t cap_extra, uint32_t* err, int grid, hipStream_t s);
 hipError_t lyg_launch_stroke_items_verify(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
                                           hipStream_t s);
 hipError_t lyg_launch_stroke_heads(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,
@@ -317,7 +317,8 @@ int32_t stroke_batch_items(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     L.item_end = as<lyg::CarryRec>(c->st_end);
     // the items, and the first points of split paths
     if (!check(c, lyg_launch_stroke_cut(&L, n, c->stroke_split_windows, c->stroke_room_scale,
-                                        parts, parts_off, rv, ri, voff, ioff, pgrid, s),
+                                        parts, parts_off, rv, ri, voff, ioff,
+                                        (uint32_t)n_extra, nullptr, pgrid, s),
                "stroke cut") ||
         !check(c, lyg_launch_stroke_heads(&L, n, pgrid, s), "stroke heads"))
         return LYG_ERR_DEVICE;
@@ -374,6 +375,8 @@ int32_t stroke_batch_items(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
     C.n_paths = n;
     C.fallback = as<uint32_t>(c->st_fallback);
     C.n_fallback = n_fb;
+    C.out_vcap = tv;
+    C.out_icap = ti;
     uint32_t* hf = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->h_pinned) + 64);
     if (!check(c, lyg_launch_fill_u32(n_fb, 1, 0, s), "reset") ||
         !check(c, lyg_launch_stroke_items_compact(&C, L.items, L.heads, L.n_items,
@@ -411,6 +414,8 @@ int32_t stroke_batch_items(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::Out
             return LYG_ERR_INTERNAL;
         }
     }
+    c->last_stroke_items = n_items;
+    c->last_stroke_again = *hf;
     if (trace_on()) {
         std::vector<lyg::StrokeHeads> hh(n);
         (void)hipMemcpy(hh.data(), L.heads, sizeof(lyg::StrokeHeads) * n, hipMemcpyDeviceToHost);

@@ -91,6 +91,11 @@ struct StrokeLaunch {
     // split walks (null items: one wave walks each whole path)
     StrokeItem* items;          // n_paths + extra items; the work queue runs over n_items
     uint32_t n_items;
+    // graph replay: the item count is n_paths + min(*n_extra_dev, cap_extra), and a
+    // path-list walk takes its length from *n_list_dev (null: the host values above)
+    const unsigned long long* n_extra_dev;
+    uint32_t cap_extra;
+    const uint32_t* n_list_dev;
     StrokeHeads* heads;         // per path
     CarryRec* item_spec;        // per item: the carry its prologue rebuilt
     CarryRec* item_end;         // per item: the carry at its end
@@ -113,6 +118,12 @@ struct StrokeCompact {
     uint32_t n_paths;
     uint32_t* fallback;          // paths that outgrew their staging room
     uint32_t* n_fallback;
+    // split walks under graph replay: the number of items is on the device, and items
+    // that would land beyond the output arrays are skipped and flagged (err bit 2)
+    const unsigned long long* n_extra_dev;
+    uint32_t cap_extra;
+    uint64_t out_vcap, out_icap;
+    uint32_t* err;
 };
 
 namespace fillk {

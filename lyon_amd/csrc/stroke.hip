@@ -854,14 +854,19 @@ __global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK)
     const int l = lane_id();
     uint32_t pid = next_path(L.work_counter);
     if constexpr (ITEMS) {
-        // parts of split paths and whole paths, as listed by stroke_items_kernel
-        while (pid < L.n_items) {
+        // parts of split paths and whole paths, as listed by stroke_cut_kernel
+        const uint32_t n_work =
+            L.n_extra_dev ? L.in.n_paths + (uint32_t)min(*L.n_extra_dev,
+                                                          (unsigned long long)L.cap_extra)
+                          : L.n_items;
+        while (pid < n_work) {
             stroke_item<EMIT>(L, pid, recs, arc_jobs, arc_first, arc_c0);
             pid = next_path(L.work_counter);
         }
         return;
     }
-    while (pid < L.in.n_paths) {
+    const uint32_t n_list = L.n_list_dev ? *L.n_list_dev : L.in.n_paths;
+    while (pid < n_list) {
         const uint32_t gp = L.path_list ? L.path_list[pid] : L.in.first_path + pid;
         uint32_t nv = 0, ni = 0;
         stroke_one_path<EMIT>(L, gp, recs, arc_jobs, arc_first, arc_c0, &nv, &ni);
@@ -1175,10 +1180,17 @@ __global__ void __launch_bounds__(WAVE) stroke_cut_kernel(StrokeLaunch L, uint32
                                                           const uint32_t* extra_off,
                                                           const uint32_t* rv, const uint32_t* ri,
                                                           const uint32_t* v_off,
-                                                          const uint32_t* i_off) {
+                                                          const uint32_t* i_off,
+                                                          uint32_t cap_extra, uint32_t* err) {
     for (uint32_t path = blockIdx.x; path < n_paths; path += gridDim.x) {
-        const uint32_t e = extra[path];
+        uint32_t e = extra[path];
         const uint32_t ef = n_paths + extra_off[path];
+        if (e && extra_off[path] + e > cap_extra) {
+            // beyond the item arrays (graph replay with more parts than captured): the
+            // path is not cut; the replay reports it
+            if (lane_id() == 0 && err) atomicOr(err, 4u);
+            e = 0;
+        }
         uint32_t v = rv[path], i = ri[path];
         cut_path<false>(L, path, split_windows, room_scale, nullptr, &v, &i, e, ef, v_off[path],
                         i_off[path]);
@@ -1305,6 +1317,8 @@ __global__ void __launch_bounds__(256) stroke_items_compact_kernel(StrokeCompact
                                                                    const StrokeItem* items,
                                                                    const StrokeHeads* heads,
                                                                    uint32_t n_items) {
+    if (C.n_extra_dev)
+        n_items = C.n_paths + (uint32_t)min(*C.n_extra_dev, (unsigned long long)C.cap_extra);
     for (uint32_t q = blockIdx.x; q < n_items; q += gridDim.x) {
         const StrokeItem it = items[q];
         const uint32_t p = it.path;
@@ -1314,6 +1328,10 @@ __global__ void __launch_bounds__(256) stroke_items_compact_kernel(StrokeCompact
             continue;
         }
         const uint32_t v0 = C.vtx_off[p] + it.vbase, i0 = C.idx_off[p] + it.ibase;
+        if ((uint64_t)v0 + it.nv > C.out_vcap || (uint64_t)i0 + it.ni > C.out_icap) {
+            if (threadIdx.x == 0 && C.err) atomicOr(C.err, 2u);
+            continue;
+        }
         const uint2* __restrict__ sv = reinterpret_cast<const uint2*>(C.stage_v) + it.v_off;
         uint2* __restrict__ dv = reinterpret_cast<uint2*>(C.out_vertices) + v0;
         const uint32_t nv = it.nv, ni = it.ni;
@@ -1409,10 +1427,11 @@ hipError_t lyg_launch_stroke_cut(const lyg::StrokeLaunch* L, uint32_t n_paths,
                                  uint32_t split_windows, float room_scale, const uint32_t* extra,
                                  const uint32_t* extra_off, const uint32_t* rv,
                                  const uint32_t* ri, const uint32_t* v_off, const uint32_t* i_off,
-                                 int grid, hipStream_t s) {
+                                 uint32_t cap_extra, uint32_t* err, int grid, hipStream_t s) {
     if (n_paths == 0) return hipSuccess;
     hipLaunchKernelGGL(lyg::stroke_cut_kernel, dim3(grid), dim3(lyg::WAVE), 0, s, *L, n_paths,
-                       split_windows, room_scale, extra, extra_off, rv, ri, v_off, i_off);
+                       split_windows, room_scale, extra, extra_off, rv, ri, v_off, i_off,
+                       cap_extra, err);
     return hipGetLastError();
 }
 hipError_t lyg_launch_stroke_heads(const lyg::StrokeLaunch* L, uint32_t n_paths, int grid,

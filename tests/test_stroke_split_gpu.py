@@ -12,11 +12,15 @@ from lyon_amd.path import PathBatch, PathBuilder
 pytestmark = pytest.mark.gpu
 
 
+LAST = {}
+
+
 def run(db, split, vertex_offset=0):
-    from lyon_amd.batch import StrokeBatcher, set_stroke_split
+    from lyon_amd.batch import StrokeBatcher, set_stroke_split, stroke_split_stats
     set_stroke_split(split)
     try:
         r = StrokeBatcher().run(db, vertex_offset=vertex_offset)
+        LAST.update(stroke_split_stats())
     finally:
         set_stroke_split(64)
     return (r.vertex_offsets.copy(), r.index_offsets.copy(),
@@ -42,6 +46,7 @@ def test_split_matches_whole_walk(closed):
     for w in (1, 3, 16):
         print(f"split into parts of {w} windows, closed={closed}", flush=True)
         same(ref, run(db, w, vertex_offset=5), f"split {w}")
+        assert LAST["items"] > len(batch), "no path was split"
 
 
 def test_split_with_short_paths_mixed_in():
@@ -61,20 +66,18 @@ def test_split_with_short_paths_mixed_in():
         same(ref, run(db, w), f"split {w}")
 
 
-def test_split_where_prologues_cannot_rebuild_the_state():
-    """Runs of points closer than the keep threshold (tolerance^2 / 2) across many part
-    boundaries: a part's one-window prologue then starts inside a run of dropped points, so
-    some checks fail and those paths are walked again whole; the bytes stay the same."""
-    from lyon_amd.batch import DeviceBatch
-    rng = np.random.default_rng(7)
+def near_duplicate_paths(n=6, steps=3000, seed=7):
+    rng = np.random.default_rng(seed)
     paths = []
-    for k in range(6):
+    for k in range(n):
         b = PathBuilder()
         x, y = 100.0, 100.0
         b.begin((x, y))
-        for i in range(3000):
-            if (i // 40) % 2 == 0:
-                # a run of tiny steps: most points are dropped by the keep filter
+        for i in range(steps):
+            if i % 340 < 300:
+                # runs of 300 tiny steps, longer than a part's prologue: the keep filter
+                # drops them against the run's first point, which a prologue starting
+                # inside the run cannot know
                 x += float(rng.uniform(-0.004, 0.004))
                 y += float(rng.uniform(-0.004, 0.004))
             else:
@@ -83,10 +86,48 @@ def test_split_where_prologues_cannot_rebuild_the_state():
             b.line_to((x, y))
         b.end(k % 2 == 1)
         paths.append(b.build())
-    batch = PathBatch.from_paths(paths)
+    return paths
+
+
+def test_split_where_prologues_cannot_rebuild_the_state():
+    """Runs of points closer than the keep threshold (tolerance^2 / 2) across many part
+    boundaries: a part's prologue then starts inside a run of dropped points, so some
+    checks fail and those paths are walked again whole; the bytes stay the same."""
+    from lyon_amd.batch import DeviceBatch
+    batch = PathBatch.from_paths(near_duplicate_paths())
     opts = workloads.stroke_options(len(batch), tolerance=0.1, seed=1)
     db = DeviceBatch(batch, opts)
     ref = run(db, 0)
+    again = 0
     for w in (1, 2, 5):
         print(f"near-duplicate points, parts of {w} windows", flush=True)
         same(ref, run(db, w), f"split {w}")
+        print(f"  {LAST}", flush=True)
+        again += LAST["walked_again"]
+    assert again > 0, "no path was walked again: the check was not exercised"
+
+
+def test_graph_replays_of_the_split_walk():
+    """The split walk captured in a HIP graph (lyg_stroke_batch_graph with splitting on),
+    replayed three times with eager work and synchronisations between: every replay gives
+    the eager whole-path bytes, on long paths and on paths whose prologues fail the check
+    (walked again by the graph's second-walk kernel, whose path count is on the device)."""
+    from lyon_amd.batch import DeviceBatch, StrokeBatcher, StrokeGraph, set_stroke_split
+    longs = long_paths(12, 21, False)
+    batch = PathBatch.from_paths([longs.path(i) for i in range(12)] + near_duplicate_paths(4))
+    opts = workloads.stroke_options(len(batch), tolerance=0.1, seed=3)
+    db = DeviceBatch(batch, opts)
+    ref = run(db, 0, vertex_offset=11)
+    set_stroke_split(2)
+    try:
+        g = StrokeGraph(db, int(ref[0][-1]) + 64, int(ref[1][-1]) + 192, vertex_offset=11)
+        for rep in range(3):
+            g.launch()
+            r = g.result()
+            got = (r.vertex_offsets.copy(), r.index_offsets.copy(),
+                   r.all_vertices().view(np.uint32).copy(), r.all_indices().copy())
+            same(ref, got, f"graph replay {rep}")
+            # eager work between replays
+            StrokeBatcher().run(db)
+    finally:
+        set_stroke_split(64)
