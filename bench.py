@@ -333,7 +333,8 @@ def main():
                 "kernel_ms": {"count_scan_avg": float(np.mean(fl_count)), "emit_avg": fl_emit_ms},
                 "roofline": {"bound": "hbm", "achieved": fl_ach, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": fl_ach / HBM_PEAK_GBS,
-                             "kernel": "flatten_emit_runs_kernel", "algorithmic_bytes": fl_bytes}}
+                             "kernel": "flatten_runs_table_kernel + flatten_emit_table_kernel",
+                             "algorithmic_bytes": fl_bytes}}
         del dk, dc, doff, dout
 
     # ---- path validators (lyon_algorithms hit_test / area) -----------------------------

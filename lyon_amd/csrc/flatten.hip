@@ -2,12 +2,14 @@
 //
 // Segments (LineSegment / QuadraticBezierSegment / CubicBezierSegment): a count kernel
 // evaluates Wang's formula per segment (one lane each), the host-side runtime scans the
-// counts, and an emit kernel gives every lane a run of 16 consecutive output points: the
-// lane finds the segment of its first point in an LDS window of segment offsets, recovers
-// that point's t with the closed-form serial chain (lyg_math.h: tchain) and continues
-// lyon's serial sum from there, evaluating the polynomial form with Horner's rule in
-// lyon's operation order (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).  The
-// points pass through LDS so that the stores are fully coalesced.
+// counts, a table kernel walks each segment's serial t sum once and records, for every run
+// of 16 consecutive output points, the segment and t of its first point, and an emit
+// kernel gives every lane one run: it starts from the table and continues lyon's serial
+// sum, evaluating the polynomial form with Horner's rule in lyon's operation order
+// (quadratic_bezier.rs:891-938, cubic_bezier.rs:1318-1364).  The points pass through LDS
+// so that the stores are fully coalesced.  Without the table (the allocation-free entry
+// point) the emit finds the segment of its run in an LDS window of segment offsets and
+// recovers t with the closed-form serial chain (lyg_math.h: tchain).
 //
 // Arcs (Arc::for_each_flattened_with_t, arc.rs:339-368) re-split the remaining arc at
 // every step, a serial recurrence, so one lane walks one arc.
@@ -70,8 +72,8 @@ __global__ void flatten_count_kernel(FlattenLaunch L) {
 
 static constexpr uint32_t FL_WAVES = 4;  // waves per block
 
-// Emit: each wave covers 1,024 consecutive output points; lane l owns points
-// [16 l, 16 l + 16) of them.  The wave first loads the offsets of every segment its
+// Emit without the run table: each wave covers 1,024 consecutive output points; lane l owns
+// points [16 l, 16 l + 16) of them.  The wave first loads the offsets of every segment its
 // points touch into LDS (each segment has at least one point, so at most 1,025 offsets);
 // a lane finds the segment of its first point there, takes t from the closed form (or the
 // serial sum near the segment's start) once, then adds the step once per point, which is
@@ -79,6 +81,37 @@ static constexpr uint32_t FL_WAVES = 4;  // waves per block
 // 64 consecutive points.
 static constexpr uint32_t FR_RUN = 16;
 static constexpr uint32_t FR_WIN = WAVE * FR_RUN + 1;
+
+// Output points are written once and not read again by the kernel: nontemporal stores
+// keep them from displacing the inputs in the caches (measured 1.46x faster than plain
+// stores on the emit).
+typedef float lyg_f2v __attribute__((ext_vector_type(2)));
+typedef float lyg_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_stream(float2* p, float2 v) {
+    const lyg_f2v x = {v.x, v.y};
+    __builtin_nontemporal_store(x, reinterpret_cast<lyg_f2v*>(p));
+}
+__device__ __forceinline__ void store_stream(float4* p, float4 v) {
+    const lyg_f4v x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<lyg_f4v*>(p));
+}
+
+// Point k of a segment (0-based among its emitted points) at t = the sum of k + 1 steps:
+// the polynomial form evaluated as lyon's for_each_flattened_with_t evaluates the curve,
+// the last point being the end point itself.
+__device__ __forceinline__ vec2 run_point(const SegPoly& sp, uint32_t k, float t) {
+    if (k + 1 >= sp.n) return sp.to;
+    vec2 v = sp.a0;
+    float tt = t;
+    v = add(v, mul(sp.a1, tt));
+    tt = tt * t;
+    v = add(v, mul(sp.a2, tt));
+    if (sp.cubic) {
+        tt = tt * t;
+        v = add(v, mul(sp.a3, tt));
+    }
+    return v;
+}
 
 __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_runs_kernel(FlattenLaunch L,
                                                                            uint32_t total) {
@@ -144,21 +177,7 @@ __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_runs_kernel(Flatt
             k = 0;
             t = 0.0f + sp.step;
         }
-        vec2 p;
-        if (k + 1 >= sp.n) {
-            p = sp.to;
-        } else {
-            vec2 v = sp.a0;
-            float tt = t;
-            v = add(v, mul(sp.a1, tt));
-            tt = tt * t;
-            v = add(v, mul(sp.a2, tt));
-            if (sp.cubic) {
-                tt = tt * t;
-                v = add(v, mul(sp.a3, tt));
-            }
-            p = v;
-        }
+        const vec2 p = run_point(sp, k, t);
         st[g - (uint32_t)g0] = make_float2(p.x, p.y);
         ++k;
         t = t + sp.step;
@@ -171,7 +190,121 @@ __global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_runs_kernel(Flatt
 #pragma unroll
     for (uint32_t r = 0; r < FR_RUN; ++r) {
         const uint32_t q = r * WAVE + (uint32_t)l;  // point base + q
-        if (base + q < wend) out[base + q] = stage[(q / FR_RUN) * (FR_RUN + 1) + q % FR_RUN];
+        if (base + q < wend)
+            store_stream(out + base + q, stage[(q / FR_RUN) * (FR_RUN + 1) + q % FR_RUN]);
+    }
+}
+
+// The run table: for every run of 16 output points, the segment of its first point and
+// that point's t by lyon's own serial sum.  One lane walks one segment (t = t + step per
+// point, the point k of a segment taking the sum of k + 1 steps) and writes the entry of
+// every point whose index is a multiple of 16; a segment's last point takes its end point,
+// so its t is never read.  The same pass stores every segment's polynomial form, step and
+// count (48 bytes), so the emit loads them instead of evaluating Wang's formula again.
+__device__ __forceinline__ void poly_store(float4* dst, const SegPoly& sp) {
+    dst[0] = make_float4(sp.a0.x, sp.a0.y, sp.a1.x, sp.a1.y);
+    dst[1] = make_float4(sp.a2.x, sp.a2.y, sp.a3.x, sp.a3.y);
+    dst[2] = make_float4(sp.to.x, sp.to.y, sp.step, u2f(sp.n | (sp.cubic ? 0x80000000u : 0u)));
+}
+__device__ __forceinline__ SegPoly poly_load(const float4* src) {
+    const float4 a = src[0], b = src[1], c = src[2];
+    SegPoly sp;
+    sp.a0 = mk(a.x, a.y);
+    sp.a1 = mk(a.z, a.w);
+    sp.a2 = mk(b.x, b.y);
+    sp.a3 = mk(b.z, b.w);
+    sp.to = mk(c.x, c.y);
+    sp.step = c.z;
+    const uint32_t nc = f2u(c.w);
+    sp.n = nc & 0x7fffffffu;
+    sp.cubic = (nc >> 31) != 0;
+    return sp;
+}
+
+__global__ void flatten_runs_table_kernel(FlattenLaunch L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const SegPoly sp = seg_poly(L.kinds[i], L.ctrl + 8ull * i, L.tolerance);
+    poly_store(reinterpret_cast<float4*>(L.polys) + 3ull * i, sp);
+    const uint32_t off = L.offsets[i];
+    uint2* runs = reinterpret_cast<uint2*>(L.runs);
+    // k: the segment's first point that starts a run; t: the sum of k + 1 steps
+    uint32_t k = (FR_RUN - (off & (FR_RUN - 1))) & (FR_RUN - 1);
+    float t = 0.0f;
+    for (uint32_t q = 0; q <= k; ++q) t = t + sp.step;
+    for (; k < sp.n; k += FR_RUN) {
+        runs[(off + k) / FR_RUN] = make_uint2(i, f2u(t));
+#pragma unroll
+        for (uint32_t q = 0; q < FR_RUN; ++q) t = t + sp.step;
+    }
+}
+
+// Emit from the run table: lane l of a wave owns points [16 l, 16 l + 16) of the wave's
+// 1,024; it reads its run's segment and t from the table, the segment's polynomial form
+// and its two offsets, and continues the serial sum with one addition per point (moving
+// to the next segment, whose offset it reads, when the run crosses one).  The points go
+// through LDS so that every store writes 128 consecutive points (64 with V16 false, for an
+// output that is only 8-byte aligned).
+template <bool V16>
+__global__ void __launch_bounds__(WAVE* FL_WAVES) flatten_emit_table_kernel(FlattenLaunch L,
+                                                                            uint32_t total) {
+    constexpr uint32_t PAD = V16 ? 2 : 1;  // V16: 16-byte aligned pairs in LDS
+    __shared__ __align__(16) float2 stage_all[FL_WAVES][WAVE * (FR_RUN + PAD)];  // a run per lane, padded
+    const int l = lane_id();
+    const uint32_t w = threadIdx.x / WAVE;
+    float2* stage = stage_all[w];
+    const uint64_t wave = (uint64_t)blockIdx.x * FL_WAVES + w;
+    const uint64_t base = wave * WAVE * FR_RUN;
+    if (base >= total) return;  // wave-uniform
+    const uint64_t wend = base + WAVE * FR_RUN < total ? base + WAVE * FR_RUN : total;
+    const uint64_t g0 = base + (uint64_t)l * FR_RUN;
+    if (g0 < wend) {
+        const uint32_t g_end = (uint32_t)(g0 + FR_RUN < wend ? g0 + FR_RUN : wend);
+        const uint2 run = reinterpret_cast<const uint2*>(L.runs)[g0 / FR_RUN];
+        uint32_t seg = run.x;
+        float t = u2f(run.y);
+        const float4* polys = reinterpret_cast<const float4*>(L.polys);
+        SegPoly sp = poly_load(polys + 3ull * seg);
+        uint32_t k = (uint32_t)g0 - L.offsets[seg];
+        uint32_t seg_end = L.offsets[seg + 1];
+        float2* st = stage + (uint32_t)l * (FR_RUN + PAD);
+        for (uint32_t g = (uint32_t)g0; g < g_end; ++g) {
+            if (g >= seg_end) {
+                // the next segment (every segment has at least one point)
+                ++seg;
+                sp = poly_load(polys + 3ull * seg);
+                seg_end = L.offsets[seg + 1];
+                k = 0;
+                t = 0.0f + sp.step;
+            }
+            const vec2 p = run_point(sp, k, t);
+            st[g - (uint32_t)g0] = make_float2(p.x, p.y);
+            ++k;
+            t = t + sp.step;
+        }
+    }
+    // the wave's points leave LDS in order: 64 consecutive points per store
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    float2* out = reinterpret_cast<float2*>(L.out_points);
+    if constexpr (!V16) {
+#pragma unroll
+        for (uint32_t r = 0; r < FR_RUN; ++r) {
+            const uint32_t q = r * WAVE + (uint32_t)l;  // point base + q
+            if (base + q < wend)
+                store_stream(out + base + q, stage[(q / FR_RUN) * (FR_RUN + PAD) + q % FR_RUN]);
+        }
+    } else {
+        // two points per lane and store: 128 consecutive points per store
+        const uint32_t nw = (uint32_t)(wend - base);
+#pragma unroll
+        for (uint32_t r = 0; r < FR_RUN / 2; ++r) {
+            const uint32_t q = 2 * (r * WAVE + (uint32_t)l);  // points base + q, q + 1
+            const float4 v = *reinterpret_cast<const float4*>(
+                stage + (q / FR_RUN) * (FR_RUN + PAD) + q % FR_RUN);
+            if (q + 1 < nw) store_stream(reinterpret_cast<float4*>(out + base + q), v);
+            else if (q < nw) store_stream(out + base + q, make_float2(v.x, v.y));
+        }
     }
 }
 
@@ -393,14 +526,32 @@ hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s) 
     hipLaunchKernelGGL(lyg::flatten_count_kernel, dim3((L->n + 255) / 256), dim3(256), 0, s, *L);
     return hipGetLastError();
 }
+// With L->runs: the run table first (8 bytes per 16 points, 48 per segment), then the
+// emit starting its runs from it; without: the emit recovers t in closed form.
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s) {
     if (total == 0) return hipSuccess;
     const uint64_t per_block = (uint64_t)lyg::WAVE * lyg::FR_RUN * lyg::FL_WAVES;
-    hipLaunchKernelGGL(lyg::flatten_emit_runs_kernel,
-                       dim3((uint32_t)((total + per_block - 1) / per_block)),
-                       dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
+    const dim3 grid((uint32_t)((total + per_block - 1) / per_block));
+    if (L->runs) {
+        hipLaunchKernelGGL(lyg::flatten_runs_table_kernel, dim3((L->n + 255) / 256), dim3(256),
+                           0, s, *L);
+        // 16-byte stores need a 16-byte aligned output (every device allocation is)
+        if ((uintptr_t)L->out_points & 15)
+            hipLaunchKernelGGL(lyg::flatten_emit_table_kernel<false>, grid,
+                               dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
+        else
+            hipLaunchKernelGGL(lyg::flatten_emit_table_kernel<true>, grid,
+                               dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
+    } else {
+        hipLaunchKernelGGL(lyg::flatten_emit_runs_kernel, grid,
+                           dim3(lyg::WAVE * lyg::FL_WAVES), 0, s, *L, total);
+    }
     return hipGetLastError();
 }
+size_t lyg_flatten_runs_bytes(uint32_t total) {
+    return 8ull * ((uint64_t)total / lyg::FR_RUN + 1);
+}
+size_t lyg_flatten_poly_bytes(uint32_t n) { return 48ull * n; }
 hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
                                   const uint32_t* offsets, float* out, hipStream_t s) {
     if (n == 0) return hipSuccess;

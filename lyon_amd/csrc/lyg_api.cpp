@@ -40,6 +40,8 @@ hipError_t lyg_launch_stroke_items_compact(const lyg::StrokeCompact* C,
 hipError_t lyg_launch_nsub_thresholds(float* thr, hipStream_t s);
 hipError_t lyg_launch_flatten_count(const lyg::FlattenLaunch* L, hipStream_t s);
 hipError_t lyg_launch_flatten_emit(const lyg::FlattenLaunch* L, uint32_t total, hipStream_t s);
+size_t lyg_flatten_runs_bytes(uint32_t total);
+size_t lyg_flatten_poly_bytes(uint32_t n);
 hipError_t lyg_launch_arc_flatten(const float* arcs, uint32_t n, float tol, uint32_t* counts,
                                   const uint32_t* offsets, float* out, hipStream_t s);
 hipError_t lyg_launch_arc_to_quads(const float* arcs, uint32_t n, uint32_t* counts,
@@ -533,7 +535,8 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->in_verbs, &c->in_points, &c->in_offs, &c->in_opts, &c->counts_v,
                       &c->counts_i, &c->offs_v,    &c->offs_i,  &c->out_v,   &c->out_i,
-                      &c->misc,     &c->arcs,      &c->flat_kinds, &c->flat_ctrl};
+                      &c->misc,     &c->arcs,      &c->flat_kinds, &c->flat_ctrl,
+                      &c->flat_runs, &c->flat_poly};
     for (DevBuf* b : bufs) free_buf(*b);
     free_buf(c->f_bound);
     free_buf(c->f_status);
@@ -835,9 +838,14 @@ int32_t lyg_flatten_segments(lyg_ctx* c, const int32_t* kinds, const float* ctrl
     uint64_t total = h[0];
     if (n_points) *n_points = (uint32_t)total;
     if (total > point_capacity || !out_points) return LYG_ERR_CAPACITY;
-    if (!ensure(c, c->out_v, 8 * total)) return LYG_ERR_DEVICE;
+    if (!ensure(c, c->out_v, 8 * total) ||
+        !ensure(c, c->flat_runs, lyg_flatten_runs_bytes((uint32_t)total)) ||
+        !ensure(c, c->flat_poly, lyg_flatten_poly_bytes(n)))
+        return LYG_ERR_DEVICE;
     L.offsets = as<uint32_t>(c->offs_v);
     L.out_points = as<float>(c->out_v);
+    L.runs = as<uint32_t>(c->flat_runs);
+    L.polys = as<float>(c->flat_poly);
     if (!check(c, lyg_launch_flatten_emit(&L, (uint32_t)total, s), "flatten emit") ||
         !check(c, hipMemcpyAsync(out_points, c->out_v.p, 8 * total, hipMemcpyDeviceToHost, s), "D2H") ||
         !check(c, hipMemcpyAsync(offsets, c->offs_v.p, 4ull * (n + 1), hipMemcpyDeviceToHost, s), "D2H") ||
@@ -883,8 +891,13 @@ int32_t lyg_flatten_segments_device(lyg_ctx* c, const int32_t* kinds, const floa
     if ((uint32_t)h[1] != 0 || total > 0xffffffffull) return LYG_ERR_TOO_MANY_VERTICES;
     if (n_points) *n_points = (uint32_t)total;
     if (total > point_capacity || !out_points) return LYG_ERR_CAPACITY;
+    if (!ensure(c, c->flat_runs, lyg_flatten_runs_bytes((uint32_t)total)) ||
+        !ensure(c, c->flat_poly, lyg_flatten_poly_bytes(n)))
+        return LYG_ERR_DEVICE;
     L.offsets = offsets;
     L.out_points = out_points;
+    L.runs = as<uint32_t>(c->flat_runs);
+    L.polys = as<float>(c->flat_poly);
     if (c->timing) (void)hipEventRecord(c->ev[2], s);
     if (!check(c, lyg_launch_flatten_emit(&L, (uint32_t)total, s), "flatten emit")) return LYG_ERR_DEVICE;
     if (c->timing) (void)hipEventRecord(c->ev[3], s);

@@ -173,6 +173,9 @@ struct FlattenLaunch {
     uint32_t* counts;         // count pass
     const uint32_t* offsets;  // emit pass (n + 1)
     float* out_points;
+    uint32_t* runs;           // per run of 16 output points: segment, t bits (null: the emit
+                              // searches the offsets and computes t)
+    float* polys;             // with runs: 12 floats per segment (polynomial form, step, n)
 };
 
 // Path validators (lyon_algorithms hit_test.rs / area.rs) over the batched layout.

@@ -25,6 +25,20 @@ def test_segments_bit_exact(tol):
     assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), "positions differ"
 
 
+def test_segments_many_per_wave():
+    """Runs of lines (one point each) put up to 1,024 segments under one wave's points,
+    past the wave's LDS window of segment offsets; curves between the runs."""
+    kinds, ctrl = random_segments(6000, 7, 2048.0)
+    kinds[:2500] = 0
+    kinds[3000:5500] = 0
+    kinds[2700] = 2
+    for tol in (0.05, 1.0):
+        gp, go = flatten_segments(kinds, ctrl, tol)
+        rp, ro = O.flatten_segments(kinds, ctrl, tol)
+        assert np.array_equal(go, ro), "point counts differ"
+        assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32)), "positions differ"
+
+
 def test_segments_degenerate():
     ctrl = np.zeros((4, 4, 2), np.float32)
     ctrl[1] = [[1, 1], [1, 1], [1, 1], [1, 1]]
