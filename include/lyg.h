@@ -370,6 +370,14 @@ int32_t lyg_ctx_set_stroke_staging(lyg_ctx* ctx, int enabled, float room_scale);
  * whole-path walk.  Default 64 (parts of 4,096 steps); 0: every path is walked by one
  * wave. */
 int32_t lyg_ctx_set_stroke_split(lyg_ctx* ctx, uint32_t windows_per_part);
+/* Placement of the batched fill sweep's working arrays (no lyon counterpart).  enabled
+ * (default): the first round sweeps each path whose estimated active edges, spans, pending
+ * intersections and monotone nodes fit 150 KiB in a form that keeps them all in LDS (a
+ * path that outgrows them is swept again in the global-memory form); 0: every path starts
+ * with its arrays in LDS and moves them to global memory as they grow.  Both give the
+ * same output.  enabled > 1 divides the LDS-resident capacities by that factor, so that
+ * paths outgrow them and take the fallback (for tests). */
+int32_t lyg_ctx_set_fill_lds_resident(lyg_ctx* ctx, int enabled);
 /* Of the last split batched stroke on this context: the items walked (whole paths and
  * parts) and the paths walked again whole because a part's prologue did not rebuild the
  * state or a part outgrew its room (no lyon counterpart). */

@@ -163,6 +163,7 @@ SIGNATURES = {
     "lyg_ctx_set_tuning": ([C.c_void_p, C.POINTER(TuningC), C.POINTER(TuningC)], C.c_int32),
     "lyg_ctx_set_stroke_staging": ([C.c_void_p, C.c_int, C.c_float], C.c_int32),
     "lyg_ctx_set_stroke_split": ([C.c_void_p, C.c_uint32], C.c_int32),
+    "lyg_ctx_set_fill_lds_resident": ([C.c_void_p, C.c_int], C.c_int32),
     "lyg_ctx_stroke_split_stats": ([C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)],
                                    C.c_int32),
     "lyg_autotune_stroke": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,

@@ -325,6 +325,17 @@ def set_stroke_split(windows_per_part: int = 64):
         raise TessellationError(st)
 
 
+def set_fill_lds_resident(enabled=True):
+    """Placement of the batched fill sweep's arrays (lyg_ctx_set_fill_lds_resident): by
+    default the first round keeps every hot array of a path that fits 150 KiB in LDS;
+    False: arrays start in LDS and move to global memory as they grow; an int > 1 divides
+    the LDS capacities by it (paths outgrow them and are swept again).  Same output."""
+    v = int(enabled) if not isinstance(enabled, bool) else (1 if enabled else 0)
+    st = B.lib().lyg_ctx_set_fill_lds_resident(_Ctx.get(), v)
+    if st != 0:
+        raise TessellationError(st)
+
+
 def stroke_split_stats() -> dict:
     """Of the last split batched stroke: items walked and paths walked again whole."""
     items = C.c_uint64(0)

@@ -36,6 +36,7 @@ struct lyg_ctx {
     // fill launch shape (set by the autotuner): waves per block, LDS bytes per wave
     uint32_t fill_waves_per_block = 2;
     uint32_t fill_hot_bytes = 16 * 1024;
+    uint32_t fill_lds_resident = 1;  // first round in the LDS-resident form where it fits
     // fill: per-path arrays, per-wave working memory and per-round staging
     DevBuf f_bound, f_status, f_round, f_hint, f_work, f_inter, f_boxes;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
@@ -52,7 +53,7 @@ struct lyg_ctx {
     DevBuf st_items, st_heads, st_spec, st_end, st_parts, st_parts_off;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
-    hipEvent_t class_ev[9] = {};
+    hipEvent_t class_ev[25] = {};
     // banded fill: band size (edge starts per band) and the most bands of one path
     BandBufs* band = nullptr;
     uint32_t band_edges_per_band = 256;

@@ -22,6 +22,8 @@
 extern "C" {
 size_t lyg_fill_work_bytes(const lyg::fillk::FillCaps* caps, size_t hot_bytes);
 int lyg_fill_occupancy(uint32_t waves_per_block, uint32_t hot_bytes);
+int lyg_fill_occupancy_form(uint32_t waves_per_block, uint32_t hot_bytes, int form);
+size_t lyg_fill_lds_bytes(const lyg::fillk::FillCaps* lc);
 hipError_t lyg_launch_fill_bound(const lyg::FillLaunch* L, uint32_t n, uint32_t* bound,
                                  hipStream_t s);
 hipError_t lyg_launch_fill_sweep(const lyg::FillLaunch* L, int grid, hipStream_t s);
@@ -42,9 +44,52 @@ constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO
 constexpr size_t WORK_BUDGET = 16ull << 30;          // working memory of one round
 constexpr size_t STAGE_BUDGET = 64ull << 30;         // staging memory of one round
 constexpr int MAX_CLASSES = 8;
+constexpr int N_FORMS = 3;  // placement forms of the sweep's hot arrays (FillLaunch::form)
+constexpr int FORM_MIGRATE = 0, FORM_LDS = 1, FORM_PART = 2;
+constexpr int MAX_LAUNCHES = N_FORMS * MAX_CLASSES;
 constexpr size_t CLASS_STREAMS = 3;
+// The first round sweeps a path with all its hot arrays in LDS when their estimate fits
+// this many bytes per wave, else with the per-event ones in LDS when those fit, else in
+// the migrating form: three waves per CU (a CU has 160 KiB; the kernel's own LDS takes
+// 4 KiB), enough to keep every large path of a launch resident at once.
+constexpr size_t LDS_WAVE_MAX = 52u * 1024u;
+constexpr size_t LDS_CU_BYTES = 156u * 1024u;
+constexpr size_t SWEEP_MAX_WAVES = 4;  // the sweep kernel's launch bound (waves per block)
 
 using lyg::fillk::FillCaps;
+
+// LDS-resident capacities for a path of at most `bound` events (two per edge): active
+// edges (and pending edges below one point) are bounded by the edges that cross one
+// sweep line, a quarter of the events plus a margin; spans by the active edges (dead
+// spans wait for the next cleanup); the pending intersections by 2.5 active edges; the
+// monotone node pool by the events.  The 513-point stars of the fill workload peak at
+// 280 active edges, 265 spans, 678 heap entries and 1,114 nodes (bound 1,044).  A path
+// that outgrows one of them is swept again in the migrating form.
+// `shrink` > 1 divides them (at least 8 each), which makes paths outgrow the LDS form on
+// purpose: the tests' way to run its fallback.
+FillCaps lds_caps(uint64_t bound, uint32_t shrink) {
+    const uint64_t d = shrink > 1 ? shrink : 1;
+    auto cap = [&](uint64_t v) {
+        return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(v / d, 8), 0x7fffffffu);
+    };
+    const uint64_t act = bound / 4 + bound / 32 + 16;
+    FillCaps k;
+    k.ev = 0;
+    k.act = cap(act);
+    k.span = cap(act + 16);
+    k.heap = cap(act * 5 / 2 + 64);
+    k.node = cap(bound + bound / 4 + 64);
+    k.below = cap(act / 4 + 16);
+    return k;
+}
+// The form a path starts in, and its LDS capacities (spans and nodes 0 in FORM_PART).
+int path_form(uint64_t bound, uint32_t mode, FillCaps* lc) {
+    *lc = lds_caps(bound, mode);
+    if (mode && lyg_fill_lds_bytes(lc) <= LDS_WAVE_MAX) return FORM_LDS;
+    lc->span = lc->node = 0;
+    if (mode && lyg_fill_lds_bytes(lc) <= LDS_WAVE_MAX) return FORM_PART;
+    return FORM_MIGRATE;
+}
 
 // Working-memory capacities of one wave for paths of at most `bound` events and `inter`
 // estimated self-intersections at `scale` (1, then growing with each retry round).
@@ -82,7 +127,7 @@ int size_class(uint64_t bound) {
 // Streams (default priority) for running a round's size classes at once, and one event
 // per launch plus the start event.
 bool class_streams(lyg_ctx* c, size_t n, size_t n_launches) {
-    if (n > (size_t)MAX_CLASSES || n_launches > (size_t)MAX_CLASSES) return false;
+    if (n > (size_t)MAX_CLASSES || n_launches > (size_t)MAX_LAUNCHES) return false;
     for (size_t i = 0; i < n; ++i)
         if (!c->class_stream[i] &&
             !check(c, hipStreamCreateWithFlags(&c->class_stream[i], hipStreamNonBlocking),
@@ -174,9 +219,16 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     uint64_t scale = 1, next_scale = 4;
     int round = 0;
     for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale = next_scale) {
-        // order by size class, then by path
+        // order by launch: size class, then the placement form (the first round picks it
+        // per path; retries migrate); then by path
         std::vector<int> cls(todo.size());
-        for (size_t k = 0; k < todo.size(); ++k) cls[k] = size_class(need[todo[k]]);
+        for (size_t k = 0; k < todo.size(); ++k) {
+            const uint32_t p = todo[k];
+            FillCaps lc;
+            const int form = round == 0 ? path_form(bound[p], c->fill_lds_resident, &lc)
+                                        : FORM_MIGRATE;
+            cls[k] = N_FORMS * size_class(need[p]) + form;
+        }
         std::vector<size_t> order(todo.size());
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] < cls[b]; });
@@ -222,6 +274,9 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             size_t k0, k1, blocks, stride, work_off;
             uint64_t est;
             FillCaps cap;
+            int form;
+            FillCaps lc;
+            uint32_t wpb, hot;
         };
         const uint32_t wpb = c->fill_waves_per_block;
         const uint32_t hot = c->fill_hot_bytes;
@@ -243,11 +298,33 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             ln.est = 0;
             for (size_t k = k0; k < k1; ++k) ln.est = std::max<uint64_t>(ln.est, need[list[k]]);
             ln.cap = class_caps(std::max<uint64_t>(maxb, 1), maxi, scale);
-            ln.stride = lyg_fill_work_bytes(&ln.cap, hot);
-            size_t blocks = (k1 - k0 + wpb - 1) / wpb;
-            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * occ));
+            ln.form = kc % N_FORMS;
+            ln.lc = FillCaps{};
+            ln.wpb = wpb;
+            ln.hot = hot;
+            int ln_occ = occ;
+            if (ln.form != FORM_MIGRATE) {
+                // every capacity grows with the bound: the largest path's fit them all
+                (void)path_form(maxb, c->fill_lds_resident, &ln.lc);
+                if (ln.form == FORM_LDS) ln.lc = lds_caps(maxb, c->fill_lds_resident);
+                ln.hot = (uint32_t)((lyg_fill_lds_bytes(&ln.lc) + 255) & ~(size_t)255);
+                ln.wpb = (uint32_t)std::max<size_t>(
+                    1, std::min<size_t>(SWEEP_MAX_WAVES, LDS_CU_BYTES / ln.hot));
+                ln_occ = lyg_fill_occupancy_form(ln.wpb, ln.hot, ln.form);
+                while (ln_occ < 1 && ln.wpb > 1)
+                    ln_occ = lyg_fill_occupancy_form(--ln.wpb, ln.hot, ln.form);
+                if (ln_occ < 1) {  // cannot launch: the migrating form
+                    ln.form = FORM_MIGRATE;
+                    ln.wpb = wpb;
+                    ln.hot = hot;
+                    ln_occ = occ;
+                }
+            }
+            ln.stride = lyg_fill_work_bytes(&ln.cap, ln.form != FORM_MIGRATE ? 0 : ln.hot);
+            size_t blocks = (k1 - k0 + ln.wpb - 1) / ln.wpb;
+            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * ln_occ));
             ln.work_off = 0;
-            want_bytes += ln.blocks * ln.stride * wpb;
+            want_bytes += ln.blocks * ln.stride * ln.wpb;
             launches.push_back(ln);
             k0 = k1;
         }
@@ -258,7 +335,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
                 ln.blocks = std::max<size_t>(
                     1, (size_t)((double)ln.blocks * (double)WORK_BUDGET / (double)want_bytes));
             ln.work_off = work_bytes;
-            work_bytes += (ln.blocks * ln.stride * wpb + 255) & ~(size_t)255;
+            work_bytes += (ln.blocks * ln.stride * ln.wpb + 255) & ~(size_t)255;
         }
         if (!ensure(c, c->f_work, work_bytes)) return LYG_ERR_NO_SPACE;
         const auto t_round = std::chrono::steady_clock::now();
@@ -279,8 +356,10 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             L.work = as<uint8_t>(c->f_work) + ln.work_off;
             L.work_stride = ln.stride;
             L.caps = ln.cap;
-            L.hot_bytes = hot;
-            L.waves_per_block = wpb;
+            L.hot_bytes = ln.hot;
+            L.waves_per_block = ln.wpb;
+            L.form = (uint32_t)ln.form;
+            L.lds_caps = ln.lc;
             L.path_list = d_list + ln.k0;
             L.n_list = (uint32_t)(ln.k1 - ln.k0);
             L.stage_v_off = d_off + ln.k0;
@@ -296,9 +375,11 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
                 (void)hipStreamSynchronize(cs);
                 fprintf(stderr,
                         "[lyg]   class launch %zu: %zu paths, %zu blocks, %.1f KiB per wave, "
-                        "heap %u node %u act %u, %.2f ms\n",
+                        "heap %u node %u act %u, %s %u waves x %u B LDS, %.2f ms\n",
                         li, ln.k1 - ln.k0, ln.blocks, ln.stride / 1024.0, ln.cap.heap, ln.cap.node,
                         ln.cap.act,
+                        ln.form == FORM_LDS ? "LDS" : ln.form == FORM_PART ? "LDS+global" : "migrating",
+                        ln.wpb, ln.hot,
                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
                                                                   t_launch).count());
                 fflush(stderr);

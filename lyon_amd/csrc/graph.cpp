@@ -356,6 +356,12 @@ int32_t lyg_ctx_stroke_split_stats(lyg_ctx* c, uint64_t* items, uint32_t* walked
     return LYG_OK;
 }
 
+int32_t lyg_ctx_set_fill_lds_resident(lyg_ctx* c, int enabled) {
+    if (!c) return LYG_ERR_INVALID_ARGUMENT;
+    c->fill_lds_resident = enabled > 0 ? (uint32_t)enabled : 0u;
+    return LYG_OK;
+}
+
 int32_t lyg_ctx_set_stroke_split(lyg_ctx* c, uint32_t windows_per_part) {
     if (!c || windows_per_part > (1u << 20)) return LYG_ERR_INVALID_ARGUMENT;
     c->stroke_split_windows = windows_per_part;

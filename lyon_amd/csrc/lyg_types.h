@@ -143,6 +143,12 @@ struct FillLaunch {
     fillk::FillCaps caps;          // capacities of the per-wave arrays
     uint32_t hot_bytes;            // per-wave dynamic LDS for the hot arrays (0: none)
     uint32_t waves_per_block;
+    // placement form of the hot arrays: 0 they start in LDS and move to global memory when
+    // they outgrow it; 1 all in LDS; 2 the arrays touched on every event in LDS, spans and
+    // nodes in global memory.  Forms 1 and 2 use the lds_caps capacities (act, span, node,
+    // heap, below) and stop a path that outgrows one with the retry status.
+    uint32_t form;
+    fillk::FillCaps lds_caps;
     float* stage_v;                // per-path staging, path i at stage_v_off[i] (vertices)
     uint32_t* stage_t;             // and stage_t_off[i] (triangles)
     const uint64_t* stage_v_off;

@@ -158,6 +158,35 @@ def test_batch_matches_plain_calls():
         assert np.array_equal(canon(gi), canon(ri)), f"batch path {i}: triangle sets differ"
 
 
+def test_lds_resident_form_and_fallback():
+    """The batched fill gives the same bytes whether its first round keeps each path's hot
+    arrays in LDS (the default), starts them in LDS and moves them to global memory as
+    they grow, or keeps them in LDS at an eighth of the estimated capacities (so that many
+    paths outgrow them and are swept again in the global form); paths of every size of the
+    fill workload, the 513-point stars included."""
+    from lyon_amd.batch import DeviceBatch, FillBatcher, set_fill_lds_resident, synchronize
+    n = 512
+    batch = workloads.fill_batch(n, seed=0)
+    opts = workloads.fill_options(n, tolerance=0.05)
+    db = DeviceBatch(batch, opts)
+    outs = []
+    try:
+        for mode in (1, 0, 8):
+            set_fill_lds_resident(mode)
+            res = FillBatcher().run(db)
+            synchronize()
+            outs.append((mode, res.status.copy(), res.all_vertices().copy(),
+                         res.all_indices().copy()))
+    finally:
+        set_fill_lds_resident(True)
+    m0, st0, v0, i0 = outs[0]
+    assert np.all(st0 == 0)
+    for m, st, v, i in outs[1:]:
+        assert np.array_equal(st, st0), f"mode {m}: statuses differ"
+        assert np.array_equal(v.view(np.uint32), v0.view(np.uint32)), f"mode {m}: vertices differ"
+        assert np.array_equal(i, i0), f"mode {m}: indices differ"
+
+
 def test_very_large_path():
     """fill_tests.rs:2140-2166: a path of 1,000,000 endpoints (two wiggling sides of a long
     band) fills without error; the same vertices and triangle count as the oracle and the
