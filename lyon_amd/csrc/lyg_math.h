@@ -191,69 +191,6 @@ LYG_HD uint64_t tchain_units_from(uint32_t m, uint32_t j, uint32_t k0, uint64_t 
     return T;
 }
 LYG_HD uint64_t tchain_units(uint32_t m, uint32_t j) { return tchain_units_from(m, j, 0, 0); }
-
-// min(cap, floor(x / y)) for x < 2^24, 0 < y < 2^24, without an integer division: when
-// the cap is reached a multiplication shows it; otherwise the quotient is estimated in f32
-// (both operands are exact floats) and corrected by exact integer comparisons, so the
-// result never depends on the estimate's rounding.
-LYG_HD uint32_t tchain_div_capped(uint32_t x, uint32_t y, uint32_t cap) {
-    if ((uint64_t)cap * y <= x) return cap;
-#if defined(__HIP_DEVICE_COMPILE__)
-    const float qf = (float)x * __builtin_amdgcn_rcpf((float)y);
-#else
-    const float qf = (float)x / (float)y;
-#endif
-    uint32_t q = qf < (float)cap ? (uint32_t)qf : cap;  // q <= cap, and floor(x / y) < cap
-    while ((uint64_t)q * y > x) --q;
-    while ((uint64_t)(q + 1) * y <= x) ++q;
-    return q;
-}
-
-// tchain_units_from(m, j, k0, T0) (the same units, step for step the same runs), with the
-// step counts of the runs from tchain_div_capped: when every remaining step lies in the
-// current run (the common case when a chain is resumed a few steps further on) the
-// count's cap is reached at once and no division is made.
-LYG_HD uint64_t tchain_units_fast(uint32_t m, uint32_t j, uint32_t k0, uint64_t T0) {
-    uint64_t T = T0;
-    uint32_t k = k0;
-    while (k < j) {
-        const uint32_t left = j - k;
-        const uint64_t S = T + m;
-        const int b = bitlen64(S) - 24;
-        if (b <= 0) {
-            // exact region: further steps while T + c*m <= 2^24 - 1 (at least one: S < 2^24)
-            const uint32_t lim = (1u << 24) - 1;
-            uint32_t c = tchain_div_capped(lim - (uint32_t)T, m, left);
-            if (c == 0) c = 1;
-            T += (uint64_t)c * m;
-            k += c;
-            continue;
-        }
-        const uint64_t g = 1ull << b;
-        const uint64_t A = T >> b;
-        const bool aligned = (T & (g - 1)) == 0;
-        const uint64_t M = (uint64_t)m >> b;
-        const uint64_t r = (uint64_t)m & (g - 1);
-        const uint64_t half = g >> 1;
-        if (!aligned || (r == half && (A & 1ull))) {
-            T = tchain_round(S);
-            k += 1;
-            continue;
-        }
-        uint64_t inc_units;
-        if (r < half) inc_units = M;
-        else if (r > half) inc_units = M + 1;
-        else inc_units = (M & 1ull) ? M + 1 : M;
-        if (inc_units == 0) break;  // step below half an ulp: t no longer moves
-        // steps whose sums stay below the binade end: 1 + floor(room / inc), room >> b < 2^23
-        const uint64_t room = (1ull << (24 + b)) - 1 - S;
-        const uint32_t c =
-            tchain_div_capped((uint32_t)(room >> b), (uint32_t)inc_units, left - 1) + 1;
-        T += (uint64_t)c * (inc_units << b);
-        k += c;
-    }
-    return T;
-}
 // The 24-bit significand m of s (s = m * ulp(s)).
 LYG_HD uint32_t tchain_mant(float s) { return (f2u(s) & 0x7fffffu) | 0x800000u; }
 // The f32 value T * ulp(s).
