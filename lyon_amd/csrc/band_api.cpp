@@ -288,11 +288,11 @@ int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc,
         P.n_records = (uint32_t)nrec;
         if (!zero(c, B.b_nrec, nb)) return LYG_ERR_DEVICE;
         if (c->timing) (void)hipEventRecord(c->ev[2], s);
-        // the sweep's three size classes on the context's stream and two side streams
-        // (default priority), joined by events
+        // the sweep's three size classes on the context's stream and two of its side
+        // streams (default priority, made with the context), joined by events
         for (int k = 0; k < 2; ++k)
-            if (!c->class_stream[6 + k] &&
-                !check(c, hipStreamCreateWithFlags(&c->class_stream[6 + k], hipStreamNonBlocking),
+            if (!c->class_stream[k] &&
+                !check(c, hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking),
                        "hipStreamCreate"))
                 return LYG_ERR_DEVICE;
         for (int k = 0; k < 3; ++k)
@@ -300,7 +300,7 @@ int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc,
                 !check(c, hipEventCreateWithFlags(&c->class_ev[6 + k], hipEventDisableTiming),
                        "hipEventCreate"))
                 return LYG_ERR_DEVICE;
-        hipStream_t s1 = c->class_stream[6], s2 = c->class_stream[7];
+        hipStream_t s1 = c->class_stream[0], s2 = c->class_stream[1];
         if (!check(c, hipEventRecord(c->class_ev[6], s), "event") ||
             !check(c, hipStreamWaitEvent(s1, c->class_ev[6], 0), "wait") ||
             !check(c, hipStreamWaitEvent(s2, c->class_ev[6], 0), "wait") ||

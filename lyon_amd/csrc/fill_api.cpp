@@ -320,6 +320,13 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
                     ln_occ = occ;
                 }
             }
+            if (ln.form != FORM_MIGRATE) {
+                // arrays that live in LDS need no global region (their overflow retries):
+                // the heap is most of a wave's worst-case working memory
+                ln.cap.heap = 0;
+                ln.cap.below = 0;
+                if (ln.form == FORM_LDS) ln.cap.span = 0;
+            }
             ln.stride = lyg_fill_work_bytes(&ln.cap, ln.form != FORM_MIGRATE ? 0 : ln.hot);
             size_t blocks = (k1 - k0 + ln.wpb - 1) / ln.wpb;
             ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * ln_occ));

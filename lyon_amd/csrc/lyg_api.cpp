@@ -513,6 +513,15 @@ lyg_ctx* lyg_ctx_create(int device) {
         delete c;
         return nullptr;
     }
+    // The side streams of the fill's concurrent launches are made here, right after the
+    // context's own stream, so that they get hardware queues of their own (a process has
+    // four): streams made later (a HIP graph's replay makes some) would share queues with
+    // them and serialise the concurrent launches.
+    for (int k = 0; k < 3; ++k)
+        if (hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking) != hipSuccess) {
+            lyg_ctx_destroy(c);
+            return nullptr;
+        }
     if (hipMalloc(&c->d_counters, 64) != hipSuccess || hipMalloc(&c->d_totals, 64) != hipSuccess ||
         hipHostMalloc(&c->h_pinned, 256, hipHostMallocDefault) != hipSuccess ||
         lyg_noalloc_scratch_init(device) != hipSuccess) {
