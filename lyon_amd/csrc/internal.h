@@ -34,4 +34,9 @@ int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc,
 int32_t fill_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
                    uint64_t* ti, std::vector<uint32_t>* status);
 
+// A context; side_streams: make the fill's side streams now (lyg_ctx_create does, so that
+// they get hardware queues of their own) rather than on the first fill (the serving loop's
+// workers, whose own streams should each keep a queue).
+lyg_ctx* ctx_create(int device, bool side_streams);
+
 }  // namespace lygi

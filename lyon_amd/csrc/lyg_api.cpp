@@ -496,7 +496,11 @@ bool valid_stroke_opts(const lyg_stroke_options& o) {
 
 extern "C" {
 
-lyg_ctx* lyg_ctx_create(int device) {
+lyg_ctx* lyg_ctx_create(int device) { return lygi::ctx_create(device, true); }
+
+}  // extern "C"
+
+lyg_ctx* lygi::ctx_create(int device, bool side_streams) {
     if (trace_on()) {
         fprintf(stderr, "[lyg] ctx_create(%d)\n", device);
         fflush(stderr);
@@ -517,7 +521,7 @@ lyg_ctx* lyg_ctx_create(int device) {
     // context's own stream, so that they get hardware queues of their own (a process has
     // four): streams made later (a HIP graph's replay makes some) would share queues with
     // them and serialise the concurrent launches.
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; side_streams && k < 3; ++k)
         if (hipStreamCreateWithFlags(&c->class_stream[k], hipStreamNonBlocking) != hipSuccess) {
             lyg_ctx_destroy(c);
             return nullptr;
@@ -537,6 +541,8 @@ lyg_ctx* lyg_ctx_create(int device) {
     }
     return c;
 }
+
+extern "C" {
 
 void lyg_ctx_destroy(lyg_ctx* c) {
     if (!c) return;

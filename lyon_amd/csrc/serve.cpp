@@ -572,7 +572,7 @@ lyg_server* lyg_server_create(const lyg_server_config* cfg) {
     if (hipSetDevice(cfg->device) != hipSuccess || !s->pool.init(cfg->pool_bytes)) return nullptr;
     for (uint32_t i = 0; i < cfg->max_inflight; ++i) {
         auto w = std::make_unique<lyg_server::Worker>();
-        w->ctx = lyg_ctx_create(cfg->device);
+        w->ctx = lygi::ctx_create(cfg->device, false);
         if (w->ctx && cfg->stroke_sizing == 1) lyg_ctx_set_stroke_staging(w->ctx, 0, 1.0f);
         if (w->ctx && cfg->stroke_split_windows)
             lyg_ctx_set_stroke_split(w->ctx, cfg->stroke_split_windows);
