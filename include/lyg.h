@@ -301,8 +301,8 @@ enum {
     LYG_SVG_ERR_COMMAND = 3,          /* ParseError::Command */
     LYG_SVG_ERR_MISSING_MOVE_TO = 4,  /* ParseError::MissingMoveTo */
     LYG_SVG_ERR_NON_ASCII = 5,        /* not in lyon: non-ASCII input */
-    LYG_SVG_ERR_UNDECIDED_NUMBER = 6  /* not in lyon: more than 19 significant digits whose
-                                         rounding they do not decide */
+    LYG_SVG_ERR_UNDECIDED_NUMBER = 6  /* no longer returned (numbers of any length are
+                                         rounded correctly); kept for the ABI */
 };
 /* Device pointers: text[text_offsets[i] .. text_offsets[i+1]) is string i.  verb_offsets /
  * point_offsets (n+1) and errors (4n, may be null) are written; verbs / points are written

@@ -10,7 +10,8 @@
 // bit); when they fit 2^53 and the decimal exponent is within ±22, one IEEE double
 // operation gives the value correctly rounded to double, and its rounding to f32 is the
 // correct one unless the double lies exactly on an f32 midpoint; every other case is
-// rounded exactly with 320-bit integer arithmetic.
+// rounded exactly with 320-bit integer arithmetic, longer digit strings by comparing their
+// remaining digits with the decimal expansion of the midpoint they straddle.
 //
 // Arcs: SvgArc::to_arc (Arc::from_svg_arc, crates/geom/src/arc.rs:58-146) and
 // Arc::for_each_quadratic_bezier_with_t (arc.rs:668-701), as the parser does
@@ -151,7 +152,70 @@ __device__ bool fast_round(uint64_t w, int e, uint32_t* bits) {
 
 __device__ __forceinline__ bool is_digit(uint32_t c) { return c >= '0' && c <= '9'; }
 
-enum { NUM_OK = 0, NUM_SYNTAX = 1, NUM_UNDECIDED = 2 };
+// Beyond 19 significant digits, when w * 10^e and (w + 1) * 10^e round to different f32
+// values lo and hi (then adjacent: the interval is 1e-18 of the value wide), the value
+// rounds by where it lies against their midpoint m = mm * 2^q.  With r = m / 10^e - w in
+// (0, 1) as the fraction rn / rd, the remaining digits d20 d21 ... (the fraction 0.d20d21...
+// of the value over 10^e) are compared with r's decimal digits one at a time; r is dyadic
+// over a power of ten, so its expansion ends.  Returns lo or hi (ties to even).
+__device__ uint32_t decide_midpoint(const uint8_t* s, uint32_t a, uint32_t b, uint64_t w, int e,
+                                    uint32_t lo) {
+    uint64_t mm;
+    int q;
+    if ((lo >> 23) == 0) {  // zero or subnormal: lo * 2^-149, the midpoint (2 lo + 1) 2^-150
+        mm = 2ull * lo + 1;
+        q = -150;
+    } else {
+        mm = 2ull * ((lo & 0x7fffffu) | 0x800000u) + 1;
+        q = (int)(lo >> 23) - 151;
+    }
+    // r = mm * 2^q * 10^-e - w = rn / rd, rd = 2^max(-q, 0) * 10^max(e, 0)
+    Big rn, rd, t;
+    big_set(rn, mm);
+    if (q > 0) big_shl(rn, q);
+    if (e < 0) big_pow10(rn, -e);
+    big_set(rd, 1);
+    if (q < 0) big_shl(rd, -q);
+    if (e > 0) big_pow10(rd, e);
+    t = rd;
+    big_mul(t, (uint32_t)(w >> 32));
+    big_shl(t, 32);
+    {
+        Big u = rd;
+        big_mul(u, (uint32_t)w);
+        // t += u
+        uint64_t c = 0;
+        for (int i = 0; i < BW; ++i) {
+            const uint64_t v = (uint64_t)t.w[i] + u.w[i] + c;
+            t.w[i] = (uint32_t)v;
+            c = v >> 32;
+        }
+    }
+    big_sub(rn, t);  // w * rd <= m * rd / 10^e: no borrow
+    // the digits after the 19th significant one, in order
+    int sig = 0;
+    for (uint32_t i = a; i < b; ++i) {
+        const uint32_t ch = s[i];
+        if (ch == 'e' || ch == 'E') break;
+        if (!is_digit(ch)) continue;
+        const uint32_t d = ch - '0';
+        if (sig == 0 && d == 0) continue;  // leading zeros
+        if (++sig <= 19) continue;
+        big_mul(rn, 10);  // next digit of r: floor(10 rn / rd), at most 9
+        uint32_t dm = 0;
+        while (big_cmp(rn, rd) >= 0) {
+            big_sub(rn, rd);
+            ++dm;
+        }
+        if (d != dm) return d > dm ? lo + 1 : lo;
+    }
+    bool zero = true;
+    for (int i = 0; i < BW; ++i) zero = zero && rn.w[i] == 0;
+    if (!zero) return lo;          // the digits end below m
+    return (lo & 1u) ? lo + 1 : lo;  // exactly m: to even
+}
+
+enum { NUM_OK = 0, NUM_SYNTAX = 1 };
 
 // str::parse::<f32> of s[a, b), which holds what the parser collected.
 __device__ int parse_f32(const uint8_t* s, uint32_t a, uint32_t b, float* out) {
@@ -226,8 +290,7 @@ __device__ int parse_f32(const uint8_t* s, uint32_t a, uint32_t b, float* out) {
         if (sticky) {
             const uint32_t lo = exact_round(w, e, 1);
             const uint32_t hi = exact_round(w + 1, e, -1);
-            if (lo != hi) return NUM_UNDECIDED;
-            bits = lo;
+            bits = lo == hi ? lo : decide_midpoint(s, a, b, w, e, lo);
         } else if (!fast_round(w, e, &bits)) {
             bits = exact_round(w, e, 0);
         }
@@ -365,7 +428,7 @@ __device__ bool parse_number(Src& S, float* v, PErr& E) {
     }
     const int r = parse_f32(S.s, a, S.at, v);
     if (r == NUM_OK) return true;
-    E = PErr{r == NUM_SYNTAX ? LYG_SVG_ERR_NUMBER : LYG_SVG_ERR_UNDECIDED_NUMBER, line, col, 0u};
+    E = PErr{LYG_SVG_ERR_NUMBER, line, col, 0u};
     return false;
 }
 

@@ -54,6 +54,78 @@ def test_numbers_bit_exact():
     assert len(bad) == 0, [(corpus[i], xs[i], want[i]) for i in bad[:5]]
 
 
+def _exact_f32(fr):
+    """Round a positive Fraction to f32 bits, ties to even (the rounding of str::parse)."""
+    from fractions import Fraction
+    if fr == 0:
+        return 0
+    lo, hi = 0, 0x7f800000
+    while hi - lo > 1:  # largest bits whose value <= fr
+        mid = (lo + hi) // 2
+        if Fraction(float(np.uint32(mid).view(np.float32))) <= fr:
+            lo = mid
+        else:
+            hi = mid
+    if lo == 0x7f7fffff:
+        up = Fraction(2) ** 128  # the next step past the largest float rounds to infinity
+    else:
+        up = Fraction(float(np.uint32(lo + 1).view(np.float32)))
+    m = (Fraction(float(np.uint32(lo).view(np.float32))) + up) / 2
+    if fr < m or (fr == m and lo % 2 == 0):
+        return lo
+    return lo + 1
+
+
+def _decimal(fr, digits_after):
+    """Plain decimal text of a Fraction whose expansion ends within digits_after places."""
+    ip = fr.numerator // fr.denominator
+    rest = fr - ip
+    out = []
+    for _ in range(digits_after):
+        rest *= 10
+        d = rest.numerator // rest.denominator
+        out.append(str(d))
+        rest -= d
+        if rest == 0:
+            break
+    assert rest == 0
+    return f"{ip}." + "".join(out)
+
+
+def test_numbers_past_19_digits_on_a_midpoint():
+    """Numbers of 25 to 160 significant digits that are an f32 midpoint exactly, or lie one
+    unit of their 60th (or last + 20th) digit above or below it: rounded like str::parse
+    (correctly, ties to even), checked against exact rational rounding and the oracle's
+    strtof; values from subnormals to the largest float."""
+    from fractions import Fraction
+    rng = np.random.default_rng(21)
+    bits = np.concatenate([rng.integers(1, 0x7f7fffff, 60, dtype=np.int64),
+                           [1, 2, 0x7fffff, 0x800000, 0x3f800000, 0x7f7ffffe, 0x7f7fffff]])
+    strings, want = [], []
+    for b in bits.tolist():
+        lo = Fraction(float(np.uint32(b).view(np.float32)))
+        up = Fraction(2) ** 128 if b == 0x7f7fffff else \
+            Fraction(float(np.uint32(b + 1).view(np.float32)))
+        m = (lo + up) / 2
+        text = _decimal(m, 200)
+        sig = len(text.replace(".", "").lstrip("0"))
+        if sig <= 19:
+            continue
+        eps = Fraction(1, 10 ** (len(text.split(".")[1]) + 20))
+        for v, t in ((m, text), (m + eps, _decimal(m + eps, 260)), (m - eps, _decimal(m - eps, 260))):
+            strings.append(t)
+            want.append(_exact_f32(v))
+    assert len(strings) > 150
+    batch, errs = S.parse_paths([f"M {t} 0" for t in strings])
+    assert all(e is None for e in errs), [(t, e) for t, e in zip(strings, errs) if e][:3]
+    got = batch.points[batch.point_offsets[:-1], 0].view(np.uint32).astype(np.int64)
+    want = np.array(want, np.int64)
+    bad = np.nonzero(got != want)[0]
+    assert len(bad) == 0, [(strings[i][:40], hex(got[i]), hex(want[i])) for i in bad[:5]]
+    ref = np.array([O.parse_f32(t) for t in strings], np.float32).view(np.uint32)
+    assert np.array_equal(ref.astype(np.int64), want)
+
+
 @pytest.mark.parametrize("arcs", [False, True])
 def test_random_svg_matches_oracle(arcs):
     rng = np.random.default_rng(17 if arcs else 4)
