@@ -38,7 +38,7 @@ struct lyg_ctx {
     uint32_t fill_hot_bytes = 16 * 1024;
     uint32_t fill_lds_resident = 1;  // first round in the LDS-resident form where it fits
     // fill: per-path arrays, per-wave working memory and per-round staging
-    DevBuf f_bound, f_status, f_round, f_hint, f_work, f_inter, f_boxes;
+    DevBuf f_bound, f_status, f_round, f_hint, f_work, f_inter, f_boxes, f_counters;
     std::vector<DevBuf> f_stage_v, f_stage_t, f_stage_off, f_list;
     // stroke: staged single pass (emit into per-path staging room, then compact)
     bool stroke_staged = true;
@@ -53,7 +53,7 @@ struct lyg_ctx {
     DevBuf st_items, st_heads, st_spec, st_end, st_parts, st_parts_off;
     // fill: one stream per size class of a round, and the events that join them
     hipStream_t class_stream[8] = {};
-    hipEvent_t class_ev[25] = {};
+    hipEvent_t class_ev[25] = {};  // one per fill launch (size class x form) + 1
     // banded fill: band size (edge starts per band) and the most bands of one path
     BandBufs* band = nullptr;
     uint32_t band_edges_per_band = 256;

@@ -31,6 +31,8 @@ size_t lyg_fill_box_bytes();
 hipError_t lyg_launch_fill_inter(const lyg::FillLaunch* L, uint32_t n, void* boxes, uint32_t* inter,
                                  hipStream_t s);
 hipError_t lyg_launch_fill_copy(const lyg::FillLaunch* L, hipStream_t s);
+hipError_t lyg_launch_fill_list_u32(uint32_t* a, const uint32_t* list, uint32_t m, uint32_t v,
+                                    hipStream_t s);
 hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
                            uint32_t* overflow, hipStream_t s);
 hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
@@ -39,6 +41,7 @@ hipError_t lyg_launch_fill_u32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t 
 namespace {
 
 constexpr uint32_t ST_RETRY = 100;
+constexpr uint32_t ST_UNSWEPT = 0xffffffffu;  // written before a round, replaced per path
 constexpr int MAX_ROUNDS = 5;
 constexpr uint64_t MAX_PATH_EVENTS = 1ull << 27;     // larger paths: LYG_ERR_NO_SPACE
 constexpr size_t WORK_BUDGET = 16ull << 30;          // working memory of one round
@@ -260,7 +263,12 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         if (!check(c, hipMemcpyAsync(d_list, list.data(), 4 * m, hipMemcpyHostToDevice, s), "H2D") ||
             !check(c, hipMemcpyAsync(d_off, offs.data(), 8 * offs.size(), hipMemcpyHostToDevice, s),
                    "H2D") ||
-            !check(c, lyg_launch_fill_u32(c->d_counters + 8, 8, 0, s), "reset"))
+            !ensure(c, c->f_counters, 4ull * MAX_LAUNCHES) ||
+            !check(c, lyg_launch_fill_u32(as<uint32_t>(c->f_counters), MAX_LAUNCHES, 0, s),
+                   "reset") ||
+            // a path of the round that no launch reaches keeps this status, never a success
+            !check(c, lyg_launch_fill_list_u32(L.path_status, d_list, (uint32_t)m, ST_UNSWEPT, s),
+                   "status"))
             return LYG_ERR_DEVICE;
         L.round = (uint32_t)round;
         L.stage_v = as<float>(c->f_stage_v[round]);
@@ -371,7 +379,7 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
             L.n_list = (uint32_t)(ln.k1 - ln.k0);
             L.stage_v_off = d_off + ln.k0;
             L.stage_t_off = d_off + (m + 1) + ln.k0;
-            L.work_counter = c->d_counters + 8 + (li & 7);
+            L.work_counter = as<uint32_t>(c->f_counters) + li;  // one per launch
             const auto t_launch = std::chrono::steady_clock::now();
             if (!check(c, hipStreamWaitEvent(cs, c->class_ev[0], 0), "wait") ||
                 !check(c, lyg_launch_fill_sweep(&L, (int)ln.blocks, cs), "fill sweep") ||
@@ -404,6 +412,10 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         std::vector<uint32_t> next;
         uint64_t grow = 4;  // next round's room: at least 4x, more if a path's hint asks
         for (uint32_t p : list) {
+            if (st_h[p] == ST_UNSWEPT) {  // no wave took the path: a launch error
+                c->last_error = "fill: a path was not swept";
+                return LYG_ERR_INTERNAL;
+            }
             if (st_h[p] == ST_RETRY) {
                 next.push_back(p);
                 uint64_t v, t;

@@ -566,6 +566,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
                       &c->st_parts, &c->st_parts_off})
         free_buf(*b);
     free_buf(c->f_boxes);
+    free_buf(c->f_counters);
     lyg_band_free(c);
     for (auto* v : {&c->f_stage_v, &c->f_stage_t, &c->f_stage_off, &c->f_list})
         for (DevBuf& b : *v) free_buf(b);
