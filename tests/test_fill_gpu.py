@@ -187,6 +187,35 @@ def test_lds_resident_form_and_fallback():
         assert np.array_equal(i, i0), f"mode {m}: indices differ"
 
 
+def test_many_launches_in_one_round():
+    """Paths in many size classes and placement forms (fill-workload stars and cubic loops,
+    closed random walks of 2,000 to 60,000 vertices) give a round more launches than there
+    used to be work counters: every path is swept (status 0) and the bytes equal those of
+    the migrating form, launched the old way."""
+    from lyon_amd.batch import DeviceBatch, FillBatcher, set_fill_lds_resident, synchronize
+    from lyon_amd.path import PathBatch
+    base = workloads.fill_batch(1024, seed=2)
+    paths = [base.path(i) for i in range(len(base))]
+    for k, nv in enumerate((2000, 7000, 20000, 60000)):
+        paths.append(workloads.random_walk_path(nv, seed=k + 1, extent=4096.0))
+    batch = PathBatch.from_paths(paths)
+    opts = workloads.fill_options(len(paths), tolerance=0.05, seed=3)
+    db = DeviceBatch(batch, opts)
+    outs = []
+    try:
+        for mode in (1, 0):
+            set_fill_lds_resident(mode)
+            res = FillBatcher().run(db)
+            synchronize()
+            outs.append((res.status.copy(), res.all_vertices().copy(), res.all_indices().copy()))
+    finally:
+        set_fill_lds_resident(True)
+    (s1, v1, i1), (s0, v0, i0) = outs
+    assert np.all(s1 == 0) and np.array_equal(s1, s0)
+    assert np.array_equal(v1.view(np.uint32), v0.view(np.uint32))
+    assert np.array_equal(i1, i0)
+
+
 def test_very_large_path():
     """fill_tests.rs:2140-2166: a path of 1,000,000 endpoints (two wiggling sides of a long
     band) fills without error; the same vertices and triangle count as the oracle and the
