@@ -251,8 +251,8 @@ def main():
         log(f"  stroke split into parts of {w} windows (0: whole paths): {sms:.2f} ms")
     set_stroke_split(64)
 
-    # the same step captured once into a HIP graph (count -> scans -> emit on fixed
-    # buffers) and replayed with no host synchronisation inside
+    # the same step captured once into a HIP graph (the staged pipeline on fixed buffers)
+    # and replayed with no host synchronisation inside
     from lyon_amd.batch import StrokeGraph
     graph = None
     try:
@@ -267,8 +267,10 @@ def main():
         gms = (time.perf_counter() - t0) * 1e3 / args.steps
         graph = {"ms_per_step": gms, "value": tris_per_step / (gms / 1e3),
                  "same_counts_as_staged": bool(ok),
-                 "note": "lyg_stroke_batch_graph: count pass, scans and emit pass captured on "
-                         "fixed output buffers, replayed back to back"}
+                 "note": "lyg_stroke_batch_graph: the default staged pipeline (plan, scans, "
+                         "cut, heads, emit of paths and parts into staging, check, scans, "
+                         "compaction, second walk of any path that needs it) captured on fixed "
+                         "output buffers, replayed back to back"}
         log(f"  stroke graph replay: {gms:.2f} ms")
         del g
     except Exception as e:  # reported, not fatal: the headline does not depend on it
