@@ -336,6 +336,19 @@ typedef struct lyg_graph lyg_graph;
 int32_t lyg_stroke_batch_graph(lyg_ctx* ctx, const lyg_batch_in* in,
                                const lyg_stroke_options* opts, uint32_t n_opts,
                                const lyg_batch_out* out, lyg_graph** graph);
+/* The batched fill's first sweep round as HIP graphs (no lyon counterpart): the round's
+ * plan (size classes, placement forms, staging and working memory) is made at capture
+ * time from the batch's event bounds and intersection estimates, as lyg_fill_batch's first
+ * round makes it; a launch replays the sweep launches (on the context's class streams, at
+ * once), the offset scans and the copy (each path's range checked against the output
+ * capacity on the device) with no host synchronisation inside, and writes
+ * out->path_status when it is set.  lyg_graph_result waits and returns lyg_fill_batch's
+ * statuses, plus LYG_ERR_NO_SPACE when a path outgrew the room planned at capture (its
+ * data changed since; lyg_fill_batch sweeps it again with more room).  The graph owns its
+ * working, staging and status memory; out's arrays must outlive it. */
+int32_t lyg_fill_batch_graph(lyg_ctx* ctx, const lyg_batch_in* in,
+                             const lyg_fill_options* opts, uint32_t n_opts,
+                             const lyg_batch_out* out, lyg_graph** graph);
 int32_t lyg_graph_launch(lyg_graph* graph);
 int32_t lyg_graph_result(lyg_graph* graph, uint64_t* n_vertices, uint64_t* n_indices);
 void lyg_graph_destroy(lyg_graph* graph);

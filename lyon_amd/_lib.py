@@ -157,6 +157,8 @@ SIGNATURES = {
                               U8P, F32P, U32P, U32P, U64P, U64P], C.c_int32),
     "lyg_stroke_batch_graph": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
                                 C.POINTER(BatchOutC), C.POINTER(C.c_void_p)], C.c_int32),
+    "lyg_fill_batch_graph": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
+                              C.POINTER(BatchOutC), C.POINTER(C.c_void_p)], C.c_int32),
     "lyg_graph_launch": ([C.c_void_p], C.c_int32),
     "lyg_graph_result": ([C.c_void_p, U64P, U64P], C.c_int32),
     "lyg_graph_destroy": ([C.c_void_p], None),

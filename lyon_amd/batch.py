@@ -235,6 +235,9 @@ class StrokeGraph:
     device buffers: ``launch()`` replays count -> scan -> emit with no host sync inside;
     ``result()`` waits and returns a BatchResult (raises on an output overflow)."""
 
+    _entry = "lyg_stroke_batch_graph"
+    _status_out = False
+
     def __init__(self, db: DeviceBatch, vertex_capacity: int, index_capacity: int,
                  vertex_offset: int = 0):
         self.db = db
@@ -244,11 +247,12 @@ class StrokeGraph:
         self.ioffs = DeviceBuffer(4 * (db.n + 1))
         self.status = DeviceBuffer(4 * max(db.n, 1))
         out = B.BatchOutC(self.vertices.ptr, self.indices.ptr, vertex_capacity, index_capacity,
-                          self.voffs.ptr, self.ioffs.ptr, vertex_offset, 0, 0, None)
+                          self.voffs.ptr, self.ioffs.ptr, vertex_offset, 0, 0,
+                          self.status.ptr if self._status_out else None)
         cin = db.c_in()
         h = C.c_void_p(0)
-        st = B.lib().lyg_stroke_batch_graph(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts,
-                                            C.byref(out), C.byref(h))
+        st = getattr(B.lib(), self._entry)(_Ctx.get(), C.byref(cin), db.opts.ptr, db.n_opts,
+                                           C.byref(out), C.byref(h))
         if st != 0:
             raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
         self._h = h.value
@@ -258,11 +262,13 @@ class StrokeGraph:
         if st != 0:
             raise TessellationError(st)
 
-    def result(self) -> BatchResult:
+    def result(self, check=True) -> BatchResult:
+        """With check=False a status that belongs to paths (a path's error, an output
+        overflow, a path that outgrew its room) is left in ``status``, not raised."""
         nv = C.c_uint64(0)
         ni = C.c_uint64(0)
         st = B.lib().lyg_graph_result(self._h, C.byref(nv), C.byref(ni))
-        if st != 0:
+        if st != 0 and (check or st in (17, 18, 19)):
             raise TessellationError(st)
         return BatchResult(self.vertices, self.indices, self.voffs, self.ioffs, self.status,
                            self.db.n, nv.value, ni.value)
@@ -274,6 +280,17 @@ class StrokeGraph:
                 self._h = None
         except Exception:
             pass
+
+
+class FillGraph(StrokeGraph):
+    """The batched fill's first sweep round captured into HIP graphs
+    (lyg_fill_batch_graph), planned from db at capture time: ``launch()`` replays the
+    sweep launches, offset scans and copy with no host sync inside and writes each path's
+    status; ``result()`` waits and returns a BatchResult (raises as lyg_fill_batch does,
+    and with LYG_ERR_NO_SPACE when a path outgrew its planned room)."""
+
+    _entry = "lyg_fill_batch_graph"
+    _status_out = True
 
 
 def autotune(batcher: _Batcher, db: DeviceBatch) -> dict:

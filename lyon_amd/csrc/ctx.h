@@ -130,13 +130,19 @@ hipError_t lyg_launch_scan_tiles(const uint32_t* counts, uint32_t n, uint32_t* o
 }
 
 // Exclusive scan of n counts into offsets (n + 1) and *total: one block for small n, the
-// whole GPU (with the context's tile buffer) beyond 32768 counts.
-inline hipError_t ctx_scan(lyg_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* offsets,
-                           uint64_t* total, uint32_t* overflow, hipStream_t s) {
+// whole GPU (with the tile buffer `tiles`) beyond 32768 counts.  A captured graph passes a
+// buffer of its own: a later call can reallocate the context's.
+inline hipError_t scan_with(lyg_ctx* c, DevBuf& tiles, const uint32_t* counts, uint32_t n,
+                            uint32_t* offsets, uint64_t* total, uint32_t* overflow,
+                            hipStream_t s) {
     if (n > 32768) {
-        if (!ensure(c, c->scan_tiles, lyg_scan_tiles_bytes(n))) return hipErrorOutOfMemory;
+        if (!ensure(c, tiles, lyg_scan_tiles_bytes(n))) return hipErrorOutOfMemory;
         return lyg_launch_scan_tiles(counts, n, offsets, total, overflow,
-                                     static_cast<uint64_t*>(c->scan_tiles.p), s);
+                                     static_cast<uint64_t*>(tiles.p), s);
     }
     return lyg_launch_scan(counts, n, offsets, total, overflow, s);
+}
+inline hipError_t ctx_scan(lyg_ctx* c, const uint32_t* counts, uint32_t n, uint32_t* offsets,
+                           uint64_t* total, uint32_t* overflow, hipStream_t s) {
+    return scan_with(c, c->scan_tiles, counts, n, offsets, total, overflow, s);
 }

@@ -31,6 +31,8 @@ size_t lyg_fill_box_bytes();
 hipError_t lyg_launch_fill_inter(const lyg::FillLaunch* L, uint32_t n, void* boxes, uint32_t* inter,
                                  hipStream_t s);
 hipError_t lyg_launch_fill_copy(const lyg::FillLaunch* L, hipStream_t s);
+hipError_t lyg_launch_fill_status_out(const uint32_t* st, uint32_t* out, uint32_t n,
+                                      hipStream_t s);
 hipError_t lyg_launch_fill_list_u32(uint32_t* a, const uint32_t* list, uint32_t m, uint32_t v,
                                     hipStream_t s);
 hipError_t lyg_launch_scan(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint64_t* total,
@@ -149,6 +151,171 @@ bool ensure_at(lyg_ctx* c, std::vector<DevBuf>& v, size_t i, size_t bytes) {
     return ensure(c, v[i], bytes);
 }
 
+// One sweep launch of a round: the paths list[k0, k1) of one size class and placement
+// form, `blocks` blocks of `wpb` waves, each wave with `stride` bytes of working memory at
+// `work_off` + its slot and `hot` bytes of LDS.
+struct Launch {
+    size_t k0, k1, blocks, stride, work_off;
+    uint64_t est;   // the class's work: its largest path's events + 2 x intersections
+    FillCaps cap;   // working-memory capacities
+    int form;       // placement form of the hot arrays
+    FillCaps lc;    // their LDS capacities (forms 1 and 2)
+    uint32_t wpb, hot;
+};
+
+// The paths of one round in launch order with their staging offsets, and its launches.
+struct RoundPlan {
+    std::vector<uint32_t> list;
+    std::vector<uint64_t> offs;  // staging offsets: vertices [0, m], triangles [m + 1, 2m + 1]
+    std::vector<Launch> launches;
+    size_t work_bytes = 0;       // working memory of all launches
+};
+
+// Plans round `round` over the paths `todo`, with `scale` times the first round's room.
+int32_t plan_round(lyg_ctx* c, const std::vector<uint32_t>& todo,
+                   const std::vector<uint32_t>& bound, const std::vector<uint32_t>& inter,
+                   const std::vector<uint64_t>& need, int round, uint64_t scale, RoundPlan* P) {
+    // order by launch: size class, then the placement form (the first round picks it
+    // per path; retries migrate); then by path
+    std::vector<int> cls(todo.size());
+    for (size_t k = 0; k < todo.size(); ++k) {
+        const uint32_t p = todo[k];
+        FillCaps lc;
+        const int form = round == 0 ? path_form(bound[p], c->fill_lds_resident, &lc)
+                                    : FORM_MIGRATE;
+        cls[k] = N_FORMS * size_class(need[p]) + form;
+    }
+    std::vector<size_t> order(todo.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] < cls[b]; });
+    std::vector<uint32_t>& list = P->list;
+    std::vector<uint64_t>& offs = P->offs;
+    list.assign(todo.size(), 0);
+    offs.assign(2 * (todo.size() + 1), 0);
+    uint64_t* sv = offs.data();
+    uint64_t* st = offs.data() + todo.size() + 1;
+    sv[0] = st[0] = 0;
+    for (size_t k = 0; k < order.size(); ++k) {
+        uint32_t p = todo[order[k]];
+        list[k] = p;
+        uint64_t v, t;
+        stage_caps(bound[p], inter[p], scale, &v, &t);
+        sv[k + 1] = sv[k] + v;
+        st[k + 1] = st[k] + t;
+    }
+    const size_t m = list.size();
+    if (8 * sv[m] + 12 * st[m] > STAGE_BUDGET) {
+        c->last_error = "fill staging exceeds the per-round budget";
+        return LYG_ERR_NO_SPACE;
+    }
+    // One launch per size class and form, each with its own part of the working memory.
+    const uint32_t wpb = c->fill_waves_per_block;
+    const uint32_t hot = c->fill_hot_bytes;
+    const int occ = std::max(1, lyg_fill_occupancy(wpb, hot));
+    std::vector<Launch>& launches = P->launches;
+    size_t want_bytes = 0;
+    for (size_t k0 = 0; k0 < m;) {
+        int kc = cls[order[k0]];
+        size_t k1 = k0;
+        uint64_t maxb = 0, maxi = 0;
+        while (k1 < m && cls[order[k1]] == kc) {
+            maxb = std::max<uint64_t>(maxb, bound[list[k1]]);
+            maxi = std::max<uint64_t>(maxi, inter[list[k1]]);
+            ++k1;
+        }
+        Launch ln;
+        ln.k0 = k0;
+        ln.k1 = k1;
+        ln.est = 0;
+        for (size_t k = k0; k < k1; ++k) ln.est = std::max<uint64_t>(ln.est, need[list[k]]);
+        ln.cap = class_caps(std::max<uint64_t>(maxb, 1), maxi, scale);
+        ln.form = kc % N_FORMS;
+        ln.lc = FillCaps{};
+        ln.wpb = wpb;
+        ln.hot = hot;
+        int ln_occ = occ;
+        if (ln.form != FORM_MIGRATE) {
+            // every capacity grows with the bound: the largest path's fit them all
+            (void)path_form(maxb, c->fill_lds_resident, &ln.lc);
+            if (ln.form == FORM_LDS) ln.lc = lds_caps(maxb, c->fill_lds_resident);
+            ln.hot = (uint32_t)((lyg_fill_lds_bytes(&ln.lc) + 255) & ~(size_t)255);
+            ln.wpb = (uint32_t)std::max<size_t>(
+                1, std::min<size_t>(SWEEP_MAX_WAVES, LDS_CU_BYTES / ln.hot));
+            ln_occ = lyg_fill_occupancy_form(ln.wpb, ln.hot, ln.form);
+            while (ln_occ < 1 && ln.wpb > 1)
+                ln_occ = lyg_fill_occupancy_form(--ln.wpb, ln.hot, ln.form);
+            if (ln_occ < 1) {  // cannot launch: the migrating form
+                ln.form = FORM_MIGRATE;
+                ln.wpb = wpb;
+                ln.hot = hot;
+                ln_occ = occ;
+            }
+        }
+        if (ln.form != FORM_MIGRATE) {
+            // arrays that live in LDS need no global region (their overflow retries):
+            // the heap is most of a wave's worst-case working memory
+            ln.cap.heap = 0;
+            ln.cap.below = 0;
+            if (ln.form == FORM_LDS) ln.cap.span = 0;
+        }
+        ln.stride = lyg_fill_work_bytes(&ln.cap, ln.form != FORM_MIGRATE ? 0 : ln.hot);
+        size_t blocks = (k1 - k0 + ln.wpb - 1) / ln.wpb;
+        ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * ln_occ));
+        ln.work_off = 0;
+        want_bytes += ln.blocks * ln.stride * ln.wpb;
+        launches.push_back(ln);
+        k0 = k1;
+    }
+    // over the budget: every class gives up blocks in proportion
+    size_t& work_bytes = P->work_bytes;
+    work_bytes = 0;
+    for (Launch& ln : launches) {
+        if (want_bytes > WORK_BUDGET)
+            ln.blocks = std::max<size_t>(
+                1, (size_t)((double)ln.blocks * (double)WORK_BUDGET / (double)want_bytes));
+        ln.work_off = work_bytes;
+        work_bytes += (ln.blocks * ln.stride * ln.wpb + 255) & ~(size_t)255;
+    }
+    return LYG_OK;
+}
+
+// The stream of every launch, for CLASS_STREAMS streams that run at once (a process gets
+// four hardware queues; the context's own stream holds the fourth): classes go largest
+// first (the last launch first) to the stream with the least work queued.
+std::vector<size_t> assign_streams(const RoundPlan& P, size_t n_streams) {
+    std::vector<size_t> stream_of(P.launches.size(), 0);
+    uint64_t load[CLASS_STREAMS] = {};
+    for (size_t r = 0; r < P.launches.size(); ++r) {
+        const size_t li = P.launches.size() - 1 - r;
+        size_t si = 0;
+        for (size_t q = 1; q < n_streams; ++q)
+            if (load[q] < load[si]) si = q;
+        load[si] += P.launches[li].est;
+        stream_of[li] = si;
+    }
+    return stream_of;
+}
+
+// The fields of L that launch li of plan P sets: its paths (from the round's list and
+// staging offsets on the device), working memory, capacities, form and work counter.
+void set_launch(lyg::FillLaunch& L, const RoundPlan& P, size_t li, uint8_t* work,
+                const uint32_t* d_list, const uint64_t* d_off, uint32_t* counters) {
+    const Launch& ln = P.launches[li];
+    const size_t m = P.list.size();
+    L.work = work + ln.work_off;
+    L.work_stride = ln.stride;
+    L.caps = ln.cap;
+    L.hot_bytes = ln.hot;
+    L.waves_per_block = ln.wpb;
+    L.form = (uint32_t)ln.form;
+    L.lds_caps = ln.lc;
+    L.path_list = d_list + ln.k0;
+    L.n_list = (uint32_t)(ln.k1 - ln.k0);
+    L.stage_v_off = d_off + ln.k0;
+    L.stage_t_off = d_off + (m + 1) + ln.k0;
+    L.work_counter = counters + li;  // one per launch
+}
+
 struct FillJob {
     lyg::PathBatchDev in;
     const lyg_fill_options* opts;
@@ -163,6 +330,36 @@ struct FillResult {
     int rounds = 0;
     std::vector<uint32_t> round_len;  // paths swept in each round
 };
+
+// Step 1 and the intersection estimate, read back to the host: every path's event bound
+// and its estimated self-intersections.
+bool measure(lyg_ctx* c, const lyg::FillLaunch& L, hipStream_t s, std::vector<uint32_t>* bound_out,
+             std::vector<uint32_t>* inter_out) {
+    const uint32_t n = L.in.n_paths;
+    if (!ensure(c, c->f_bound, 4ull * n)) return false;
+    std::vector<uint32_t>& bound = *bound_out;
+    std::vector<uint32_t>& inter = *inter_out;
+    bound.assign(n, 0);
+    inter.assign(n, 0);
+    uint32_t n_verbs = 0;
+    if (!check(c, lyg_launch_fill_bound(&L, n, as<uint32_t>(c->f_bound), s), "fill bound") ||
+        !check(c, hipMemcpyAsync(bound.data(), c->f_bound.p, 4ull * n, hipMemcpyDeviceToHost, s),
+               "D2H") ||
+        !check(c, hipMemcpyAsync(&n_verbs, L.in.verb_off + n, 4, hipMemcpyDeviceToHost, s), "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return false;
+    // intersection estimate: one box record per verb, then a pair count per path
+    if (!ensure(c, c->f_inter, 4ull * n) ||
+        !ensure(c, c->f_boxes, lyg_fill_box_bytes() * ((size_t)n_verbs + 1)))
+        return false;
+    if (!check(c, lyg_launch_fill_inter(&L, n, c->f_boxes.p, as<uint32_t>(c->f_inter), s),
+               "fill inter") ||
+        !check(c, hipMemcpyAsync(inter.data(), c->f_inter.p, 4ull * n, hipMemcpyDeviceToHost, s),
+               "D2H") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return false;
+    return true;
+}
 
 // Steps 1-2 and the scan of step 3; per-path offsets go to vtx_off/idx_off (n+1 each).
 int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* idx_off,
@@ -183,24 +380,8 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     L.path_round = as<uint32_t>(c->f_round);
     L.path_hint = as<uint32_t>(c->f_hint);
     L.vertex_offset = J.vertex_offset;
-    std::vector<uint32_t> bound(n), inter(n);
-    uint32_t n_verbs = 0;
-    if (!check(c, lyg_launch_fill_bound(&L, n, as<uint32_t>(c->f_bound), s), "fill bound") ||
-        !check(c, hipMemcpyAsync(bound.data(), c->f_bound.p, 4ull * n, hipMemcpyDeviceToHost, s),
-               "D2H") ||
-        !check(c, hipMemcpyAsync(&n_verbs, J.in.verb_off + n, 4, hipMemcpyDeviceToHost, s), "D2H") ||
-        !check(c, hipStreamSynchronize(s), "sync"))
-        return LYG_ERR_DEVICE;
-    // intersection estimate: one box record per verb, then a pair count per path
-    if (!ensure(c, c->f_inter, 4ull * n) ||
-        !ensure(c, c->f_boxes, lyg_fill_box_bytes() * ((size_t)n_verbs + 1)))
-        return LYG_ERR_DEVICE;
-    if (!check(c, lyg_launch_fill_inter(&L, n, c->f_boxes.p, as<uint32_t>(c->f_inter), s),
-               "fill inter") ||
-        !check(c, hipMemcpyAsync(inter.data(), c->f_inter.p, 4ull * n, hipMemcpyDeviceToHost, s),
-               "D2H") ||
-        !check(c, hipStreamSynchronize(s), "sync"))
-        return LYG_ERR_DEVICE;
+    std::vector<uint32_t> bound, inter;
+    if (!measure(c, L, s, &bound, &inter)) return LYG_ERR_DEVICE;
     // size classes go by the larger of the two room drivers
     std::vector<uint64_t> need(n);
     for (uint32_t i = 0; i < n; ++i) need[i] = (uint64_t)bound[i] + 2ull * inter[i];
@@ -222,37 +403,14 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
     uint64_t scale = 1, next_scale = 4;
     int round = 0;
     for (; round < MAX_ROUNDS && !todo.empty(); ++round, scale = next_scale) {
-        // order by launch: size class, then the placement form (the first round picks it
-        // per path; retries migrate); then by path
-        std::vector<int> cls(todo.size());
-        for (size_t k = 0; k < todo.size(); ++k) {
-            const uint32_t p = todo[k];
-            FillCaps lc;
-            const int form = round == 0 ? path_form(bound[p], c->fill_lds_resident, &lc)
-                                        : FORM_MIGRATE;
-            cls[k] = N_FORMS * size_class(need[p]) + form;
-        }
-        std::vector<size_t> order(todo.size());
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cls[a] < cls[b]; });
-        std::vector<uint32_t> list(todo.size());
-        std::vector<uint64_t> offs(2 * (todo.size() + 1));
-        uint64_t* sv = offs.data();
-        uint64_t* st = offs.data() + todo.size() + 1;
-        sv[0] = st[0] = 0;
-        for (size_t k = 0; k < order.size(); ++k) {
-            uint32_t p = todo[order[k]];
-            list[k] = p;
-            uint64_t v, t;
-            stage_caps(bound[p], inter[p], scale, &v, &t);
-            sv[k + 1] = sv[k] + v;
-            st[k + 1] = st[k] + t;
-        }
+        RoundPlan P;
+        const int32_t planned = plan_round(c, todo, bound, inter, need, round, scale, &P);
+        if (planned != LYG_OK) return planned;
+        const std::vector<uint32_t>& list = P.list;
+        const std::vector<uint64_t>& offs = P.offs;
         const size_t m = list.size();
-        if (8 * sv[m] + 12 * st[m] > STAGE_BUDGET) {
-            c->last_error = "fill staging exceeds the per-round budget";
-            return LYG_ERR_NO_SPACE;
-        }
+        const uint64_t* sv = offs.data();
+        const uint64_t* st = offs.data() + m + 1;
         if (!ensure_at(c, c->f_list, round, 4 * m) ||
             !ensure_at(c, c->f_stage_off, round, 8 * offs.size()) ||
             !ensure_at(c, c->f_stage_v, round, 8 * sv[m]) ||
@@ -273,113 +431,21 @@ int32_t fill_sweep(lyg_ctx* c, const FillJob& J, uint32_t* vtx_off, uint32_t* id
         L.round = (uint32_t)round;
         L.stage_v = as<float>(c->f_stage_v[round]);
         L.stage_t = as<uint32_t>(c->f_stage_t[round]);
-        // One launch per size class, each with its own part of the working memory, on
-        // CLASS_STREAMS streams that run at once (a process gets four hardware queues;
-        // the context's own stream holds the fourth).  Classes go largest first to the
-        // stream with the least work queued, the work of a class being its largest path
-        // (its events plus twice its estimated intersections).
-        struct Launch {
-            size_t k0, k1, blocks, stride, work_off;
-            uint64_t est;
-            FillCaps cap;
-            int form;
-            FillCaps lc;
-            uint32_t wpb, hot;
-        };
-        const uint32_t wpb = c->fill_waves_per_block;
-        const uint32_t hot = c->fill_hot_bytes;
-        const int occ = std::max(1, lyg_fill_occupancy(wpb, hot));
-        std::vector<Launch> launches;
-        size_t want_bytes = 0;
-        for (size_t k0 = 0; k0 < m;) {
-            int kc = cls[order[k0]];
-            size_t k1 = k0;
-            uint64_t maxb = 0, maxi = 0;
-            while (k1 < m && cls[order[k1]] == kc) {
-                maxb = std::max<uint64_t>(maxb, bound[list[k1]]);
-                maxi = std::max<uint64_t>(maxi, inter[list[k1]]);
-                ++k1;
-            }
-            Launch ln;
-            ln.k0 = k0;
-            ln.k1 = k1;
-            ln.est = 0;
-            for (size_t k = k0; k < k1; ++k) ln.est = std::max<uint64_t>(ln.est, need[list[k]]);
-            ln.cap = class_caps(std::max<uint64_t>(maxb, 1), maxi, scale);
-            ln.form = kc % N_FORMS;
-            ln.lc = FillCaps{};
-            ln.wpb = wpb;
-            ln.hot = hot;
-            int ln_occ = occ;
-            if (ln.form != FORM_MIGRATE) {
-                // every capacity grows with the bound: the largest path's fit them all
-                (void)path_form(maxb, c->fill_lds_resident, &ln.lc);
-                if (ln.form == FORM_LDS) ln.lc = lds_caps(maxb, c->fill_lds_resident);
-                ln.hot = (uint32_t)((lyg_fill_lds_bytes(&ln.lc) + 255) & ~(size_t)255);
-                ln.wpb = (uint32_t)std::max<size_t>(
-                    1, std::min<size_t>(SWEEP_MAX_WAVES, LDS_CU_BYTES / ln.hot));
-                ln_occ = lyg_fill_occupancy_form(ln.wpb, ln.hot, ln.form);
-                while (ln_occ < 1 && ln.wpb > 1)
-                    ln_occ = lyg_fill_occupancy_form(--ln.wpb, ln.hot, ln.form);
-                if (ln_occ < 1) {  // cannot launch: the migrating form
-                    ln.form = FORM_MIGRATE;
-                    ln.wpb = wpb;
-                    ln.hot = hot;
-                    ln_occ = occ;
-                }
-            }
-            if (ln.form != FORM_MIGRATE) {
-                // arrays that live in LDS need no global region (their overflow retries):
-                // the heap is most of a wave's worst-case working memory
-                ln.cap.heap = 0;
-                ln.cap.below = 0;
-                if (ln.form == FORM_LDS) ln.cap.span = 0;
-            }
-            ln.stride = lyg_fill_work_bytes(&ln.cap, ln.form != FORM_MIGRATE ? 0 : ln.hot);
-            size_t blocks = (k1 - k0 + ln.wpb - 1) / ln.wpb;
-            ln.blocks = std::max<size_t>(1, std::min<size_t>(blocks, (size_t)c->cu_count * ln_occ));
-            ln.work_off = 0;
-            want_bytes += ln.blocks * ln.stride * ln.wpb;
-            launches.push_back(ln);
-            k0 = k1;
-        }
-        // over the budget: every class gives up blocks in proportion
-        size_t work_bytes = 0;
-        for (Launch& ln : launches) {
-            if (want_bytes > WORK_BUDGET)
-                ln.blocks = std::max<size_t>(
-                    1, (size_t)((double)ln.blocks * (double)WORK_BUDGET / (double)want_bytes));
-            ln.work_off = work_bytes;
-            work_bytes += (ln.blocks * ln.stride * ln.wpb + 255) & ~(size_t)255;
-        }
-        if (!ensure(c, c->f_work, work_bytes)) return LYG_ERR_NO_SPACE;
+        const std::vector<Launch>& launches = P.launches;
+        if (!ensure(c, c->f_work, P.work_bytes)) return LYG_ERR_NO_SPACE;
         const auto t_round = std::chrono::steady_clock::now();
         if (c->timing && round == 0) (void)hipEventRecord(c->ev[2], s);
         const size_t n_streams = std::min<size_t>(launches.size(), CLASS_STREAMS);
         if (!class_streams(c, n_streams, launches.size()) ||
             !check(c, hipEventRecord(c->class_ev[0], s), "event"))
             return LYG_ERR_DEVICE;
-        uint64_t load[CLASS_STREAMS] = {};
+        const std::vector<size_t> stream_of = assign_streams(P, n_streams);
         for (size_t r = 0; r < launches.size(); ++r) {
             const size_t li = launches.size() - 1 - r;  // largest class first
             const Launch& ln = launches[li];
-            size_t si = 0;
-            for (size_t q = 1; q < n_streams; ++q)
-                if (load[q] < load[si]) si = q;
-            load[si] += ln.est;
-            hipStream_t cs = c->class_stream[si];
-            L.work = as<uint8_t>(c->f_work) + ln.work_off;
-            L.work_stride = ln.stride;
-            L.caps = ln.cap;
-            L.hot_bytes = ln.hot;
-            L.waves_per_block = ln.wpb;
-            L.form = (uint32_t)ln.form;
-            L.lds_caps = ln.lc;
-            L.path_list = d_list + ln.k0;
-            L.n_list = (uint32_t)(ln.k1 - ln.k0);
-            L.stage_v_off = d_off + ln.k0;
-            L.stage_t_off = d_off + (m + 1) + ln.k0;
-            L.work_counter = as<uint32_t>(c->f_counters) + li;  // one per launch
+            hipStream_t cs = c->class_stream[stream_of[li]];
+            set_launch(L, P, li, as<uint8_t>(c->f_work), d_list, d_off,
+                       as<uint32_t>(c->f_counters));
             const auto t_launch = std::chrono::steady_clock::now();
             if (!check(c, hipStreamWaitEvent(cs, c->class_ev[0], 0), "wait") ||
                 !check(c, lyg_launch_fill_sweep(&L, (int)ln.blocks, cs), "fill sweep") ||
@@ -473,6 +539,8 @@ int32_t fill_copy(lyg_ctx* c, const FillJob& J, const FillResult& R, const uint3
     L.out_indices = out_i;
     L.vertex_offset = J.vertex_offset;
     L.local_ids = J.local_ids ? 1u : 0u;
+    L.vertex_capacity = ~0ull;  // the totals were checked against the output on the host
+    L.index_capacity = ~0ull;
     (void)n;
     if (c->timing) (void)hipEventRecord(c->ev[0], c->stream);
     for (int r = 0; r < R.rounds; ++r) {
@@ -520,6 +588,255 @@ int32_t lygi::fill_batch(lyg_ctx* c, const lygi::BatchSpec& b, const lygi::OutAl
     if (st != LYG_OK) return st;
     if (!check(c, hipStreamSynchronize(c->stream), "sync")) return LYG_ERR_DEVICE;
     return LYG_OK;
+}
+
+// ---- HIP graph form -------------------------------------------------------------------
+// The first round of a batched fill, planned at capture time from the batch's bounds and
+// intersection estimates (size classes, placement forms, staging and working memory, the
+// streams of the launches), is replayed without the host: a head graph on the context's
+// stream (counter reset, unswept status), one graph per class stream holding that stream's
+// sweep launches, and a tail graph (offset scans, copy with the output capacity checked on
+// the device, per-path statuses).  The class streams' graphs are launched on the context's
+// own side streams, ordered by events: the launches of one round must run at once, and
+// streams that the runtime would make for the branches of a single graph can share a
+// hardware queue (a process has four), which would run them one after another.
+// Every buffer the replay touches belongs to the graph: a later call can reallocate the
+// context's.  A path that outgrows its planned room in a replay (its data changed since
+// the capture) produces nothing and reports LYG_ERR_NO_SPACE; lyg_fill_batch sweeps it
+// again with more room.
+struct lygi::FillGraph {
+    lyg_ctx* c = nullptr;
+    uint32_t n = 0;
+    uint32_t vertex_offset = 0;
+    uint64_t vertex_capacity = 0, index_capacity = 0;
+    DevBuf status, round, nv, ni, list, off, stage_v, stage_t, work, counters, totals, tiles;
+    std::vector<hipGraph_t> graphs;
+    hipGraphExec_t head = nullptr, tail = nullptr;
+    std::vector<hipGraphExec_t> part;  // per class stream
+    std::vector<hipStream_t> part_stream;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+};
+
+namespace {
+
+// Captures what `body` enqueues on s into an instantiated graph.
+template <class F>
+bool capture_on(lyg_ctx* c, hipStream_t s, lygi::FillGraph* g, hipGraphExec_t* exec, F&& body) {
+    hipGraph_t graph = nullptr;
+    if (!check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture"))
+        return false;
+    const bool ok = body();
+    const bool ended = check(c, hipStreamEndCapture(s, &graph), "end capture");
+    if (graph) g->graphs.push_back(graph);
+    return ok && ended && graph &&
+           check(c, hipGraphInstantiate(exec, graph, nullptr, nullptr, 0), "instantiate");
+}
+
+}  // namespace
+
+int32_t lygi::fill_graph_capture(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_options* opts,
+                                 uint32_t n_opts, const lyg_batch_out* out, FillGraph** graph) {
+    const uint32_t n = in->n_paths;
+    hipStream_t s = c->stream;
+    auto* g = new FillGraph();
+    *graph = g;  // the caller destroys it on an error
+    g->c = c;
+    g->n = n;
+    g->vertex_offset = out->vertex_offset;
+    g->vertex_capacity = out->vertex_capacity;
+    g->index_capacity = out->index_capacity;
+    if (!ensure(c, g->status, 4ull * n) || !ensure(c, g->round, 4ull * n) ||
+        !ensure(c, g->nv, 4ull * n) || !ensure(c, g->ni, 4ull * n) ||
+        !ensure(c, g->counters, 4ull * (MAX_LAUNCHES + 1)) || !ensure(c, g->totals, 16) ||
+        // the scans' tile buffer is allocated before the capture, not inside it
+        (n > 32768 && !ensure(c, g->tiles, lyg_scan_tiles_bytes(n))))
+        return LYG_ERR_DEVICE;
+    uint32_t* counters = as<uint32_t>(g->counters);
+    uint32_t* overflow = counters + MAX_LAUNCHES;  // the offset scans' 32-bit overflow flag
+    lyg::FillLaunch L{};
+    L.in.verbs = in->verbs;
+    L.in.points = in->points;
+    L.in.verb_off = in->verb_offsets;
+    L.in.point_off = in->point_offsets;
+    L.in.n_paths = n;
+    L.opts = opts;
+    L.opts_stride = n_opts == 1 ? 0 : 1;
+    L.path_nv = as<uint32_t>(g->nv);
+    L.path_ni = as<uint32_t>(g->ni);
+    L.path_status = as<uint32_t>(g->status);
+    L.path_round = as<uint32_t>(g->round);
+    L.path_hint = nullptr;
+    L.vertex_offset = out->vertex_offset;
+    L.round = 0;
+    // the plan: lyg_fill_batch's first round over every path
+    std::vector<uint32_t> bound, inter;
+    if (!measure(c, L, s, &bound, &inter)) return LYG_ERR_DEVICE;
+    std::vector<uint64_t> need(n);
+    std::vector<uint32_t> todo(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (bound[i] > MAX_PATH_EVENTS) {
+            c->last_error = "a path too large for a captured fill";
+            return LYG_ERR_NO_SPACE;
+        }
+        need[i] = (uint64_t)bound[i] + 2ull * inter[i];
+        todo[i] = i;
+    }
+    RoundPlan P;
+    const int32_t planned = plan_round(c, todo, bound, inter, need, 0, 1, &P);
+    if (planned != LYG_OK) return planned;
+    const size_t m = P.list.size();
+    const uint64_t* sv = P.offs.data();
+    const uint64_t* st = P.offs.data() + m + 1;
+    if (!ensure(c, g->list, 4 * m) || !ensure(c, g->off, 8 * P.offs.size()) ||
+        !ensure(c, g->stage_v, 8 * sv[m]) || !ensure(c, g->stage_t, 12 * st[m]))
+        return LYG_ERR_DEVICE;
+    if (!ensure(c, g->work, P.work_bytes)) return LYG_ERR_NO_SPACE;
+    uint32_t* d_list = as<uint32_t>(g->list);
+    uint64_t* d_off = as<uint64_t>(g->off);
+    if (!check(c, hipMemcpyAsync(d_list, P.list.data(), 4 * m, hipMemcpyHostToDevice, s), "H2D") ||
+        !check(c, hipMemcpyAsync(d_off, P.offs.data(), 8 * P.offs.size(), hipMemcpyHostToDevice, s),
+               "H2D") ||
+        !check(c, hipStreamSynchronize(s), "sync"))
+        return LYG_ERR_DEVICE;
+    L.stage_v = as<float>(g->stage_v);
+    L.stage_t = as<uint32_t>(g->stage_t);
+    const size_t n_streams = std::min<size_t>(P.launches.size(), CLASS_STREAMS);
+    if (!class_streams(c, n_streams, P.launches.size()) ||
+        !check(c, hipEventCreateWithFlags(&g->fork, hipEventDisableTiming), "hipEventCreate"))
+        return LYG_ERR_DEVICE;
+    const std::vector<size_t> stream_of = assign_streams(P, n_streams);
+    // head: counters and the status a path keeps if no launch reaches it
+    if (!capture_on(c, s, g, &g->head, [&]() {
+            return check(c, lyg_launch_fill_u32(counters, MAX_LAUNCHES + 1, 0, s), "reset") &&
+                   check(c, lyg_launch_fill_list_u32(L.path_status, d_list, (uint32_t)m,
+                                                     ST_UNSWEPT, s),
+                         "status");
+        }))
+        return LYG_ERR_DEVICE;
+    // one graph per class stream: its launches, largest class first
+    for (size_t si = 0; si < n_streams; ++si) {
+        hipStream_t cs = c->class_stream[si];
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t ev = nullptr;
+        if (!check(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate"))
+            return LYG_ERR_DEVICE;
+        g->join.push_back(ev);
+        const bool ok = capture_on(c, cs, g, &exec, [&]() {
+            for (size_t r = 0; r < P.launches.size(); ++r) {
+                const size_t li = P.launches.size() - 1 - r;
+                if (stream_of[li] != si) continue;
+                set_launch(L, P, li, as<uint8_t>(g->work), d_list, d_off, counters);
+                if (!check(c, lyg_launch_fill_sweep(&L, (int)P.launches[li].blocks, cs),
+                           "fill sweep"))
+                    return false;
+            }
+            return true;
+        });
+        if (exec) {
+            g->part.push_back(exec);
+            g->part_stream.push_back(cs);
+        }
+        if (!ok) return LYG_ERR_DEVICE;
+    }
+    // tail: offsets, copy (each path's range checked against the output capacity), statuses
+    lyg::FillLaunch K{};
+    K.in = L.in;
+    K.path_status = L.path_status;
+    K.path_round = L.path_round;
+    K.vtx_off = out->path_vertex_offsets;
+    K.idx_off = out->path_index_offsets;
+    K.out_vertices = out->vertices;
+    K.out_indices = out->indices;
+    K.vertex_offset = out->vertex_offset;
+    K.local_ids = 0;
+    K.vertex_capacity = out->vertex_capacity;
+    K.index_capacity = out->index_capacity;
+    K.round = 0;
+    K.path_list = d_list;
+    K.n_list = (uint32_t)m;
+    K.stage_v = L.stage_v;
+    K.stage_t = L.stage_t;
+    K.stage_v_off = d_off;
+    K.stage_t_off = d_off + (m + 1);
+    uint64_t* totals = as<uint64_t>(g->totals);
+    if (!capture_on(c, s, g, &g->tail, [&]() {
+            return check(c, scan_with(c, g->tiles, L.path_nv, n, out->path_vertex_offsets,
+                                      totals + 0, overflow, s),
+                         "scan") &&
+                   check(c, scan_with(c, g->tiles, L.path_ni, n, out->path_index_offsets,
+                                      totals + 1, overflow, s),
+                         "scan") &&
+                   check(c, lyg_launch_fill_copy(&K, s), "fill copy") &&
+                   (!out->path_status ||
+                    check(c, lyg_launch_fill_status_out(
+                              L.path_status, reinterpret_cast<uint32_t*>(out->path_status), n, s),
+                          "status"));
+        }))
+        return LYG_ERR_DEVICE;
+    return LYG_OK;
+}
+
+int32_t lygi::fill_graph_launch(FillGraph* g) {
+    lyg_ctx* c = g->c;
+    hipStream_t s = c->stream;
+    if (!check(c, hipGraphLaunch(g->head, s), "graph launch") ||
+        !check(c, hipEventRecord(g->fork, s), "event"))
+        return LYG_ERR_DEVICE;
+    for (size_t i = 0; i < g->part.size(); ++i) {
+        hipStream_t cs = g->part_stream[i];
+        if (!check(c, hipStreamWaitEvent(cs, g->fork, 0), "wait") ||
+            !check(c, hipGraphLaunch(g->part[i], cs), "graph launch") ||
+            !check(c, hipEventRecord(g->join[i], cs), "event") ||
+            !check(c, hipStreamWaitEvent(s, g->join[i], 0), "wait"))
+            return LYG_ERR_DEVICE;
+    }
+    return check(c, hipGraphLaunch(g->tail, s), "graph launch") ? LYG_OK : LYG_ERR_DEVICE;
+}
+
+// Waits for the last replay; statuses in lyg_fill_batch's order: 32-bit offsets or vertex
+// ids out of range, then the output capacity, then the first failing path.
+int32_t lygi::fill_graph_result(FillGraph* g, uint64_t* n_vertices, uint64_t* n_indices) {
+    lyg_ctx* c = g->c;
+    hipStream_t s = c->stream;
+    uint64_t tot[2] = {0, 0};
+    uint32_t ovf = 0;
+    std::vector<uint32_t> st(g->n);
+    if (!check(c, hipStreamSynchronize(s), "sync") ||
+        !check(c, hipMemcpy(tot, g->totals.p, 16, hipMemcpyDeviceToHost), "D2H") ||
+        !check(c, hipMemcpy(&ovf, as<uint32_t>(g->counters) + MAX_LAUNCHES, 4,
+                            hipMemcpyDeviceToHost),
+               "D2H") ||
+        !check(c, hipMemcpy(st.data(), g->status.p, 4ull * g->n, hipMemcpyDeviceToHost), "D2H"))
+        return LYG_ERR_DEVICE;
+    if (n_vertices) *n_vertices = tot[0];
+    if (n_indices) *n_indices = tot[1];
+    if (ovf) return LYG_ERR_TOO_MANY_VERTICES;
+    // BuffersBuilder::add_vertex_impl: ids up to u32::MAX - 1 (geometry_builder.rs:338-350)
+    if (tot[0] > 0 && (uint64_t)g->vertex_offset + tot[0] - 1 > 0xfffffffeull)
+        return LYG_ERR_TOO_MANY_VERTICES;
+    if (tot[0] > g->vertex_capacity || tot[1] > g->index_capacity) return LYG_ERR_CAPACITY;
+    for (uint32_t v : st) {
+        if (v == 0) continue;
+        if (v == ST_RETRY) return LYG_ERR_NO_SPACE;
+        if (v == ST_UNSWEPT) return LYG_ERR_INTERNAL;
+        return (int32_t)v;
+    }
+    return LYG_OK;
+}
+
+void lygi::fill_graph_destroy(FillGraph* g) {
+    if (!g) return;
+    if (g->head) (void)hipGraphExecDestroy(g->head);
+    if (g->tail) (void)hipGraphExecDestroy(g->tail);
+    for (hipGraphExec_t e : g->part) (void)hipGraphExecDestroy(e);
+    for (hipGraph_t gr : g->graphs) (void)hipGraphDestroy(gr);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    for (hipEvent_t e : g->join) (void)hipEventDestroy(e);
+    for (DevBuf* b : {&g->status, &g->round, &g->nv, &g->ni, &g->list, &g->off, &g->stage_v,
+                      &g->stage_t, &g->work, &g->counters, &g->totals, &g->tiles})
+        free_buf(*b);
+    delete g;
 }
 
 extern "C" {

@@ -1,4 +1,4 @@
-// HIP graph form of the batched stroke and the launch-shape autotuner.
+// HIP graph forms of the batched stroke and fill, and the launch-shape autotuner.
 //
 // lyg_stroke_batch_graph captures one batched stroke into a HIP graph bound to the caller's
 // device buffers: the default pipeline (plan, scans, cut, heads, item emit, check, scans,
@@ -8,6 +8,7 @@
 // checks the output capacity on the device (a path that does not fit is skipped and
 // reported), so the host reads the totals and the status only when it wants them.
 // Counters are cleared by a fill kernel inside the graph (no memset nodes).
+// lyg_fill_batch_graph (fill_api.cpp) captures the batched fill's first sweep round.
 //
 // The autotuner times the caller's batch under each candidate launch shape with HIP events
 // and keeps the fastest in the context.
@@ -62,6 +63,8 @@ struct lyg_graph {
     uint32_t cap_extra = 0;
     DevBuf items, heads, spec, end, parts, parts_off, rv, ri, voff, ioff, stage_v, stage_i,
         fallback;
+    DevBuf scan_tiles;  // the graph's own: a later call can reallocate the context's
+    lygi::FillGraph* fill = nullptr;  // a captured fill (lyg_fill_batch_graph)
 };
 
 namespace {
@@ -149,11 +152,14 @@ bool capture_stroke_items(lyg_ctx* c, lyg_graph* g, const lyg_batch_in* in,
     const int pgrid = (int)std::min<uint32_t>(n, (uint32_t)std::max(1, c->cu_count) * 32u);
     const uint32_t W = c->stroke_split_windows;
     const float room = c->stroke_room_scale;
+    auto scan = [&](const uint32_t* counts, uint32_t* offsets, uint64_t* total) {
+        return check(c, scan_with(c, g->scan_tiles, counts, n, offsets, total, g->counters + 4, s),
+                     "scan");
+    };
     auto plan_and_scans = [&]() {
         return check(c, lyg_launch_stroke_plan(&L, n, W, room, parts, rv, ri, pgrid, s), "plan") &&
-               check(c, ctx_scan(c, parts, n, parts_off, g->totals + 2, g->counters + 4, s), "scan") &&
-               check(c, ctx_scan(c, rv, n, voff, g->totals + 0, g->counters + 4, s), "scan") &&
-               check(c, ctx_scan(c, ri, n, ioff, g->totals + 1, g->counters + 4, s), "scan");
+               scan(parts, parts_off, g->totals + 2) && scan(rv, voff, g->totals + 0) &&
+               scan(ri, ioff, g->totals + 1);
     };
     // 1. sizes, eagerly
     uint64_t h[3] = {0, 0, 0};
@@ -232,10 +238,8 @@ bool capture_stroke_items(lyg_ctx* c, lyg_graph* g, const lyg_batch_in* in,
     ok = ok && check(c, lyg_launch_stroke_heads(&L, n, pgrid, s), "heads");
     ok = ok && check(c, lyg_launch_stroke(&E, 1, egrid, s), "stroke emit (items)");
     ok = ok && check(c, lyg_launch_stroke_items_verify(&L, n, pgrid, s), "verify");
-    ok = ok && check(c, ctx_scan(c, L.path_nv, n, out->path_vertex_offsets, g->totals + 3,
-                                 g->counters + 4, s), "scan");
-    ok = ok && check(c, ctx_scan(c, L.path_ni, n, out->path_index_offsets, g->totals + 4,
-                                 g->counters + 4, s), "scan");
+    ok = ok && scan(L.path_nv, out->path_vertex_offsets, g->totals + 3);
+    ok = ok && scan(L.path_ni, out->path_index_offsets, g->totals + 4);
     ok = ok && check(c, lyg_launch_stroke_items_compact(&C, L.items, L.heads, (uint32_t)cap_items,
                                                         (int)std::min<uint64_t>(cap_items, 1u << 20),
                                                         s), "compact");
@@ -269,6 +273,8 @@ int32_t lyg_stroke_batch_graph(lyg_ctx* c, const lyg_batch_in* in, const lyg_str
     if (!check(c, hipMalloc(&g->counters, 64), "hipMalloc") ||
         !check(c, hipMalloc(&g->totals, 64), "hipMalloc") ||
         !ensure(c, g->counts_v, 4ull * in->n_paths) || !ensure(c, g->counts_i, 4ull * in->n_paths) ||
+        // the scans' tile buffer is allocated before the capture, not inside it
+        (in->n_paths > 32768 && !ensure(c, g->scan_tiles, lyg_scan_tiles_bytes(in->n_paths))) ||
         !(c->stroke_staged && c->stroke_split_windows > 0
               ? capture_stroke_items(c, g, in, opts, n_opts, out)
               : capture_stroke(c, g, in, opts, n_opts, out))) {
@@ -279,7 +285,29 @@ int32_t lyg_stroke_batch_graph(lyg_ctx* c, const lyg_batch_in* in, const lyg_str
     return LYG_OK;
 }
 
+int32_t lyg_fill_batch_graph(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_options* opts,
+                             uint32_t n_opts, const lyg_batch_out* out, lyg_graph** graph) {
+    if (!c || !in || !opts || !out || !graph) return LYG_ERR_INVALID_ARGUMENT;
+    if (n_opts != 1 && n_opts != in->n_paths) return LYG_ERR_INVALID_ARGUMENT;
+    if (in->n_paths == 0 || !out->path_vertex_offsets || !out->path_index_offsets)
+        return LYG_ERR_INVALID_ARGUMENT;
+    *graph = nullptr;
+    if (!check(c, hipSetDevice(c->device), "hipSetDevice")) return LYG_ERR_DEVICE;
+    auto* g = new lyg_graph();
+    g->ctx = c;
+    g->vertex_capacity = out->vertex_capacity;
+    g->index_capacity = out->index_capacity;
+    const int32_t st = lygi::fill_graph_capture(c, in, opts, n_opts, out, &g->fill);
+    if (st != LYG_OK) {
+        lyg_graph_destroy(g);
+        return st;
+    }
+    *graph = g;
+    return LYG_OK;
+}
+
 int32_t lyg_graph_launch(lyg_graph* g) {
+    if (g && g->fill) return lygi::fill_graph_launch(g->fill);
     if (!g || !g->exec) return LYG_ERR_INVALID_ARGUMENT;
     return check(g->ctx, hipGraphLaunch(g->exec, g->ctx->stream), "graph launch") ? LYG_OK
                                                                                   : LYG_ERR_DEVICE;
@@ -287,6 +315,7 @@ int32_t lyg_graph_launch(lyg_graph* g) {
 
 int32_t lyg_graph_result(lyg_graph* g, uint64_t* n_vertices, uint64_t* n_indices) {
     if (!g) return LYG_ERR_INVALID_ARGUMENT;
+    if (g->fill) return lygi::fill_graph_result(g->fill, n_vertices, n_indices);
     lyg_ctx* c = g->ctx;
     uint64_t tot[5] = {0, 0, 0, 0, 0};
     uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -313,6 +342,7 @@ int32_t lyg_graph_result(lyg_graph* g, uint64_t* n_vertices, uint64_t* n_indices
 
 void lyg_graph_destroy(lyg_graph* g) {
     if (!g) return;
+    lygi::fill_graph_destroy(g->fill);
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->counters) (void)hipFree(g->counters);
@@ -320,7 +350,8 @@ void lyg_graph_destroy(lyg_graph* g) {
     free_buf(g->counts_v);
     free_buf(g->counts_i);
     for (DevBuf* b : {&g->items, &g->heads, &g->spec, &g->end, &g->parts, &g->parts_off, &g->rv,
-                      &g->ri, &g->voff, &g->ioff, &g->stage_v, &g->stage_i, &g->fallback})
+                      &g->ri, &g->voff, &g->ioff, &g->stage_v, &g->stage_i, &g->fallback,
+                      &g->scan_tiles})
         free_buf(*b);
     delete g;
 }

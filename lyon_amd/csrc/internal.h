@@ -34,6 +34,15 @@ int32_t fill_batch_banded(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc,
 int32_t fill_batch(lyg_ctx* c, const BatchSpec& b, const OutAlloc& alloc, uint64_t* tv,
                    uint64_t* ti, std::vector<uint32_t>* status);
 
+// A batched fill captured as HIP graphs (lyg_fill_batch_graph): the first sweep round,
+// planned from the batch at capture time, the offset scans and the copy.
+struct FillGraph;
+int32_t fill_graph_capture(lyg_ctx* c, const lyg_batch_in* in, const lyg_fill_options* opts,
+                           uint32_t n_opts, const lyg_batch_out* out, FillGraph** graph);
+int32_t fill_graph_launch(FillGraph* g);
+int32_t fill_graph_result(FillGraph* g, uint64_t* n_vertices, uint64_t* n_indices);
+void fill_graph_destroy(FillGraph* g);
+
 // A context; side_streams: make the fill's side streams now (lyg_ctx_create does, so that
 // they get hardware queues of their own) rather than on the first fill (the serving loop's
 // workers, whose own streams should each keep a queue).

@@ -169,6 +169,9 @@ struct FillLaunch {
     uint32_t* out_indices;
     uint32_t vertex_offset;
     uint32_t local_ids;            // ids count from vertex_offset in every path
+    // output capacity (vertices, indices): a path whose range ends past it is not copied
+    uint64_t vertex_capacity;
+    uint64_t index_capacity;
 };
 
 struct FlattenLaunch {
