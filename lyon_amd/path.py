@@ -70,11 +70,17 @@ class PathEvent:
 class Path:
     """An immutable lyon path: ``verbs`` (uint8) and ``points`` (float32, shape (n, 2))."""
 
-    __slots__ = ("verbs", "points")
+    __slots__ = ("verbs", "points", "attributes")
 
-    def __init__(self, verbs, points):
+    def __init__(self, verbs, points, attributes=None):
         self.verbs = np.ascontiguousarray(verbs, dtype=np.uint8)
         self.points = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 2)
+        # per point, the custom attributes of the endpoint stored there (lyon's
+        # builder_with_attributes; control points' rows are NaN): float32 (n, k) or None
+        self.attributes = None
+        if attributes is not None:
+            a = np.ascontiguousarray(attributes, dtype=np.float32)
+            self.attributes = a.reshape(self.points.shape[0], -1)
 
     @staticmethod
     def builder() -> "PathBuilder":
@@ -146,38 +152,57 @@ class Path:
 class PathBuilder:
     """Mirror of lyon_path's ``BuilderImpl`` (path.rs:472-611)."""
 
-    def __init__(self):
+    def __init__(self, num_attributes: int = 0):
+        """num_attributes > 0: Path::builder_with_attributes (path.rs:115-118); every
+        endpoint then takes that many custom attributes."""
         self._verbs: list[int] = []
         self._points: list[tuple[float, float]] = []
         self._first = (0.0, 0.0)
         self._in_sub_path = False
+        self._n_attr = int(num_attributes)
+        self._attrs: list[list[float]] = []  # one row per point
+        self._first_attrs: list[float] = []
 
-    def begin(self, at):
+    def _row(self, attributes, control=0):
+        if self._n_attr == 0:
+            return []
+        for _ in range(control):
+            self._attrs.append([float("nan")] * self._n_attr)
+        row = [float(a) for a in (attributes if attributes is not None else [])]
+        assert len(row) == self._n_attr, "one value per custom attribute"
+        self._attrs.append(row)
+        return row
+
+    def begin(self, at, attributes=None):
         assert not self._in_sub_path, "begin() inside a sub-path"
         self._in_sub_path = True
         self._first = (float(at[0]), float(at[1]))
         self._points.append(self._first)
+        self._first_attrs = self._row(attributes)
         self._verbs.append(VERB_BEGIN)
         return len(self._points) - 1
 
-    def line_to(self, to):
+    def line_to(self, to, attributes=None):
         assert self._in_sub_path, "line_to() outside a sub-path"
         self._points.append((float(to[0]), float(to[1])))
+        self._row(attributes)
         self._verbs.append(VERB_LINE)
         return len(self._points) - 1
 
-    def quadratic_bezier_to(self, ctrl, to):
+    def quadratic_bezier_to(self, ctrl, to, attributes=None):
         assert self._in_sub_path
         self._points.append((float(ctrl[0]), float(ctrl[1])))
         self._points.append((float(to[0]), float(to[1])))
+        self._row(attributes, control=1)
         self._verbs.append(VERB_QUADRATIC)
         return len(self._points) - 1
 
-    def cubic_bezier_to(self, ctrl1, ctrl2, to):
+    def cubic_bezier_to(self, ctrl1, ctrl2, to, attributes=None):
         assert self._in_sub_path
         self._points.append((float(ctrl1[0]), float(ctrl1[1])))
         self._points.append((float(ctrl2[0]), float(ctrl2[1])))
         self._points.append((float(to[0]), float(to[1])))
+        self._row(attributes, control=2)
         self._verbs.append(VERB_CUBIC)
         return len(self._points) - 1
 
@@ -186,6 +211,8 @@ class PathBuilder:
         self._in_sub_path = False
         if close:
             self._points.append(self._first)
+            if self._n_attr:
+                self._attrs.append(list(self._first_attrs))
         self._verbs.append(VERB_CLOSE if close else VERB_END)
 
     def close(self):
@@ -245,7 +272,10 @@ class PathBuilder:
     def build(self) -> Path:
         assert not self._in_sub_path, "build() with an open sub-path"
         pts = np.asarray(self._points, dtype=np.float32).reshape(-1, 2)
-        return Path(np.asarray(self._verbs, dtype=np.uint8), pts)
+        attrs = None
+        if self._n_attr:
+            attrs = np.asarray(self._attrs, dtype=np.float32).reshape(-1, self._n_attr)
+        return Path(np.asarray(self._verbs, dtype=np.uint8), pts, attrs)
 
 
 def polygon(points, closed: bool = True) -> Path:

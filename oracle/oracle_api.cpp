@@ -59,6 +59,26 @@ orc_result* orc_stroke_path(const uint8_t* verbs, uint32_t n_verbs, const float*
     return r;
 }
 
+// The same with StrokeOptions::variable_line_width: widths[i] is the width factor of the
+// endpoint stored at point i (tessellate_with_ids_vw, stroke.rs:653-767).
+orc_result* orc_stroke_path_vw(const uint8_t* verbs, uint32_t n_verbs, const float* points,
+                               uint32_t n_points, const float* widths, float line_width,
+                               float miter_limit, float tolerance, int start_cap, int end_cap,
+                               int line_join, uint32_t vertex_offset) {
+    orc_result* r = new orc_result();
+    refpath::PathView pv{verbs, n_verbs, points, n_points};
+    refstroke::StrokeOptions o;
+    o.line_width = line_width;
+    o.miter_limit = miter_limit;
+    o.tolerance = tolerance;
+    o.start_cap = start_cap;
+    o.end_cap = end_cap;
+    o.line_join = line_join;
+    refstroke::Output out{&r->vertices, &r->indices, vertex_offset};
+    r->status = refstroke::stroke_path_vw(pv, o, widths, out);
+    return r;
+}
+
 // FillTessellator::tessellate_path into BuffersBuilder (fill.rs:628-641).
 orc_result* orc_fill_path(const uint8_t* verbs, uint32_t n_verbs, const float* points,
                           uint32_t n_points, float tolerance, int fill_rule,

@@ -36,6 +36,10 @@ def lib():
         L.orc_stroke_path.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float, C.c_float,
                                       C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint32]
         L.orc_stroke_path.restype = C.c_void_p
+        L.orc_stroke_path_vw.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, F32P, C.c_float,
+                                         C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                         C.c_uint32]
+        L.orc_stroke_path_vw.restype = C.c_void_p
         L.orc_fill_path.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float, C.c_int,
                                     C.c_int, C.c_int, C.c_uint32]
         L.orc_fill_path.restype = C.c_void_p
@@ -85,10 +89,20 @@ def _collect(h):
 
 
 def stroke(path, line_width=1.0, miter_limit=4.0, tolerance=0.1, start_cap=0, end_cap=0,
-           line_join=0, vertex_offset=0):
-    """StrokeTessellator::tessellate_path on the CPU.  Returns (status, vertices, indices)."""
+           line_join=0, vertex_offset=0, widths=None):
+    """StrokeTessellator::tessellate_path on the CPU.  Returns (status, vertices, indices).
+    widths: per point of the path, the width factor of the endpoint stored there
+    (StrokeOptions::variable_line_width); None strokes at the fixed width."""
     v = np.ascontiguousarray(path.verbs, np.uint8)
     p = np.ascontiguousarray(path.points, np.float32)
+    if widths is not None:
+        w = np.ascontiguousarray(widths, np.float32)
+        assert len(w) == len(p)
+        h = lib().orc_stroke_path_vw(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p),
+                                     _ptr(w, F32P), line_width, miter_limit, tolerance,
+                                     start_cap, end_cap, line_join, vertex_offset)
+        st, vv, ii, _ = _collect(h)
+        return st, vv, ii
     h = lib().orc_stroke_path(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), line_width,
                               miter_limit, tolerance, start_cap, end_cap, line_join,
                               vertex_offset)

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see refgeom.h).
 //
-// Fixed-width StrokeTessellator restated from crates/tessellation/src/stroke.rs.
+// StrokeTessellator restated from crates/tessellation/src/stroke.rs: the fixed-width
+// stroker and the variable-width one (StrokeOptions::variable_line_width).
 // Function names follow the reference; each cites the lines it restates.
 #include "refstroke.h"
 
@@ -94,6 +95,8 @@ struct Builder {
     float square_merge_threshold;
     bool may_need_empty_cap = false;
     bool error = false;
+    bool vw = false;  // variable_line_width is set
+    float sub_path_start_advancement = 0.0f;
 
     // StrokeBuilderImpl::new (stroke.rs:591-631)
     Builder(const StrokeOptions& o, Output& out) : options(o), output(out) {
@@ -277,7 +280,9 @@ struct Builder {
     }
 
     // stroke.rs:1743-1806.  Returns false on a builder error; *skipped mirrors Ok(true).
-    bool flattened_step(EndpointData& prev, EndpointData& join, EndpointData& next) {
+    bool flattened_step(EndpointData& prev, EndpointData& join, EndpointData& next,
+                        bool* skipped = nullptr) {
+        if (skipped) *skipped = false;
         V2 prev_edge = join.position - prev.position;
         float prev_length = length(prev_edge);
         V2 prev_tangent = prev_edge / prev_length;
@@ -285,6 +290,8 @@ struct Builder {
         float next_length = length(next_edge);
         V2 next_tangent = next_edge / next_length;
         V2 normal = compute_normal(prev_tangent, next_tangent);
+        if (std::isnan(join.advancement)) join.advancement = prev.advancement + prev_length;
+        if (std::isnan(next.advancement)) next.advancement = join.advancement + next_length;
 
         V2 p0 = join.position + normal * vertex.half_width;
         join.side_points[SIDE_POSITIVE].prev = p0;
@@ -299,7 +306,10 @@ struct Builder {
 
         V2 v0 = p0 - prev.side_points[SIDE_POSITIVE].next;
         V2 v1 = p1 - prev.side_points[SIDE_NEGATIVE].next;
-        if (dot(prev_edge, v0) < 0.0f && dot(prev_edge, v1) < 0.0f) return true;
+        if (dot(prev_edge, v0) < 0.0f && dot(prev_edge, v1) < 0.0f) {
+            if (skipped) *skipped = true;
+            return true;
+        }
 
         vertex.normal = normal;
         vertex.side = SIDE_POSITIVE;
@@ -366,9 +376,129 @@ struct Builder {
         return true;
     }
 
+    // stroke.rs:1809-1836
+    static void compute_edge_attachment_positions(EndpointData& p0, EndpointData& p1) {
+        V2 edge = p1.position - p0.position;
+        float d = length(edge);
+        float edge_angle = angle_from_x_axis(edge);
+        // sin(vwidth_angle) = (hw1 - hw0) / d; NaN (|.| > 1) counts as 0
+        float sin_vwidth_angle = (p1.half_width - p0.half_width) / d;
+        float vwidth_angle = std::asin(sin_vwidth_angle);
+        if (std::isnan(vwidth_angle)) vwidth_angle = 0.0f;
+        compute_side_attachment_positions(p0, p1, edge_angle, vwidth_angle, SIDE_POSITIVE);
+        compute_side_attachment_positions(p0, p1, edge_angle, vwidth_angle, SIDE_NEGATIVE);
+        if (std::isnan(p1.advancement)) p1.advancement = p0.advancement + d;
+    }
+
+    // stroke.rs:1838-1866 (side_sign: stroke.rs:2324-2330)
+    static void compute_side_attachment_positions(EndpointData& p0, EndpointData& p1,
+                                                  float edge_angle, float vwidth_angle,
+                                                  int side) {
+        float nl = side == SIDE_NEGATIVE ? -1.0f : 1.0f;
+        float normal_angle = edge_angle + nl * (PI_F * 0.5f + vwidth_angle);
+        V2 normal = v2(std::cos(normal_angle), std::sin(normal_angle));
+        p0.side_points[side].next = p0.position + normal * p0.half_width;
+        p1.side_points[side].prev = p1.position + normal * p1.half_width;
+    }
+
+    // stroke.rs:2060-2128
+    static void compute_join_side_positions(const EndpointData& prev, EndpointData& join,
+                                            const EndpointData& next, float miter_limit,
+                                            int side) {
+        float sign = side == SIDE_NEGATIVE ? -1.0f : 1.0f;
+        V2 v0 = normalize(join.side_points[side].prev - prev.side_points[side].next);
+        V2 v1 = normalize(next.side_points[side].prev - join.side_points[side].next);
+        bool inward = cross(v0, v1) * sign > 0.0f;
+        bool forward = dot(v0, v1) > 0.0f;
+        V2 normal = compute_normal(v0, v1) * sign;
+        V2 path_v0 = normalize(join.position - prev.position);
+        V2 path_v1 = normalize(next.position - join.position);
+        bool normal_same_side = dot(v0 + v1, path_v0 + path_v1) >= 0.0f;
+        bool angle_is_sharp = inward && !forward && normal_same_side;
+        if (angle_is_sharp) {
+            V2 extruded_normal = normal * join.half_width;
+            float prev_length = join.advancement - prev.advancement;
+            float next_length = next.advancement - join.advancement;
+            float d_next = dot(extruded_normal, v1) - next_length;
+            float d_prev = dot(extruded_normal, -v0) - prev_length;
+            if (fminr(d_next, d_prev) > 0.0f || square_length(normal) < 1e-5f)
+                join.fold[side] = true;
+        }
+        bool concave = inward && normal_same_side && !join.fold[side];
+        if (concave || ((join.line_join == JOIN_MITER || join.line_join == JOIN_MITER_CLIP) &&
+                        !miter_limit_is_exceeded(normal, miter_limit))) {
+            join.side_points[side].has_single = true;
+            join.side_points[side].single_vertex = join.position + normal * join.half_width;
+        } else if (join.line_join == JOIN_MITER_CLIP) {
+            V2 n0 = join.side_points[side].prev - join.position;
+            V2 n1 = join.side_points[side].next - join.position;
+            V2 pn, nn;
+            get_clip_intersections(n0, n1, normal, miter_limit * join.half_width, &pn, &nn);
+            join.side_points[side].prev = join.position + pn;
+            join.side_points[side].next = join.position + nn;
+        }
+    }
+
+    // stroke.rs:1380-1512 (the variable-width step)
+    bool step_impl_vw(EndpointData next, bool* added) {
+        size_t count = point_buffer.count();
+        if (count > 0 && points_are_too_close(point_buffer.last().position, next.position)) {
+            if (count == 1) may_need_empty_cap = true;
+            *added = false;
+            return true;
+        }
+        if (count > 0) {
+            EndpointData& join = point_buffer.last();
+            if (!join.is_flattening_step || !next.is_flattening_step)
+                compute_edge_attachment_positions(join, next);
+        }
+        bool skip = false;
+        if (count > 1) {
+            EndpointData& prev = point_buffer.second_last();
+            EndpointData& join = point_buffer.last_of_two();
+            vertex.position_on_path = join.position;
+            vertex.half_width = join.half_width;
+            bool fast_path = false;
+            if (join.is_flattening_step) {
+                V2 v0 = join.position - prev.position;
+                V2 v1 = next.position - join.position;
+                fast_path = dot(v0, v1) > 0.0f;
+            }
+            if (fast_path) {
+                join.line_join = JOIN_MITER;
+                if (!flattened_step(prev, join, next, &skip)) return false;
+            } else {
+                compute_join_side_positions(prev, join, next, options.miter_limit, SIDE_POSITIVE);
+                compute_join_side_positions(prev, join, next, options.miter_limit, SIDE_NEGATIVE);
+                // no folding when the other side is concave
+                if (join.side_points[SIDE_POSITIVE].has_single) join.fold[SIDE_NEGATIVE] = false;
+                if (join.side_points[SIDE_NEGATIVE].has_single) join.fold[SIDE_POSITIVE] = false;
+                if (!add_join_base_vertices(join, SIDE_NEGATIVE)) return false;
+                if (!add_join_base_vertices(join, SIDE_POSITIVE)) return false;
+            }
+            if (!skip) {
+                if (count > 2) add_edge_triangles(prev, join);
+                if (!tessellate_join(join)) return false;
+                if (count == 2) {
+                    firsts[0] = prev;
+                    firsts[1] = join;
+                    n_firsts = 2;
+                }
+            }
+        }
+        if (skip) point_buffer.replace_last(next);
+        else point_buffer.push(next);
+        *added = true;
+        return true;
+    }
+
+    bool step_any(const EndpointData& next, bool* added) {
+        return vw ? step_impl_vw(next, added) : step_impl(next, added);
+    }
+
     void step(const EndpointData& next) {
         bool added;
-        if (!step_impl(next, &added)) error = true;
+        if (!step_any(next, &added)) error = true;
     }
 
     EndpointData make_point(V2 pos, bool is_flattening_step) const {
@@ -454,6 +584,7 @@ struct Builder {
     // stroke.rs:2130-2206
     bool tessellate_last_edge(const EndpointData& p0, EndpointData& p1, bool is_first_edge) {
         V2 v = p1.position - p0.position;
+        p1.advancement = p0.advancement + length(v);
         vertex.position_on_path = p1.position;
         vertex.half_width = p1.half_width;
         for (int side = 0; side < 2; ++side) {
@@ -527,12 +658,15 @@ struct Builder {
         if (count >= 2) {
             EndpointData p0 = point_buffer.second_last();
             EndpointData p1 = point_buffer.last_of_two();
-            V2 t = normalize(p1.position - p0.position);
-            V2 n = v2(-t.y, t.x) * p1.half_width;
-            p1.side_points[SIDE_POSITIVE].prev = p1.position + n;
-            p1.side_points[SIDE_NEGATIVE].prev = p1.position - n;
+            if (!vw) {  // the fixed-width step leaves the last edge's side points unset
+                V2 t = normalize(p1.position - p0.position);
+                V2 n = v2(-t.y, t.x) * p1.half_width;
+                p1.side_points[SIDE_POSITIVE].prev = p1.position + n;
+                p1.side_points[SIDE_NEGATIVE].prev = p1.position - n;
+            }
             bool is_first = count == 2;
             if (!tessellate_last_edge(p0, p1, is_first)) return false;
+            sub_path_start_advancement = p1.advancement;
             if (count > 2) {
                 p0 = firsts[0];
                 p1 = firsts[1];
@@ -549,13 +683,14 @@ struct Builder {
         }
         if (point_buffer.count() <= 2) return true;
         EndpointData p = firsts[0];
+        p.advancement = NAN;  // recomputed, not the sub-path start's
         bool segment_added;
-        if (!step_impl(p, &segment_added)) return false;
+        if (!step_any(p, &segment_added)) return false;
         if (!segment_added) point_buffer.last().position = p.position;
         if (n_firsts >= 2) {
             EndpointData p2 = firsts[1];
             bool dummy;
-            if (!step_impl(p2, &dummy)) return false;
+            if (!step_any(p2, &dummy)) return false;
             EndpointData& p0 = point_buffer.second_last();
             EndpointData& p1 = point_buffer.last_of_two();
             vertex.position_on_path = p0.position;
@@ -602,6 +737,42 @@ struct Builder {
         }
     }
 
+    // quadratic_bezier_to with widths (stroke.rs:997-1030) over flatten_quad
+    // (stroke.rs:2755-2779): the half width at each flattened point interpolates the
+    // endpoints' widths at its t on the whole curve.
+    void quadratic_to_vw(V2 from, V2 ctrl, V2 to, float w0, float w1) {
+        float t_split;
+        float tol = options.tolerance;
+        auto at = [&](V2 p, float t, bool flat) {
+            EndpointData e = make_point(p, flat);
+            e.half_width = (w0 * (1.0f - t) + w1 * t) * 0.5f;
+            step(e);
+        };
+        if (find_sharp_turn(from, ctrl, to, &t_split)) {
+            V2 a[3], b[3];
+            quad_split(from, ctrl, to, t_split, a, b);
+            quad_flatten_with_t(a[0], a[1], a[2], tol, [&](V2, V2 p, float, float t1) {
+                at(p, t1 * t_split, t1 != 1.0f);
+            });
+            quad_flatten_with_t(b[0], b[1], b[2], tol, [&](V2, V2 p, float, float t1) {
+                at(p, t_split + t1 * (1.0f - t_split), t1 != 1.0f);
+            });
+        } else {
+            quad_flatten_with_t(from, ctrl, to, tol,
+                                [&](V2, V2 p, float, float t1) { at(p, t1, t1 != 1.0f); });
+        }
+    }
+
+    // cubic_bezier_to with widths (stroke.rs:1032-1067)
+    void cubic_to_vw(V2 from, V2 c1, V2 c2, V2 to, float w0, float w1) {
+        cubic_flatten_with_t(from, c1, c2, to, options.tolerance,
+                             [&](V2, V2 p, float, float t1) {
+                                 EndpointData e = make_point(p, t1 != 1.0f);
+                                 e.half_width = (w0 * (1.0f - t1) + w1 * t1) * 0.5f;
+                                 step(e);
+                             });
+    }
+
     // cubic_bezier_to_fw (stroke.rs:1145-1177)
     void cubic_to(V2 from, V2 c1, V2 c2, V2 to) {
         cubic_flatten_with_t(from, c1, c2, to, options.tolerance,
@@ -638,6 +809,67 @@ int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output
                 V2 from = current;
                 current = e.to;
                 b.cubic_to(from, e.ctrl1, e.ctrl2, e.to);
+                break;
+            }
+            case refpath::EV_END:
+                b.end(e.close);
+                break;
+        }
+        if (b.error) {
+            out.abort_geometry();
+            return TOO_MANY_VERTICES;
+        }
+    }
+    return OK;
+}
+
+// StrokeTessellator::tessellate_path -> tessellate_with_ids_vw (stroke.rs:653-767): each
+// endpoint's half width is line_width * its width factor * 0.5; curves interpolate the
+// full widths of their two endpoints.
+int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
+                   const float* widths, Output& out) {
+    Builder b(opts, out);
+    b.vw = true;
+    refpath::Iter it(path);
+    refpath::Event e;
+    V2 current{NAN, NAN};
+    float cur_w = NAN;  // width factor of the current endpoint
+    auto width_at = [&](uint32_t pi) { return pi < path.n_points ? widths[pi] : NAN; };
+    while (it.next(e)) {
+        const uint32_t pi = it.pi - 1;  // the event's endpoint (Begin, Line, Quad, Cubic)
+        switch (e.kind) {
+            case refpath::EV_BEGIN: {
+                current = e.to;
+                cur_w = width_at(pi);
+                b.may_need_empty_cap = false;
+                EndpointData p = b.make_point(e.to, false);
+                p.half_width = opts.line_width * cur_w * 0.5f;
+                p.advancement = b.sub_path_start_advancement;
+                b.step(p);
+                break;
+            }
+            case refpath::EV_LINE: {
+                current = e.to;
+                cur_w = width_at(pi);
+                EndpointData p = b.make_point(e.to, false);
+                p.half_width = opts.line_width * cur_w * 0.5f;
+                b.step(p);
+                break;
+            }
+            case refpath::EV_QUAD: {
+                V2 from = current;
+                current = e.to;
+                const float w0 = opts.line_width * cur_w;
+                cur_w = width_at(pi);
+                b.quadratic_to_vw(from, e.ctrl1, e.to, w0, opts.line_width * cur_w);
+                break;
+            }
+            case refpath::EV_CUBIC: {
+                V2 from = current;
+                current = e.to;
+                const float w0 = opts.line_width * cur_w;
+                cur_w = width_at(pi);
+                b.cubic_to_vw(from, e.ctrl1, e.ctrl2, e.to, w0, opts.line_width * cur_w);
                 break;
             }
             case refpath::EV_END:

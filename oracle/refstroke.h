@@ -61,11 +61,12 @@ struct SidePoints {
     uint32_t prev_vertex = INVALID_ID, next_vertex = INVALID_ID;
 };
 
-// EndpointData (stroke.rs:320-352); advancement and vertex sources are omitted
-// because they never influence positions or topology.
+// EndpointData (stroke.rs:320-352); vertex sources are omitted because they never
+// influence positions or topology (advancement does, through the variable-width joins).
 struct EndpointData {
     V2 position{0, 0};
     float half_width = NAN;
+    float advancement = NAN;
     int line_join = JOIN_MITER;
     SidePoints side_points[2];
     bool fold[2] = {false, false};
@@ -107,9 +108,15 @@ struct PointBuffer {
     // last_two_mut returns (second to last, last)
     EndpointData& second_last() { return points[(start + count_ - 2) % 3]; }
     EndpointData& last_of_two() { return points[(start + count_ - 1) % 3]; }
+    // replace_last (stroke.rs:2400-2406): the slot last written
+    void replace_last(const EndpointData& p) { points[(start == 0 ? count_ : start) - 1] = p; }
 };
 
 int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output& out);
+// StrokeOptions::variable_line_width: widths[i] is the line-width factor attribute of the
+// endpoint stored at point i (control points' entries are not read).
+int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
+                   const float* widths, Output& out);
 bool find_sharp_turn_export(float fx, float fy, float cx, float cy, float tx, float ty,
                             float* t_out);
 uint32_t arc_subdivisions_export(float diff, float radius, float tolerance);

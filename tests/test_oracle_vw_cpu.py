@@ -1,0 +1,228 @@
+"""The oracle's variable-width stroker (StrokeOptions::variable_line_width,
+stroke.rs:653-767, 1380-1512, 1809-1866, 2060-2128), pinned by lyon's own tests, a closed
+form and the fixed-width stroker it must reduce to."""
+import math
+
+import numpy as np
+import pytest
+
+from lyon_amd.path import PathBuilder
+from oracle import pyoracle as O
+
+BUTT, SQUARE, ROUND = 0, 1, 2
+MITER, MITER_CLIP, ROUND_JOIN, BEVEL = 0, 1, 2, 3
+
+
+def vw(path, **kw):
+    return O.stroke(path, widths=path.attributes[:, 0], **kw)
+
+
+def check_mesh(v, i):
+    """stroke.rs:2787-2849 TestBuilder: no triangle repeats a vertex, every triangle is
+    clockwise (y down) up to -0.035, positions are finite."""
+    assert np.all(np.isfinite(v))
+    t = i.reshape(-1, 3)
+    assert np.all(t[:, 0] != t[:, 1]) and np.all(t[:, 0] != t[:, 2]) and \
+        np.all(t[:, 1] != t[:, 2])
+    a, b, c = v[t[:, 0]].astype(np.float64), v[t[:, 1]].astype(np.float64), \
+        v[t[:, 2]].astype(np.float64)
+    cr = (a[:, 0] - b[:, 0]) * (c[:, 1] - b[:, 1]) - (a[:, 1] - b[:, 1]) * (c[:, 0] - b[:, 0])
+    assert np.all(cr >= -0.035), float(cr.min())
+
+
+def square_path():
+    """stroke.rs:2851-2867 (test_square): the square both ways round, width factor 0.3."""
+    b = PathBuilder(1)
+    for pts in (((-1, 1), (1, 1), (1, -1), (-1, -1)), ((-1, -1), (1, -1), (1, 1), (-1, 1))):
+        b.begin(pts[0], [0.3])
+        for q in pts[1:]:
+            b.line_to(q, [0.3])
+        b.end(False)
+    return b.build()
+
+
+def test_square_kats():
+    """test_square (stroke.rs:2851-2908) with variable_line_width(0): 12 triangles with
+    miter joins and butt caps, 16 with bevel joins and square caps; the clipped-miter and
+    round forms pass the mesh checks."""
+    p = square_path()
+    st, v, i = vw(p, line_join=MITER, start_cap=BUTT, end_cap=BUTT)
+    assert st == 0 and len(i) // 3 == 12
+    check_mesh(v, i)
+    st, v, i = vw(p, line_join=BEVEL, start_cap=SQUARE, end_cap=SQUARE)
+    assert st == 0 and len(i) // 3 == 16
+    check_mesh(v, i)
+    for kw in (dict(line_join=MITER_CLIP, miter_limit=1.0, start_cap=ROUND, end_cap=ROUND),
+               dict(tolerance=0.001, line_join=ROUND_JOIN, start_cap=ROUND, end_cap=ROUND)):
+        st, v, i = vw(p, **kw)
+        assert st == 0
+        check_mesh(v, i)
+
+
+def test_empty_caps():
+    """test_empty_caps (stroke.rs:2918-2976) with variable_line_width(0): no triangles
+    with butt caps, 8 with square caps."""
+    b = PathBuilder(1)
+    b.begin((1, 0), [1.0]); b.end(True)
+    b.begin((2, 0), [1.0]); b.line_to((2, 0), [1.0]); b.end(False)
+    b.begin((3, 0), [1.0]); b.line_to((3, 0), [1.0]); b.end(True)
+    b.begin((3, 0), [1.0])
+    for _ in range(3):
+        b.line_to((3, 0), [1.0])
+    b.end(True)
+    b.begin((4, 0), [1.0]); b.end(False)
+    p = b.build()
+    assert len(vw(p, start_cap=BUTT, end_cap=BUTT)[2]) == 0
+    st, v, i = vw(p, start_cap=SQUARE, end_cap=SQUARE)
+    assert st == 0 and len(i) // 3 == 8
+    st, v, i = vw(p, start_cap=ROUND, end_cap=ROUND)
+    assert st == 0
+    check_mesh(v, i)
+
+
+def test_issue_821_and_894():
+    """issue_821 (stroke.rs:3318-3340) and issue_894 (:3342-3370) tessellate without an
+    error into finite positions (lyon checks no more: both use plain builders)."""
+    b = PathBuilder(1)
+    pts = [(-45.192276, -69.800575), (-45.164116, -69.84931), (-45.135952, -69.90564),
+           (-45.10779, -69.93248), (-45.052788, -69.96064), (-45.026604, -69.898155)]
+    b.begin(pts[0], [1.0])
+    for q in pts[1:]:
+        b.line_to(q, [1.0])
+    b.end(False)
+    st, v, i = vw(b.build(), tolerance=0.001, start_cap=ROUND, end_cap=ROUND)
+    assert st == 0 and len(i) > 0 and np.all(np.isfinite(v))
+    b = PathBuilder(1)
+    b.begin((435.72, 368.42), [38.82])
+    b.line_to((433.53, 366.06), [38.82])
+    b.quadratic_bezier_to((431.35, 363.70), (430.22, 362.52), [39.59])
+    b.quadratic_bezier_to((429.09, 361.34), (429.05, 362.14), [41.62])
+    b.line_to((429.00, 362.95), [41.63])
+    b.end(False)
+    st, v, i = vw(b.build(), tolerance=0.01, start_cap=ROUND, end_cap=ROUND)
+    assert st == 0 and len(i) > 0 and np.all(np.isfinite(v))
+
+
+def mesh_area(v, i):
+    t = i.reshape(-1, 3)
+    a, b, c = v[t[:, 0]].astype(np.float64), v[t[:, 1]].astype(np.float64), \
+        v[t[:, 2]].astype(np.float64)
+    return float(np.sum(np.abs((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) -
+                               (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))) * 0.5)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_single_edge_closed_form(seed):
+    """One straight edge of length d whose ends have half widths h0 and h1: its sides are
+    the common tangents of the two end circles (normal turned by a = asin((h1-h0)/d)), at
+    half distance y(x) = h0 / cos a + x tan a from the edge at x along it.  Butt caps clip
+    them to the end normals: the trapezoid of area d (h0 + h1) / cos a; square caps to the
+    normals h0 before and h1 past the ends: the integral of 2 y over [-h0, d + h1],
+    (2 h0 (d + h0 + h1) + (h1 - h0) ((d + h1)^2 - h0^2) / d) / cos a.  Any direction, any
+    widths with |h1 - h0| < d."""
+    rng = np.random.default_rng(seed)
+    d = float(rng.uniform(20, 200))
+    ang = float(rng.uniform(-math.pi, math.pi))
+    w0, w1 = (float(x) for x in rng.uniform(0.5, 8.0, 2))
+    lw = float(rng.uniform(0.5, 3.0))
+    x0, y0 = (float(x) for x in rng.uniform(-100, 100, 2))
+    b = PathBuilder(1)
+    b.begin((x0, y0), [w0])
+    b.line_to((x0 + d * math.cos(ang), y0 + d * math.sin(ang)), [w1])
+    b.end(False)
+    p = b.build()
+    p0, p1 = p.points.astype(np.float64)
+    dd = float(np.hypot(*(p1 - p0)))
+    h0, h1 = lw * np.float32(w0) * 0.5, lw * np.float32(w1) * 0.5
+    ca = math.sqrt(1.0 - ((h1 - h0) / dd) ** 2)
+    st, v, i = vw(p, line_width=lw, start_cap=BUTT, end_cap=BUTT)
+    assert st == 0 and len(i) // 3 == 2
+    assert mesh_area(v, i) == pytest.approx(dd * (h0 + h1) / ca, rel=2e-4)
+    st, v, i = vw(p, line_width=lw, start_cap=SQUARE, end_cap=SQUARE)
+    assert st == 0
+    want = (2 * h0 * (dd + h0 + h1) + (h1 - h0) * ((dd + h1) ** 2 - h0 ** 2) / dd) / ca
+    assert mesh_area(v, i) == pytest.approx(want, rel=2e-4)
+
+
+def random_path(rng, n, with_curves):
+    b = PathBuilder(1)
+    pts = rng.uniform(0, 400, (n, 2))
+    b.begin(pts[0], [1.0])
+    for k in range(1, n):
+        kind = rng.integers(0, 3) if with_curves else 0
+        if kind == 0:
+            b.line_to(pts[k], [1.0])
+        elif kind == 1:
+            b.quadratic_bezier_to(rng.uniform(0, 400, 2), pts[k], [1.0])
+        else:
+            b.cubic_bezier_to(rng.uniform(0, 400, 2), rng.uniform(0, 400, 2), pts[k], [1.0])
+    b.end(bool(rng.integers(0, 2)))
+    return b.build()
+
+
+def coverage(v, i, lo, n, step):
+    """Which centres of an n x n grid of `step`-sized cells from `lo` some triangle covers."""
+    img = np.zeros((n, n), bool)
+    t = i.reshape(-1, 3)
+    P = v.astype(np.float64)
+    for a, b, c in t:
+        A, B, C = P[a], P[b], P[c]
+        x0 = max(int(np.floor((min(A[0], B[0], C[0]) - lo[0]) / step)), 0)
+        x1 = min(int(np.ceil((max(A[0], B[0], C[0]) - lo[0]) / step)), n)
+        y0 = max(int(np.floor((min(A[1], B[1], C[1]) - lo[1]) / step)), 0)
+        y1 = min(int(np.ceil((max(A[1], B[1], C[1]) - lo[1]) / step)), n)
+        if x1 <= x0 or y1 <= y0:
+            continue
+        xs = lo[0] + (np.arange(x0, x1) + 0.5) * step
+        ys = lo[1] + (np.arange(y0, y1) + 0.5) * step
+        X, Y = np.meshgrid(xs, ys)
+        d1 = (X - B[0]) * (A[1] - B[1]) - (A[0] - B[0]) * (Y - B[1])
+        d2 = (X - C[0]) * (B[1] - C[1]) - (B[0] - C[0]) * (Y - C[1])
+        d3 = (X - A[0]) * (C[1] - A[1]) - (C[0] - A[0]) * (Y - A[1])
+        neg = (d1 < 0) | (d2 < 0) | (d3 < 0)
+        pos = (d1 > 0) | (d2 > 0) | (d3 > 0)
+        img[y0:y1, x0:x1] |= ~(neg & pos)
+    return img
+
+
+@pytest.mark.parametrize("join", [MITER, MITER_CLIP, ROUND_JOIN, BEVEL])
+def test_constant_width_covers_the_fixed_width_stroke(join):
+    """With every width factor 1 the variable-width stroker covers the fixed-width stroke's
+    region.  Vertex for vertex they may differ: its side lines come from euclid's fast
+    atan2 and cos/sin (directions off by up to ~1e-3 rad), which can flip which side of a
+    nearly straight join counts as inward, and it folds sharp joins under its own rule
+    (stroke.rs:2087-2103 against :1687-1698).  On a 0.25-unit grid over random paths of
+    lines and curves (width 6), the cells covered by exactly one of the two meshes are
+    under 0.1% of those covered by either on all but two paths (sharp turns inside cubics
+    with clipped miters reach about 1%) and under 2% on every path, and most paths give
+    the same triangle count.
+    A cubic that turns sharply between two of its flattened points gets NaN vertices from
+    lyon's variable-width step (both points are flattening steps, so no edge attachment is
+    computed, stroke.rs:1395-1401, yet the turn refuses the fast path, :1427-1432, and the
+    join reads the unset side points): such paths are counted, not compared."""
+    rng = np.random.default_rng(100 + join)
+    same_count = compared = loose = 0
+    for k in range(16):
+        p = random_path(rng, int(rng.integers(3, 8)), with_curves=k % 2 == 1)
+        kw = dict(line_width=6.0, tolerance=0.05, line_join=join, miter_limit=4.0,
+                  start_cap=k % 3, end_cap=(k + 1) % 3)
+        sf, vf, i_f = O.stroke(p, **kw)
+        sv, vv, iv = vw(p, **kw)
+        assert sf == 0 and sv == 0
+        assert np.all(np.isfinite(vf))
+        if not np.all(np.isfinite(vv)):
+            assert k % 2 == 1, "NaN vertices on a path of lines"
+            continue
+        compared += 1
+        same_count += len(i_f) == len(iv)
+        lo = np.minimum(vf.min(axis=0), vv.min(axis=0)) - 1.0
+        hi = np.maximum(vf.max(axis=0), vv.max(axis=0)) + 1.0
+        step = 0.25
+        n = int(np.ceil(float(np.max(hi - lo)) / step))
+        cf, cv = coverage(vf, i_f, lo, n, step), coverage(vv, iv, lo, n, step)
+        either = int(np.count_nonzero(cf | cv))
+        differ = int(np.count_nonzero(cf ^ cv))
+        assert differ <= 0.02 * either, (k, differ, either)
+        loose += differ > 0.001 * either
+    assert compared >= 12 and same_count >= compared - 4, (compared, same_count)
+    assert loose <= 2, loose
