@@ -226,6 +226,24 @@ int32_t lyg_stroke_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_stroke_
 int32_t lyg_fill_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_options* opts,
                        uint32_t n_opts, lyg_batch_out* out);
 
+/* ---- variable-width stroke ---------------------------------------------------------------
+ * StrokeTessellator::tessellate_path with StrokeOptions::variable_line_width(width_attribute)
+ * (stroke.rs:653-767, 1380-1512, 1809-1866, 2060-2128): every endpoint carries n_attributes
+ * custom attributes (Path::builder_with_attributes, path.rs:115-118), given per point of
+ * the path(s) — n_points x n_attributes floats, the rows of control points unread — and
+ * its half width is line_width * attributes[width_attribute] * 0.5; curves interpolate the
+ * widths of their endpoints.  Positions agree with lyon's within float rounding of asin /
+ * cos / sin; status as lyg_stroke_path / lyg_stroke_batch, plus LYG_ERR_NO_SPACE for a path
+ * with a curve of more than 4,194,304 flattened points.  One GPU lane strokes one path. */
+int32_t lyg_stroke_path_vw(lyg_ctx* ctx, const lyg_path_view* path, const float* attributes,
+                           uint32_t n_attributes, uint32_t width_attribute,
+                           const lyg_stroke_options* opts, lyg_output* out);
+/* attributes in device memory, per point of the batch (lyg_batch_in's points). */
+int32_t lyg_stroke_batch_vw(lyg_ctx* ctx, const lyg_batch_in* in, const float* attributes,
+                            uint32_t n_attributes, uint32_t width_attribute,
+                            const lyg_stroke_options* opts, uint32_t n_opts,
+                            lyg_batch_out* out);
+
 /* ---- banded fill (no lyon counterpart for the method) ---------------------------------
  * The fill of FillTessellator::tessellate_path (fill.rs:628-641) computed for large paths
  * across the whole GPU: the flattened edges (as lyon's event queue builder flattens them,

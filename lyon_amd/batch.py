@@ -196,6 +196,40 @@ class StrokeBatcher(_Batcher):
         return tv.value, ti.value
 
 
+class StrokeVWBatcher(_Batcher):
+    """Batched variable-width stroke (lyg_stroke_batch_vw): ``attributes`` holds, per point
+    of the batch, the custom attributes of the endpoint stored there (n_points x k float32);
+    column ``width_attribute`` is the width factor."""
+
+    _entry = "lyg_stroke_batch_vw"
+
+    def __init__(self, attributes, width_attribute: int = 0):
+        super().__init__()
+        a = np.ascontiguousarray(attributes, np.float32)
+        self.n_attributes = 1 if a.ndim == 1 else int(a.shape[1])
+        self.width_attribute = int(width_attribute)
+        self.attributes = DeviceBuffer.from_numpy(a.reshape(-1))
+
+    def run(self, db: DeviceBatch, vertex_offset: int = 0, check=True) -> BatchResult:
+        self._ensure(db.n, 0, 0)
+        cin = db.c_in()
+        for _ in range(3):
+            out = B.BatchOutC(self.vertices.ptr, self.indices.ptr, self.vertices.nbytes // 8,
+                              self.indices.nbytes // 4, self.voffs.ptr, self.ioffs.ptr,
+                              vertex_offset, 0, 0, self.status.ptr)
+            st = B.lib().lyg_stroke_batch_vw(_Ctx.get(), C.byref(cin), self.attributes.ptr,
+                                             self.n_attributes, self.width_attribute,
+                                             db.opts.ptr, db.n_opts, C.byref(out))
+            if st == LYG_ERR_CAPACITY:
+                self._ensure(db.n, out.n_vertices, out.n_indices)
+                continue
+            if st != 0 and (check or st in (17, 18, 19)):
+                raise TessellationError(st, B.lib().lyg_ctx_last_error(_Ctx.get()).decode())
+            return BatchResult(self.vertices, self.indices, self.voffs, self.ioffs, self.status,
+                               db.n, out.n_vertices, out.n_indices)
+        raise RuntimeError("batch did not converge")
+
+
 class FillBatcher(_Batcher):
     """Batched fill; with check=False a path's failure is reported in ``status`` only.
     ``banded=True``: every path through the banded fill (lyg_fill_batch_banded)."""

@@ -174,6 +174,24 @@ struct FillLaunch {
     uint64_t index_capacity;
 };
 
+// Variable-width stroke (stroke_vw_kernel): one lane per path, count then emit.
+struct StrokeVWLaunch {
+    PathBatchDev in;
+    const lyg_stroke_options* opts;
+    uint32_t opts_stride;
+    const float* widths;     // per point of the batch: the width factor of its endpoint
+    uint32_t widths_stride;  // floats per point (custom attributes per endpoint)
+    uint32_t* path_nv;       // count pass
+    uint32_t* path_ni;
+    uint32_t* path_status;
+    const uint32_t* vtx_off;  // emit pass
+    const uint32_t* idx_off;
+    float* out_vertices;
+    uint32_t* out_indices;
+    uint32_t vertex_offset;
+    uint32_t* err;
+};
+
 struct FlattenLaunch {
     const int32_t* kinds;  // 0 line, 1 quadratic, 2 cubic
     const float* ctrl;     // 8 floats per segment: from, ctrl1, ctrl2, to

@@ -1394,6 +1394,543 @@ __global__ void __launch_bounds__(256) stroke_compact_kernel(StrokeCompact C) {
     }
 }
 
+// ---- variable-width stroke (StrokeOptions::variable_line_width) ---------------------------
+// lyon's second stroker (stroke.rs:653-767, 1380-1512): every endpoint carries a width
+// factor (a custom attribute), curves interpolate it, and each edge gets its own side lines
+// (tangent to its two end circles), from which every join finds its sides.  Its steps read
+// the side lines of the edges on both sides of a join, so the walk is serial: one lane
+// strokes one path, in a count pass (sizes), then an emit pass at the scanned offsets,
+// through the same vertex/triangle sink, joins' arcs and caps' arithmetic as the
+// fixed-width kernel.
+namespace vw {
+
+static constexpr uint32_t MAX_CURVE_POINTS = 1u << 22;  // per curve; more: LYG_ERR_NO_SPACE
+
+struct Side {
+    vec2 prev, next, single;
+    uint32_t prev_v, next_v;
+    bool has_single;
+};
+// EndpointData (stroke.rs:320-352) without the vertex source
+struct Pt {
+    vec2 pos;
+    float hw, adv;
+    int join;
+    Side sp[2];
+    bool fold[2];
+    bool flat;
+};
+
+__device__ __forceinline__ void pt_init(Pt& e, vec2 pos, float hw, int join, bool flat) {
+    const float nan = __builtin_nanf("");
+    e.pos = pos;
+    e.hw = hw;
+    e.adv = nan;
+    e.join = join;
+    e.flat = flat;
+    for (int k = 0; k < 2; ++k) {
+        e.sp[k].prev = e.sp[k].next = mk(nan, nan);
+        e.sp[k].single = mk(0.0f, 0.0f);
+        e.sp[k].prev_v = e.sp[k].next_v = INVALID_ID;
+        e.sp[k].has_single = false;
+        e.fold[k] = false;
+    }
+}
+
+struct Stroker {
+    float lw, tol, miter_limit, merge2;
+    uint32_t start_cap, end_cap, line_join;
+    Sink* s;
+    vec2 on_path;  // StrokeVertexData: position_on_path, half_width
+    float vhw;
+    Pt buf[3];     // PointBuffer (stroke.rs:2371-2457)
+    uint32_t start, count;
+    Pt firsts[2];
+    uint32_t n_firsts;
+    bool may_need_empty_cap;
+    float sub_path_start_adv;
+    uint32_t status;
+};
+
+__device__ __forceinline__ Pt& at(Stroker& S, uint32_t i) { return S.buf[(S.start + i) % 3]; }
+__device__ __forceinline__ Pt& last(Stroker& S) { return at(S, S.count - 1); }
+__device__ __forceinline__ void push(Stroker& S, const Pt& p) {
+    if (S.count < 3) {
+        S.buf[S.count++] = p;
+        return;
+    }
+    S.buf[S.start] = p;
+    S.start = S.start == 2 ? 0 : S.start + 1;
+}
+__device__ __forceinline__ uint32_t vtx(Stroker& S, vec2 normal) {
+    return sink_sv(*S.s, S.on_path, normal, S.vhw);
+}
+
+// stroke.rs:1869-1904
+__device__ __forceinline__ void edge_tris(Stroker& S, const Pt& p0, const Pt& p1) {
+    uint32_t p0n = p0.sp[SN].next_v, p0p = p0.sp[SP].next_v;
+    uint32_t p1n = p1.sp[SN].prev_v, p1p = p1.sp[SP].prev_v;
+    if (p0.fold[SP]) p0n = p0.sp[SP].prev_v;
+    if (p0.fold[SN]) p0p = p0.sp[SN].prev_v;
+    if (p1.fold[SP]) p1n = p1.sp[SP].next_v;
+    if (p1.fold[SN]) p1p = p1.sp[SN].next_v;
+    if (p0n == p1p) return;
+    if (p0n != p0p && p0p != p1p) sink_tri(*S.s, p0n, p0p, p1p);
+    if (p0n != p1n && p1p != p1n) sink_tri(*S.s, p0n, p1p, p1n);
+}
+
+// stroke.rs:1968-2021
+__device__ __noinline__ void round_join(Stroker& S, Pt& j, int side) {
+    const float PI = 3.14159265358979323846f;
+    vec2 sn = sub(j.sp[side].prev, j.pos), en = sub(j.sp[side].next, j.pos);
+    uint32_t sv = j.sp[side].prev_v, ev = j.sp[side].next_v;
+    const float sign = side == SN ? 1.0f : -1.0f;
+    float a0 = angle_of(sn);
+    float diff = angle_to(a0, angle_of(en));
+    if (diff * sign < 0.0f) diff = sign * (2.0f * PI - ffabs(diff));
+    float a1 = a0 + diff;
+    if (side == SN) {
+        float ta = a0;
+        a0 = a1;
+        a1 = ta;
+        uint32_t tv = sv;
+        sv = ev;
+        ev = tv;
+    }
+    tessellate_arc(a0, a1, sv, ev, arc_subdivisions(diff, j.hw, S.tol), S.on_path, S.vhw, *S.s);
+}
+
+// stroke.rs:1907-1965
+__device__ __forceinline__ void join_tris(Stroker& S, Pt& j) {
+    const bool need0 = !j.sp[SP].has_single && !j.fold[SN];
+    const bool need1 = !j.sp[SN].has_single && !j.fold[SP];
+    const Side &P = j.sp[SP], &N = j.sp[SN];
+    if (!j.fold[SP] && !j.fold[SN]) {
+        if (need0 && need1) {
+            sink_tri(*S.s, P.prev_v, P.next_v, N.next_v);
+            sink_tri(*S.s, P.prev_v, N.next_v, N.prev_v);
+        } else if (!need0 && need1) {
+            sink_tri(*S.s, N.prev_v, P.prev_v, N.next_v);
+        } else if (need0 && !need1) {
+            sink_tri(*S.s, N.prev_v, P.prev_v, P.next_v);
+        }
+    }
+    if (j.join == LYG_JOIN_ROUND) {
+        if (need0) round_join(S, j, SP);
+        if (need1) round_join(S, j, SN);
+    }
+}
+
+// stroke.rs:2024-2055
+__device__ __forceinline__ void base_vertices(Stroker& S, Pt& j, int side) {
+    Side& sp = j.sp[side];
+    if (sp.has_single) {
+        uint32_t id = vtx(S, divs(sub(sp.single, j.pos), j.hw));
+        sp.prev_v = sp.next_v = id;
+    } else {
+        uint32_t a = vtx(S, divs(sub(sp.prev, j.pos), j.hw));
+        uint32_t b = vtx(S, divs(sub(sp.next, j.pos), j.hw));
+        sp.prev_v = a;
+        sp.next_v = b;
+    }
+}
+
+// stroke.rs:1743-1806; true: the join folds back and is skipped
+__device__ __forceinline__ bool flattened_step(Stroker& S, Pt& prev, Pt& j, Pt& next) {
+    vec2 pe = sub(j.pos, prev.pos);
+    float pl = vlen(pe);
+    vec2 pt = divs(pe, pl);
+    vec2 ne = sub(next.pos, j.pos);
+    float nl = vlen(ne);
+    vec2 nt = divs(ne, nl);
+    vec2 n = compute_normal(pt, nt);
+    if (j.adv != j.adv) j.adv = prev.adv + pl;
+    if (next.adv != next.adv) next.adv = j.adv + nl;
+    vec2 p0 = add(j.pos, mul(n, S.vhw));
+    vec2 p1 = sub(j.pos, mul(n, S.vhw));
+    j.sp[SP].prev = j.sp[SP].next = j.sp[SP].single = p0;
+    j.sp[SP].has_single = true;
+    j.sp[SN].prev = j.sp[SN].next = j.sp[SN].single = p1;
+    j.sp[SN].has_single = true;
+    vec2 v0 = sub(p0, prev.sp[SP].next), v1 = sub(p1, prev.sp[SN].next);
+    if (dot(pe, v0) < 0.0f && dot(pe, v1) < 0.0f) return true;
+    uint32_t a = vtx(S, n);
+    uint32_t b = vtx(S, neg(n));
+    j.sp[SP].prev_v = j.sp[SP].next_v = a;
+    j.sp[SN].prev_v = j.sp[SN].next_v = b;
+    return false;
+}
+
+// stroke.rs:1809-1866: side lines of the edge p0 -> p1, tangent to both end circles
+__device__ __forceinline__ void attachments(Pt& p0, Pt& p1) {
+    const float PI = 3.14159265358979323846f;
+    vec2 e = sub(p1.pos, p0.pos);
+    float d = vlen(e);
+    float ea = angle_of(e);
+    float va = (float)asin((double)((p1.hw - p0.hw) / d));
+    if (va != va) va = 0.0f;
+    for (int side = 0; side < 2; ++side) {
+        float nl = side == SN ? -1.0f : 1.0f;
+        float na = ea + nl * (PI * 0.5f + va);
+        vec2 n = mk(t_cos(na), t_sin(na));
+        p0.sp[side].next = add(p0.pos, mul(n, p0.hw));
+        p1.sp[side].prev = add(p1.pos, mul(n, p1.hw));
+    }
+    if (p1.adv != p1.adv) p1.adv = p0.adv + d;
+}
+
+// stroke.rs:2060-2128
+__device__ __forceinline__ void join_side(const Pt& prev, Pt& j, const Pt& next, float ml,
+                                          int side) {
+    const float sign = side == SN ? -1.0f : 1.0f;
+    vec2 v0 = normalize(sub(j.sp[side].prev, prev.sp[side].next));
+    vec2 v1 = normalize(sub(next.sp[side].prev, j.sp[side].next));
+    bool inward = cross(v0, v1) * sign > 0.0f;
+    bool forward = dot(v0, v1) > 0.0f;
+    vec2 n = mul(compute_normal(v0, v1), sign);
+    vec2 pv0 = normalize(sub(j.pos, prev.pos));
+    vec2 pv1 = normalize(sub(next.pos, j.pos));
+    bool same_side = dot(add(v0, v1), add(pv0, pv1)) >= 0.0f;
+    if (inward && !forward && same_side) {
+        vec2 en = mul(n, j.hw);
+        float pl = j.adv - prev.adv;
+        float nl = next.adv - j.adv;
+        float dn = dot(en, v1) - nl;
+        float dp = dot(en, neg(v0)) - pl;
+        if (ffmin(dn, dp) > 0.0f || sqlen(n) < 1e-5f) j.fold[side] = true;
+    }
+    bool concave = inward && same_side && !j.fold[side];
+    bool miter = j.join == LYG_JOIN_MITER || j.join == LYG_JOIN_MITER_CLIP;
+    if (concave || (miter && !(sqlen(n) > ml * ml * 4.0f))) {
+        j.sp[side].has_single = true;
+        j.sp[side].single = add(j.pos, mul(n, j.hw));
+    } else if (j.join == LYG_JOIN_MITER_CLIP) {
+        vec2 n0 = sub(j.sp[side].prev, j.pos), n1 = sub(j.sp[side].next, j.pos);
+        vec2 i0, i1;
+        clip_intersections(n0, n1, n, ml * j.hw, &i0, &i1);
+        j.sp[side].prev = add(j.pos, i0);
+        j.sp[side].next = add(j.pos, i1);
+    }
+}
+
+// step_impl (stroke.rs:1380-1512); returns whether the point was added
+__device__ __noinline__ bool step(Stroker& S, Pt next) {
+    const uint32_t count = S.count;
+    if (count > 0) {
+        vec2 d = sub(last(S).pos, next.pos);
+        if (sqlen(d) < S.merge2) {
+            if (count == 1) S.may_need_empty_cap = true;
+            return false;
+        }
+        Pt& j = last(S);
+        if (!j.flat || !next.flat) attachments(j, next);
+    }
+    bool skip = false;
+    if (count > 1) {
+        Pt& prev = at(S, count - 2);
+        Pt& j = at(S, count - 1);
+        S.on_path = j.pos;
+        S.vhw = j.hw;
+        bool fast = false;
+        if (j.flat) fast = dot(sub(j.pos, prev.pos), sub(next.pos, j.pos)) > 0.0f;
+        if (fast) {
+            j.join = LYG_JOIN_MITER;
+            skip = flattened_step(S, prev, j, next);
+        } else {
+            join_side(prev, j, next, S.miter_limit, SP);
+            join_side(prev, j, next, S.miter_limit, SN);
+            if (j.sp[SP].has_single) j.fold[SN] = false;
+            if (j.sp[SN].has_single) j.fold[SP] = false;
+            base_vertices(S, j, SN);
+            base_vertices(S, j, SP);
+        }
+        if (!skip) {
+            if (count > 2) edge_tris(S, prev, j);
+            join_tris(S, j);
+            if (count == 2) {
+                S.firsts[0] = prev;
+                S.firsts[1] = j;
+                S.n_firsts = 2;
+            }
+        }
+    }
+    if (skip) S.buf[(S.start == 0 ? S.count : S.start) - 1] = next;  // replace_last
+    else push(S, next);
+    return true;
+}
+
+// stroke.rs:2459-2524
+__device__ __noinline__ void round_cap(Stroker& S, vec2 center, float radius, vec2 sn,
+                                       uint32_t sv, uint32_t ev, vec2 edge_n, bool is_start) {
+    if (radius < S.tol) return;
+    float a0 = angle_of(sn);
+    float diff = angle_to(a0, angle_of(edge_n));
+    float mid = a0 + diff;
+    float a1 = mid + diff;
+    uint32_t nsub = arc_subdivisions(diff, radius, S.tol);
+    S.on_path = center;
+    S.vhw = radius;
+    uint32_t mv = vtx(S, normalize(edge_n));
+    sink_tri(*S.s, sv, mv, ev);
+    tessellate_arc(a0, mid, sv, mv, nsub, center, radius, *S.s);
+    tessellate_arc(mid, a1, mv, ev, nsub, center, radius, *S.s);
+}
+
+// stroke.rs:1282-1316, 2526-2609
+__device__ __noinline__ void empty_cap(Stroker& S) {
+    Pt& p = at(S, 0);
+    S.vhw = p.hw;
+    S.on_path = p.pos;
+    if (S.start_cap == LYG_CAP_SQUARE) {
+        uint32_t a = vtx(S, mk(1.0f, 1.0f));
+        uint32_t b = vtx(S, mk(1.0f, -1.0f));
+        uint32_t c = vtx(S, mk(-1.0f, -1.0f));
+        uint32_t d = vtx(S, mk(-1.0f, 1.0f));
+        sink_tri(*S.s, a, b, c);
+        sink_tri(*S.s, a, c, d);
+    } else if (S.start_cap == LYG_CAP_ROUND) {
+        const vec2 center = p.pos;
+        const float r = S.vhw;
+        uint32_t l = vtx(S, mk(-1.0f, 0.0f));
+        uint32_t rr = vtx(S, mk(1.0f, 0.0f));
+        round_cap(S, center, r, mk(-1.0f, 0.0f), l, rr, mk(0.0f, 1.0f), true);
+        round_cap(S, center, r, mk(1.0f, 0.0f), rr, l, mk(0.0f, -1.0f), false);
+    }
+}
+
+// stroke.rs:2130-2206
+__device__ __noinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool is_first) {
+    vec2 v = sub(p1.pos, p0.pos);
+    p1.adv = p0.adv + vlen(v);
+    S.on_path = p1.pos;
+    S.vhw = p1.hw;
+    for (int side = 0; side < 2; ++side) {
+        vec2 sp = p1.sp[side].prev;
+        if (S.end_cap != LYG_CAP_ROUND) {
+            float clip = S.end_cap == LYG_CAP_SQUARE ? p1.hw : 0.0f;
+            vec2 n = normalize(sub(p1.pos, p0.pos));
+            vec2 inter;
+            if (!line_isect(add(p1.pos, mul(n, clip)), perp(n), sp, sub(sp, p0.sp[side].next),
+                            &inter))
+                inter = p1.sp[side].prev;
+            p1.sp[side].prev = inter;
+        }
+        p1.sp[side].prev_v = vtx(S, divs(sub(p1.sp[side].prev, p1.pos), p1.hw));
+    }
+    if (!is_first) edge_tris(S, p0, p1);
+    if (S.end_cap == LYG_CAP_ROUND)
+        round_cap(S, p1.pos, p1.hw, sub(p1.sp[SP].prev, p1.pos), p1.sp[SP].prev_v,
+                  p1.sp[SN].prev_v, v, false);
+}
+
+// stroke.rs:2208-2276
+__device__ __noinline__ void first_edge(Stroker& S, Pt& f, const Pt& second) {
+    S.on_path = f.pos;
+    S.vhw = f.hw;
+    for (int side = 0; side < 2; ++side) {
+        vec2 sp = f.sp[side].next;
+        if (S.start_cap != LYG_CAP_ROUND) {
+            float clip = S.start_cap == LYG_CAP_SQUARE ? f.hw : 0.0f;
+            vec2 n = normalize(sub(f.pos, second.pos));
+            vec2 inter;
+            if (!line_isect(add(f.pos, mul(n, clip)), perp(n), sp,
+                            sub(sp, second.sp[side].prev), &inter))
+                inter = f.sp[side].next;
+            sp = inter;
+        }
+        f.sp[side].next_v = vtx(S, divs(sub(sp, f.pos), f.hw));
+    }
+    edge_tris(S, f, second);
+    if (S.start_cap == LYG_CAP_ROUND)
+        round_cap(S, f.pos, f.hw, sub(f.sp[SN].next, f.pos), f.sp[SN].next_v, f.sp[SP].next_v,
+                  sub(f.pos, second.pos), true);
+}
+
+// stroke.rs:1322-1378
+__device__ __noinline__ void end_with_caps(Stroker& S) {
+    if (S.may_need_empty_cap && S.count == 1) {
+        empty_cap(S);
+        return;
+    }
+    if (S.count >= 2) {
+        Pt p0 = at(S, S.count - 2);
+        Pt p1 = at(S, S.count - 1);
+        last_edge(S, p0, p1, S.count == 2);
+        S.sub_path_start_adv = p1.adv;
+        if (S.count > 2) {
+            p0 = S.firsts[0];
+            p1 = S.firsts[1];
+        }
+        first_edge(S, p0, p1);
+    }
+}
+
+// stroke.rs:1206-1279
+__device__ __noinline__ void close_sub(Stroker& S) {
+    if (S.count == 1) empty_cap(S);
+    if (S.count <= 2) return;
+    Pt p = S.firsts[0];
+    p.adv = __builtin_nanf("");
+    if (!step(S, p)) last(S).pos = p.pos;
+    if (S.n_firsts >= 2) {
+        step(S, S.firsts[1]);
+        Pt& p0 = at(S, S.count - 2);
+        Pt& p1 = at(S, S.count - 1);
+        S.on_path = p0.pos;
+        S.vhw = p0.hw;
+        for (int side = 0; side < 2; ++side) {
+            vec2 sp = p0.sp[side].has_single ? p0.sp[side].single : p0.sp[side].next;
+            p0.sp[side].next_v = vtx(S, divs(sub(sp, p0.pos), p0.hw));
+        }
+        edge_tris(S, p0, p1);
+    }
+}
+
+// stroke.rs:1179-1193
+__device__ __forceinline__ void end_sub(Stroker& S, bool close) {
+    S.may_need_empty_cap = S.may_need_empty_cap || (close && S.count == 1);
+    if (close && S.count > 2) close_sub(S);
+    else end_with_caps(S);
+    S.count = S.start = 0;
+    S.n_firsts = 0;
+}
+
+// One flattened curve piece (half `b` of a split quadratic, or the whole curve): its
+// points with the half width interpolated at their t on the whole curve
+// (t_whole = t0 + t * t_scale), as quadratic_bezier_to / cubic_bezier_to do.
+__device__ __forceinline__ void curve_piece(Stroker& S, const VRec& r, bool b, bool cubic,
+                                            float t0, float t_scale, float w0, float w1) {
+    const uint32_t n = b ? r.nB : r.nA;
+    const float stp = b ? r.sB : r.sA;
+    if (n > MAX_CURVE_POINTS) {
+        S.status = LYG_ERR_NO_SPACE;
+        return;
+    }
+    float t = 0.0f;
+    for (uint32_t k = 0; k < n; ++k) {
+        vec2 pos;
+        float t1;
+        if (k + 1 == n) {
+            pos = b ? r.toB : r.toA;
+            t1 = 1.0f;
+        } else {
+            t = t + stp;
+            t1 = t;
+            pos = horner(r, b, cubic && !b, t);
+        }
+        const float tw = t_scale == 1.0f && t0 == 0.0f ? t1 : t0 + t1 * t_scale;
+        Pt e;
+        pt_init(e, pos, (w0 * (1.0f - tw) + w1 * tw) * 0.5f, (int)S.line_join, t1 != 1.0f);
+        step(S, e);
+    }
+}
+
+}  // namespace vw
+
+
+// One lane per path (grid-stride), counting (EMIT false) or writing at the scanned offsets.
+template <bool EMIT>
+__global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
+    using namespace vw;
+    for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < L.in.n_paths;
+         path += gridDim.x * blockDim.x) {
+        const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
+        const uint32_t vb = L.in.verb_off[path], ve = L.in.verb_off[path + 1];
+        const uint32_t pb = L.in.point_off[path], pe = L.in.point_off[path + 1];
+        const float* pts = L.in.points + 2ull * pb;
+        const uint32_t npts = pe - pb;
+        Sink sk;
+        sk.nv = sk.ni = 0;
+        sk.emit = EMIT;
+        sk.err = L.err;
+        if (EMIT) {
+            sk.vid = sk.vid0 = L.vertex_offset + L.vtx_off[path];
+            sk.vout = L.out_vertices + 2ull * L.vtx_off[path];
+            sk.iout = L.out_indices + L.idx_off[path];
+            sk.vlim = L.vtx_off[path + 1] - L.vtx_off[path];
+            sk.ileft = L.idx_off[path + 1] - L.idx_off[path];
+            if (L.path_status[path] != 0) continue;  // the count pass failed it: no output
+        } else {
+            sk.vid = sk.vid0 = 0;
+            sk.vout = nullptr;
+            sk.iout = nullptr;
+            sk.vlim = sk.ileft = 0;
+        }
+        Stroker S;
+        S.lw = o.line_width;
+        S.tol = o.tolerance;
+        S.miter_limit = o.miter_limit;
+        S.merge2 = ffmax(ffmin(o.tolerance * o.tolerance * 0.5f,
+                               o.line_width * o.line_width * 0.05f),
+                         1e-8f);  // StrokeBuilderImpl::new (stroke.rs:591-631)
+        S.start_cap = o.start_cap;
+        S.end_cap = o.end_cap;
+        S.line_join = o.line_join;
+        S.s = &sk;
+        S.on_path = mk(0.0f, 0.0f);
+        S.vhw = o.line_width * 0.5f;
+        S.start = S.count = 0;
+        S.n_firsts = 0;
+        S.may_need_empty_cap = false;
+        S.sub_path_start_adv = 0.0f;
+        S.status = 0;
+        auto width = [&](uint32_t i) {
+            return i < npts ? L.widths[(size_t)(pb + i) * L.widths_stride]
+                            : __builtin_nanf("");
+        };
+        float cur_w = __builtin_nanf("");
+        uint32_t p = 0;
+        for (uint32_t v = vb; v < ve && S.status == 0; ++v) {
+            const uint8_t verb = L.in.verbs[v];
+            VRec r;
+            decode_verb(verb, pts, npts, p, o.tolerance, r);
+            if (verb == LYG_VERB_BEGIN) {
+                cur_w = width(p);
+                S.may_need_empty_cap = false;
+                Pt e;
+                pt_init(e, r.toA, o.line_width * cur_w * 0.5f, (int)o.line_join, false);
+                e.adv = S.sub_path_start_adv;
+                step(S, e);
+                p += 1;
+            } else if (verb == LYG_VERB_LINE) {
+                cur_w = width(p);
+                Pt e;
+                pt_init(e, r.toA, o.line_width * cur_w * 0.5f, (int)o.line_join, false);
+                step(S, e);
+                p += 1;
+            } else if (verb == LYG_VERB_QUADRATIC) {
+                const float w0 = o.line_width * cur_w;
+                cur_w = width(p + 1);
+                const float w1 = o.line_width * cur_w;
+                vec2 from = p > 0 ? ldp(pts, npts, p - 1) : mk(__builtin_nanf(""), __builtin_nanf(""));
+                const float ts = find_sharp_turn(from, ldp(pts, npts, p), ldp(pts, npts, p + 1));
+                if (ts > 0.0f) {  // flatten_quad's two halves (stroke.rs:2755-2779)
+                    curve_piece(S, r, false, false, 0.0f, ts, w0, w1);
+                    if (S.status == 0) curve_piece(S, r, true, false, ts, 1.0f - ts, w0, w1);
+                } else {
+                    curve_piece(S, r, false, false, 0.0f, 1.0f, w0, w1);
+                }
+                p += 2;
+            } else if (verb == LYG_VERB_CUBIC) {
+                const float w0 = o.line_width * cur_w;
+                cur_w = width(p + 2);
+                curve_piece(S, r, false, true, 0.0f, 1.0f, w0, o.line_width * cur_w);
+                p += 3;
+            } else {
+                const bool close = verb == LYG_VERB_CLOSE;
+                end_sub(S, close);
+                if (close) p += 1;
+            }
+        }
+        if (!EMIT) {
+            const bool ok = S.status == 0;
+            L.path_status[path] = S.status;
+            L.path_nv[path] = ok ? sk.nv : 0u;
+            L.path_ni[path] = ok ? 3u * sk.ni : 0u;
+        }
+    }
+}
+
 }  // namespace lyg
 
 extern "C" {
@@ -1466,6 +2003,16 @@ hipError_t lyg_launch_stroke(const lyg::StrokeLaunch* L, int emit, int grid, hip
     } else {
         hipLaunchKernelGGL((lyg::stroke_kernel<false, false>), dim3(grid), block, 0, stream, *L);
     }
+    return hipGetLastError();
+}
+
+hipError_t lyg_launch_stroke_vw(const lyg::StrokeVWLaunch* L, int emit, int grid,
+                                hipStream_t s) {
+    if (L->in.n_paths == 0) return hipSuccess;
+    if (emit)
+        hipLaunchKernelGGL(lyg::stroke_vw_kernel<true>, dim3(grid), dim3(64), 0, s, *L);
+    else
+        hipLaunchKernelGGL(lyg::stroke_vw_kernel<false>, dim3(grid), dim3(64), 0, s, *L);
     return hipGetLastError();
 }
 

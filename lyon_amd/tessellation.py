@@ -290,7 +290,20 @@ class StrokeTessellator:
 
     def tessellate_path(self, path: Path, options: StrokeOptions, builder: BuffersBuilder):
         if options.variable_line_width is not None:
-            raise TessellationError(19, "variable_line_width")
+            # tessellate_with_ids_vw (stroke.rs:653-767): the width factor is the path's
+            # custom attribute `variable_line_width` of each endpoint
+            if path.attributes is None:
+                raise ValueError("variable_line_width needs a path with custom attributes")
+            attrs = np.ascontiguousarray(path.attributes, np.float32)
+            idx = int(options.variable_line_width)
+            if not 0 <= idx < attrs.shape[1]:
+                raise ValueError("variable_line_width is not an attribute of the path")
+
+            def call(ctx, pv, copts, out):
+                return B.lib().lyg_stroke_path_vw(ctx, pv, attrs.ctypes.data_as(B.F32P),
+                                                  attrs.shape[1], idx, copts, out)
+            _run(call, path, options.to_c(), builder)
+            return
         _run(B.lib().lyg_stroke_path, path, options.to_c(), builder)
 
     tessellate = tessellate_path

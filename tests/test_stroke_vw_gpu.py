@@ -1,0 +1,182 @@
+"""Variable-width stroke (StrokeOptions::variable_line_width) on the GPU against the oracle
+(stroke.rs:653-767, 1380-1512, 1809-1866, 2060-2128), through lyg_stroke_path_vw and
+lyg_stroke_batch_vw.
+
+Tolerance: the same triangles (index triples) and positions within 1e-4 of the path's
+scale (max |coordinate|, at least 1) -- the edge normals go through asin, cos and sin,
+evaluated in f64 and rounded once on the GPU and by the C library on the CPU, which may
+differ in the last bit.  NaN positions (lyon's own, at sharp turns inside cubics) must be
+NaN on both sides.  On random paths a near-tie decided differently by one bit may change a
+join's layout: at least 97% of the paths must match exactly, and every path must have the
+same status."""
+import numpy as np
+import pytest
+
+from lyon_amd import (BuffersBuilder, LineCap, LineJoin, StrokeOptions, StrokeTessellator,
+                      VertexBuffers)
+from lyon_amd import workloads
+from lyon_amd.batch import DeviceBatch, StrokeVWBatcher, synchronize
+from lyon_amd.path import PathBuilder
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_vw(path, opts):
+    buf = VertexBuffers()
+    StrokeTessellator().tessellate_path(path, opts, BuffersBuilder(buf))
+    return np.asarray(buf.vertices, np.float32).reshape(-1, 2), np.asarray(buf.indices)
+
+
+def ref_vw(path, opts):
+    st, v, i = O.stroke(path, line_width=opts.line_width, miter_limit=opts.miter_limit,
+                        tolerance=opts.tolerance, start_cap=int(opts.start_cap),
+                        end_cap=int(opts.end_cap), line_join=int(opts.line_join),
+                        widths=path.attributes[:, opts.variable_line_width])
+    assert st == 0
+    return v, i
+
+
+def close_enough(v, rv, scale):
+    if v.shape != rv.shape:
+        return False
+    nan_a, nan_b = np.isnan(v), np.isnan(rv)
+    if not np.array_equal(nan_a, nan_b):
+        return False
+    ok = ~nan_a
+    return bool(np.all(np.abs(v[ok] - rv[ok]) <= 1e-4 * scale))
+
+
+def square_path():
+    b = PathBuilder(1)
+    for pts in (((-1, 1), (1, 1), (1, -1), (-1, -1)), ((-1, -1), (1, -1), (1, 1), (-1, 1))):
+        b.begin(pts[0], [0.3])
+        for q in pts[1:]:
+            b.line_to(q, [0.3])
+        b.end(False)
+    return b.build()
+
+
+def issue_894_path():
+    b = PathBuilder(1)
+    b.begin((435.72, 368.42), [38.82])
+    b.line_to((433.53, 366.06), [38.82])
+    b.quadratic_bezier_to((431.35, 363.70), (430.22, 362.52), [39.59])
+    b.quadratic_bezier_to((429.09, 361.34), (429.05, 362.14), [41.62])
+    b.line_to((429.00, 362.95), [41.63])
+    b.end(False)
+    return b.build()
+
+
+def growing_spiral():
+    """Curves and lines whose width factor grows along the path (0.2 to 3), closed."""
+    b = PathBuilder(2)
+    b.begin((0, 0), [9.0, 0.2])
+    b.quadratic_bezier_to((50, -40), (100, 0), [9.0, 0.8])
+    b.cubic_bezier_to((140, 40), (60, 120), (20, 80), [9.0, 1.6])
+    b.line_to((-30, 60), [9.0, 2.2])
+    b.line_to((-10, 10), [9.0, 3.0])
+    b.end(True)
+    return b.build()
+
+
+@pytest.mark.parametrize("case", ["square", "issue_894", "spiral"])
+@pytest.mark.parametrize("join", [LineJoin.Miter, LineJoin.MiterClip, LineJoin.Round,
+                                  LineJoin.Bevel])
+@pytest.mark.parametrize("cap", [LineCap.Butt, LineCap.Square, LineCap.Round])
+def test_plain_calls_match_the_oracle(case, join, cap):
+    path = {"square": square_path, "issue_894": issue_894_path, "spiral": growing_spiral}[case]()
+    attr = 1 if case == "spiral" else 0
+    opts = StrokeOptions(line_width=2.0 if case == "spiral" else 1.0, line_join=join,
+                         start_cap=cap, end_cap=cap, miter_limit=2.0, tolerance=0.01,
+                         variable_line_width=attr)
+    v, i = gpu_vw(path, opts)
+    rv, ri = ref_vw(path, opts)
+    if case == "square" and join == LineJoin.Miter and cap == LineCap.Butt:
+        assert len(i) // 3 == 12  # test_square (stroke.rs:2851-2908)
+    if case == "square" and join == LineJoin.Bevel and cap == LineCap.Square:
+        assert len(i) // 3 == 16
+    assert np.array_equal(i, ri)
+    scale = max(1.0, float(np.max(np.abs(path.points))))
+    assert close_enough(v, rv, scale)
+
+
+def test_empty_caps():
+    """test_empty_caps (stroke.rs:2918-2976): none with butt caps, 8 triangles with square
+    caps, the oracle's bytes with round caps."""
+    b = PathBuilder(1)
+    b.begin((1, 0), [1.0]); b.end(True)
+    b.begin((2, 0), [1.0]); b.line_to((2, 0), [1.0]); b.end(False)
+    b.begin((3, 0), [1.0]); b.line_to((3, 0), [1.0]); b.end(True)
+    b.begin((3, 0), [1.0])
+    for _ in range(3):
+        b.line_to((3, 0), [1.0])
+    b.end(True)
+    b.begin((4, 0), [1.0]); b.end(False)
+    p = b.build()
+    for cap, want in ((LineCap.Butt, 0), (LineCap.Square, 8), (LineCap.Round, None)):
+        opts = StrokeOptions(start_cap=cap, end_cap=cap, variable_line_width=0)
+        v, i = gpu_vw(p, opts)
+        rv, ri = ref_vw(p, opts)
+        if want is not None:
+            assert len(i) // 3 == want
+        assert np.array_equal(i, ri) and close_enough(v, rv, 4.0)
+
+
+def random_batch(n, seed):
+    batch = workloads.stroke_batch(n, 4, 48, 1024.0, seed=seed)
+    opts = workloads.stroke_options(n, tolerance=0.05, seed=seed)
+    rng = np.random.default_rng(seed)
+    widths = rng.uniform(0.25, 2.5, len(batch.points)).astype(np.float32)
+    return batch, opts, widths
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_batch_matches_the_oracle(seed):
+    n = 400
+    batch, opts, widths = random_batch(n, seed)
+    db = DeviceBatch(batch, opts)
+    res = StrokeVWBatcher(widths).run(db, vertex_offset=7, check=False)
+    synchronize()
+    st = res.status
+    vo, io = res.vertex_offsets, res.index_offsets
+    v, i = res.all_vertices(), res.all_indices()
+    exact = 0
+    for p in range(n):
+        path = batch.path(p)
+        o = opts[p]
+        pb, pe = int(batch.point_offsets[p]), int(batch.point_offsets[p + 1])
+        rs, rv, ri = O.stroke(path, line_width=float(o["line_width"]),
+                              miter_limit=float(o["miter_limit"]),
+                              tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
+                              end_cap=int(o["end_cap"]), line_join=int(o["line_join"]),
+                              vertex_offset=7 + int(vo[p]), widths=widths[pb:pe])
+        assert int(st[p]) == rs
+        pv = v[vo[p]:vo[p + 1]]
+        pi = i[io[p]:io[p + 1]]
+        scale = max(1.0, float(np.max(np.abs(path.points))))
+        if np.array_equal(pi, ri) and close_enough(pv, rv, scale):
+            exact += 1
+    print(f"{exact} of {n} paths match exactly", flush=True)
+    assert exact >= 0.97 * n
+
+
+def test_batch_equals_plain_calls():
+    n = 24
+    batch, opts, widths = random_batch(n, 3)
+    db = DeviceBatch(batch, opts)
+    res = StrokeVWBatcher(widths).run(db, check=False)
+    vo, io = res.vertex_offsets, res.index_offsets
+    v, i = res.all_vertices(), res.all_indices()
+    for p in range(n):
+        o = opts[p]
+        pb, pe = int(batch.point_offsets[p]), int(batch.point_offsets[p + 1])
+        path = batch.path(p)
+        path.attributes = widths[pb:pe].reshape(-1, 1)
+        so = StrokeOptions(line_width=float(o["line_width"]), miter_limit=float(o["miter_limit"]),
+                           tolerance=float(o["tolerance"]), start_cap=int(o["start_cap"]),
+                           end_cap=int(o["end_cap"]), line_join=int(o["line_join"]),
+                           variable_line_width=0)
+        gv, gi = gpu_vw(path, so)
+        assert np.array_equal(gv.view(np.uint32), v[vo[p]:vo[p + 1]].view(np.uint32))
+        assert np.array_equal(gi.astype(np.int64) + int(vo[p]), i[io[p]:io[p + 1]])
