@@ -31,6 +31,8 @@ The same JSON line carries:
     Poisson request streams, with p50/p99/max request latency: short strokes, a 4-256
     segment mix, and the serving configuration (100k req/s offered, log-uniform 4-65536
     segments, 60% strokes / 40% random-walk fills, 32 GiB pool);
+  * stroke_variable_width: the stroke workload's paths through the variable-width stroker
+    (a width factor per point), and stroke_graph_replay / fill.graph: HIP-graph replays;
   * cpu_baseline: the oracle (C++ restatement of lyon's tessellators) on the box's host
     cores over a bounded sample of the same workloads, single thread and all cores.
 """
@@ -141,6 +143,8 @@ def main():
                          "reference kernel for the traffic counters), for rocprofv3")
     ap.add_argument("--profile-large", action="store_true",
                     help="only run the single large fill (banded) steps, for rocprofv3")
+    ap.add_argument("--profile-vw", action="store_true",
+                    help="only run the variable-width stroke steps, for rocprofv3")
     ap.add_argument("--profile-fill", action="store_true",
                     help="only run the batched fill steps and a 1 GiB device copy, for rocprofv3")
     args = ap.parse_args()
@@ -160,6 +164,19 @@ def main():
     L = B.lib()
     ctx = _Ctx.get()
     L.lyg_ctx_enable_timing(ctx, 1)
+
+    if args.profile_vw:
+        from lyon_amd.batch import StrokeVWBatcher
+        batch = workloads.stroke_batch(args.paths, 8, 256, 2048.0, seed=0)
+        opts = workloads.stroke_options(args.paths, tolerance=0.05, w_min=0.5, w_max=8.0)
+        db = DeviceBatch(batch, opts)
+        widths = np.random.default_rng(12345).uniform(0.5, 2.0, len(batch.points))
+        vwb = StrokeVWBatcher(widths.astype(np.float32))
+        for k in range(args.steps):
+            vwb.run(db, check=False)
+            synchronize()
+            log(f"variable-width step {k} done")
+        return
 
     if args.profile:
         batch = workloads.stroke_batch(args.paths, 8, 256, 2048.0, seed=0)
@@ -275,6 +292,35 @@ def main():
         del g
     except Exception as e:  # reported, not fatal: the headline does not depend on it
         graph = {"error": str(e)}
+
+    # the same paths and options through the variable-width stroker, every endpoint with
+    # its own width factor (seeded U[0.5, 2] per point)
+    vw_run = None
+    try:
+        from lyon_amd.batch import StrokeVWBatcher
+        wrng = np.random.default_rng(12345)
+        widths = wrng.uniform(0.5, 2.0, len(batch.points)).astype(np.float32)
+        vwb = StrokeVWBatcher(widths)
+        vres = vwb.run(db, check=False)
+        synchronize()
+        vts = []
+        for _ in range(max(3, min(args.steps, 10))):
+            t0 = time.perf_counter()
+            vwb.run(db, check=False)
+            synchronize()
+            vts.append((time.perf_counter() - t0) * 1e3)
+        vms = float(np.median(vts))
+        vtris = vres.n_indices // 3
+        vw_run = {"ms_per_step": vms, "triangles_per_step": vtris,
+                  "value": vtris / (vms / 1e3), "unit": "triangles/s",
+                  "failed_paths": int(np.count_nonzero(vres.status)),
+                  "note": "lyg_stroke_batch_vw over the stroke workload's paths and options, "
+                          "width factor per point U[0.5, 2]: one GPU lane per path (count "
+                          "pass, scans, emit pass); median of steps"}
+        log(f"  variable-width stroke: {vms:.2f} ms, {vtris} triangles")
+        del vwb
+    except Exception as e:  # reported, not fatal: the headline does not depend on it
+        vw_run = {"error": str(e)}
 
     # measured copy rate of this device (1 GiB device-to-device copy kernel, read + write)
     from lyon_amd.batch import DeviceBuffer as _DB
@@ -732,6 +778,7 @@ def main():
         "kernel_ms": {"sizing_avg": float(np.mean(count_ms)), "emit_avg": emit_avg},
         "stroke_split": split_runs,
         "stroke_graph_replay": graph,
+        "stroke_variable_width": vw_run,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs,

@@ -190,6 +190,7 @@ struct StrokeVWLaunch {
     uint32_t* out_indices;
     uint32_t vertex_offset;
     uint32_t* err;
+    uint32_t paths_per_wave;  // lanes of a wave that take paths (1-64); the rest idle
 };
 
 struct FlattenLaunch {

@@ -1443,8 +1443,8 @@ struct Stroker {
     Sink* s;
     vec2 on_path;  // StrokeVertexData: position_on_path, half_width
     float vhw;
-    Pt buf[3];     // PointBuffer (stroke.rs:2371-2457)
-    uint32_t start, count;
+    Pt buf[3];     // PointBuffer (stroke.rs:2371-2457), oldest first: buf[count - 1] is last
+    uint32_t count;
     Pt firsts[2];
     uint32_t n_firsts;
     bool may_need_empty_cap;
@@ -1452,15 +1452,18 @@ struct Stroker {
     uint32_t status;
 };
 
-__device__ __forceinline__ Pt& at(Stroker& S, uint32_t i) { return S.buf[(S.start + i) % 3]; }
-__device__ __forceinline__ Pt& last(Stroker& S) { return at(S, S.count - 1); }
-__device__ __forceinline__ void push(Stroker& S, const Pt& p) {
-    if (S.count < 3) {
-        S.buf[S.count++] = p;
-        return;
+// Slots are addressed with constant indices only (the step is instantiated per count), so
+// the points stay in registers.
+template <uint32_t C>
+__device__ __forceinline__ void push_n(Stroker& S, const Pt& p) {
+    if (C < 3) {
+        S.buf[C < 3 ? C : 2] = p;
+        S.count = C + 1;
+    } else {
+        S.buf[0] = S.buf[1];
+        S.buf[1] = S.buf[2];
+        S.buf[2] = p;
     }
-    S.buf[S.start] = p;
-    S.start = S.start == 2 ? 0 : S.start + 1;
 }
 __device__ __forceinline__ uint32_t vtx(Stroker& S, vec2 normal) {
     return sink_sv(*S.s, S.on_path, normal, S.vhw);
@@ -1480,7 +1483,7 @@ __device__ __forceinline__ void edge_tris(Stroker& S, const Pt& p0, const Pt& p1
 }
 
 // stroke.rs:1968-2021
-__device__ __noinline__ void round_join(Stroker& S, Pt& j, int side) {
+__device__ __forceinline__ void round_join(Stroker& S, Pt& j, int side) {
     const float PI = 3.14159265358979323846f;
     vec2 sn = sub(j.sp[side].prev, j.pos), en = sub(j.sp[side].next, j.pos);
     uint32_t sv = j.sp[side].prev_v, ev = j.sp[side].next_v;
@@ -1613,22 +1616,23 @@ __device__ __forceinline__ void join_side(const Pt& prev, Pt& j, const Pt& next,
     }
 }
 
-// step_impl (stroke.rs:1380-1512); returns whether the point was added
-__device__ __noinline__ bool step(Stroker& S, Pt next) {
-    const uint32_t count = S.count;
-    if (count > 0) {
-        vec2 d = sub(last(S).pos, next.pos);
+// step_impl (stroke.rs:1380-1512) with C points buffered; returns whether the point was
+// added
+template <uint32_t C>
+__device__ __forceinline__ bool step_n(Stroker& S, Pt next) {
+    if (C > 0) {
+        Pt& lp = S.buf[C > 0 ? C - 1 : 0];
+        vec2 d = sub(lp.pos, next.pos);
         if (sqlen(d) < S.merge2) {
-            if (count == 1) S.may_need_empty_cap = true;
+            if (C == 1) S.may_need_empty_cap = true;
             return false;
         }
-        Pt& j = last(S);
-        if (!j.flat || !next.flat) attachments(j, next);
+        if (!lp.flat || !next.flat) attachments(lp, next);
     }
     bool skip = false;
-    if (count > 1) {
-        Pt& prev = at(S, count - 2);
-        Pt& j = at(S, count - 1);
+    if (C > 1) {
+        Pt& prev = S.buf[C > 1 ? C - 2 : 0];
+        Pt& j = S.buf[C > 1 ? C - 1 : 0];
         S.on_path = j.pos;
         S.vhw = j.hw;
         bool fast = false;
@@ -1645,22 +1649,30 @@ __device__ __noinline__ bool step(Stroker& S, Pt next) {
             base_vertices(S, j, SP);
         }
         if (!skip) {
-            if (count > 2) edge_tris(S, prev, j);
+            if (C > 2) edge_tris(S, prev, j);
             join_tris(S, j);
-            if (count == 2) {
+            if (C == 2) {
                 S.firsts[0] = prev;
                 S.firsts[1] = j;
                 S.n_firsts = 2;
             }
         }
     }
-    if (skip) S.buf[(S.start == 0 ? S.count : S.start) - 1] = next;  // replace_last
-    else push(S, next);
+    if (skip) S.buf[C > 0 ? C - 1 : 0] = next;  // replace_last
+    else push_n<C>(S, next);
     return true;
+}
+__device__ __forceinline__ bool step(Stroker& S, const Pt& next) {
+    switch (S.count) {
+        case 0: return step_n<0>(S, next);
+        case 1: return step_n<1>(S, next);
+        case 2: return step_n<2>(S, next);
+        default: return step_n<3>(S, next);
+    }
 }
 
 // stroke.rs:2459-2524
-__device__ __noinline__ void round_cap(Stroker& S, vec2 center, float radius, vec2 sn,
+__device__ __forceinline__ void round_cap(Stroker& S, vec2 center, float radius, vec2 sn,
                                        uint32_t sv, uint32_t ev, vec2 edge_n, bool is_start) {
     if (radius < S.tol) return;
     float a0 = angle_of(sn);
@@ -1677,8 +1689,8 @@ __device__ __noinline__ void round_cap(Stroker& S, vec2 center, float radius, ve
 }
 
 // stroke.rs:1282-1316, 2526-2609
-__device__ __noinline__ void empty_cap(Stroker& S) {
-    Pt& p = at(S, 0);
+__device__ __forceinline__ void empty_cap(Stroker& S) {
+    Pt& p = S.buf[0];
     S.vhw = p.hw;
     S.on_path = p.pos;
     if (S.start_cap == LYG_CAP_SQUARE) {
@@ -1699,7 +1711,7 @@ __device__ __noinline__ void empty_cap(Stroker& S) {
 }
 
 // stroke.rs:2130-2206
-__device__ __noinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool is_first) {
+__device__ __forceinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool is_first) {
     vec2 v = sub(p1.pos, p0.pos);
     p1.adv = p0.adv + vlen(v);
     S.on_path = p1.pos;
@@ -1724,7 +1736,7 @@ __device__ __noinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool is
 }
 
 // stroke.rs:2208-2276
-__device__ __noinline__ void first_edge(Stroker& S, Pt& f, const Pt& second) {
+__device__ __forceinline__ void first_edge(Stroker& S, Pt& f, const Pt& second) {
     S.on_path = f.pos;
     S.vhw = f.hw;
     for (int side = 0; side < 2; ++side) {
@@ -1747,14 +1759,14 @@ __device__ __noinline__ void first_edge(Stroker& S, Pt& f, const Pt& second) {
 }
 
 // stroke.rs:1322-1378
-__device__ __noinline__ void end_with_caps(Stroker& S) {
+__device__ __forceinline__ void end_with_caps(Stroker& S) {
     if (S.may_need_empty_cap && S.count == 1) {
         empty_cap(S);
         return;
     }
     if (S.count >= 2) {
-        Pt p0 = at(S, S.count - 2);
-        Pt p1 = at(S, S.count - 1);
+        Pt p0 = S.count == 2 ? S.buf[0] : S.buf[1];
+        Pt p1 = S.count == 2 ? S.buf[1] : S.buf[2];
         last_edge(S, p0, p1, S.count == 2);
         S.sub_path_start_adv = p1.adv;
         if (S.count > 2) {
@@ -1766,16 +1778,17 @@ __device__ __noinline__ void end_with_caps(Stroker& S) {
 }
 
 // stroke.rs:1206-1279
-__device__ __noinline__ void close_sub(Stroker& S) {
+__device__ __forceinline__ void close_sub(Stroker& S) {
     if (S.count == 1) empty_cap(S);
     if (S.count <= 2) return;
     Pt p = S.firsts[0];
     p.adv = __builtin_nanf("");
-    if (!step(S, p)) last(S).pos = p.pos;
+    if (!step(S, p)) S.buf[2].pos = p.pos;
     if (S.n_firsts >= 2) {
         step(S, S.firsts[1]);
-        Pt& p0 = at(S, S.count - 2);
-        Pt& p1 = at(S, S.count - 1);
+        // here count is 3: both steps kept or replaced a point after the two firsts
+        Pt& p0 = S.buf[1];
+        Pt& p1 = S.buf[2];
         S.on_path = p0.pos;
         S.vhw = p0.hw;
         for (int side = 0; side < 2; ++side) {
@@ -1791,7 +1804,7 @@ __device__ __forceinline__ void end_sub(Stroker& S, bool close) {
     S.may_need_empty_cap = S.may_need_empty_cap || (close && S.count == 1);
     if (close && S.count > 2) close_sub(S);
     else end_with_caps(S);
-    S.count = S.start = 0;
+    S.count = 0;
     S.n_firsts = 0;
 }
 
@@ -1828,12 +1841,17 @@ __device__ __forceinline__ void curve_piece(Stroker& S, const VRec& r, bool b, b
 }  // namespace vw
 
 
-// One lane per path (grid-stride), counting (EMIT false) or writing at the scanned offsets.
+// One lane per path, the first L.paths_per_wave lanes of each one-wave block taking paths
+// (grid-stride): few paths per wave spread the serial walks over every SIMD and keep the
+// lanes of a wave from waiting on each other's branches.  Counting (EMIT false) or writing
+// at the scanned offsets.
 template <bool EMIT>
 __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
     using namespace vw;
-    for (uint32_t path = blockIdx.x * blockDim.x + threadIdx.x; path < L.in.n_paths;
-         path += gridDim.x * blockDim.x) {
+    const uint32_t ppw = L.paths_per_wave;
+    if (threadIdx.x >= ppw) return;
+    for (uint32_t path = blockIdx.x * ppw + threadIdx.x; path < L.in.n_paths;
+         path += gridDim.x * ppw) {
         const lyg_stroke_options o = L.opts[L.opts_stride ? path : 0];
         const uint32_t vb = L.in.verb_off[path], ve = L.in.verb_off[path + 1];
         const uint32_t pb = L.in.point_off[path], pe = L.in.point_off[path + 1];
@@ -1869,7 +1887,7 @@ __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
         S.s = &sk;
         S.on_path = mk(0.0f, 0.0f);
         S.vhw = o.line_width * 0.5f;
-        S.start = S.count = 0;
+        S.count = 0;
         S.n_firsts = 0;
         S.may_need_empty_cap = false;
         S.sub_path_start_adv = 0.0f;

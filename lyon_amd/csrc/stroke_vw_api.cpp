@@ -35,7 +35,12 @@ int32_t stroke_vw_run(lyg_ctx* c, lyg::StrokeVWLaunch& L, uint32_t* vtx_off, uin
     L.path_status = as<uint32_t>(c->f_status);
     L.vertex_offset = vertex_offset;
     L.err = c->d_counters + 5;
-    const int grid = (int)std::min<uint32_t>((n + 63) / 64, 65535u);
+    // about six resident waves per SIMD (four SIMDs per CU), each with as few paths as
+    // fill them
+    const uint32_t slots = (uint32_t)std::max(1, c->cu_count) * 4u * 6u;
+    L.paths_per_wave = std::min<uint32_t>(64u, std::max<uint32_t>(1u, (n + slots - 1) / slots));
+    const int grid =
+        (int)std::min<uint32_t>((n + L.paths_per_wave - 1) / L.paths_per_wave, 1u << 20);
     if (!check(c, lyg_launch_fill_u32(c->d_counters + 4, 2, 0, s), "reset") ||
         !check(c, lyg_launch_stroke_vw(&L, 0, grid, s), "stroke vw count") ||
         !check(c, ctx_scan(c, L.path_nv, n, vtx_off, c->d_totals + 0, c->d_counters + 4, s),
