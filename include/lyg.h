@@ -234,15 +234,20 @@ int32_t lyg_fill_batch(lyg_ctx* ctx, const lyg_batch_in* in, const lyg_fill_opti
  * its half width is line_width * attributes[width_attribute] * 0.5; curves interpolate the
  * widths of their endpoints.  Positions agree with lyon's within float rounding of asin /
  * cos / sin; status as lyg_stroke_path / lyg_stroke_batch, plus LYG_ERR_NO_SPACE for a path
- * with a curve of more than 4,194,304 flattened points.  One GPU lane strokes one path. */
+ * with a curve of more than 4,194,304 flattened points.  One GPU lane strokes one path.
+ * out_attributes (optional, n_attributes floats per output vertex, vertex_capacity rows):
+ * StrokeVertex::interpolated_attributes (stroke.rs:2728-2746) of every vertex — its
+ * source endpoint's row, or on a curve a*(1-t) + b*t of the curve's two endpoint rows. */
 int32_t lyg_stroke_path_vw(lyg_ctx* ctx, const lyg_path_view* path, const float* attributes,
                            uint32_t n_attributes, uint32_t width_attribute,
-                           const lyg_stroke_options* opts, lyg_output* out);
-/* attributes in device memory, per point of the batch (lyg_batch_in's points). */
+                           const lyg_stroke_options* opts, lyg_output* out,
+                           float* out_attributes);
+/* attributes and out_attributes in device memory, per point of the batch (lyg_batch_in's
+ * points) and per output vertex. */
 int32_t lyg_stroke_batch_vw(lyg_ctx* ctx, const lyg_batch_in* in, const float* attributes,
                             uint32_t n_attributes, uint32_t width_attribute,
                             const lyg_stroke_options* opts, uint32_t n_opts,
-                            lyg_batch_out* out);
+                            lyg_batch_out* out, float* out_attributes);
 
 /* ---- banded fill (no lyon counterpart for the method) ---------------------------------
  * The fill of FillTessellator::tessellate_path (fill.rs:628-641) computed for large paths

@@ -108,10 +108,10 @@ SIGNATURES = {
     "lyg_stroke_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(StrokeOptionsC),
                          C.POINTER(OutputC)], C.c_int32),
     "lyg_stroke_path_vw": ([C.c_void_p, C.POINTER(PathViewC), F32P, C.c_uint32, C.c_uint32,
-                            C.POINTER(StrokeOptionsC), C.POINTER(OutputC)], C.c_int32),
+                            C.POINTER(StrokeOptionsC), C.POINTER(OutputC), F32P], C.c_int32),
     "lyg_stroke_batch_vw": ([C.c_void_p, C.POINTER(BatchInC), C.c_void_p, C.c_uint32,
-                             C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(BatchOutC)],
-                            C.c_int32),
+                             C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(BatchOutC),
+                             C.c_void_p], C.c_int32),
     "lyg_fill_path": ([C.c_void_p, C.POINTER(PathViewC), C.POINTER(FillOptionsC),
                        C.POINTER(OutputC)], C.c_int32),
     "lyg_flatten_segments": ([C.c_void_p, I32P, F32P, C.c_uint32, C.c_float, F32P, C.c_uint32,

@@ -203,12 +203,23 @@ class StrokeVWBatcher(_Batcher):
 
     _entry = "lyg_stroke_batch_vw"
 
-    def __init__(self, attributes, width_attribute: int = 0):
+    def __init__(self, attributes, width_attribute: int = 0, vertex_attributes=False):
+        """vertex_attributes: also write every vertex's interpolated attributes into
+        ``self.vertex_attributes`` (a DeviceBuffer of n_vertices x k float32)."""
         super().__init__()
         a = np.ascontiguousarray(attributes, np.float32)
         self.n_attributes = 1 if a.ndim == 1 else int(a.shape[1])
         self.width_attribute = int(width_attribute)
         self.attributes = DeviceBuffer.from_numpy(a.reshape(-1))
+        self.want_vertex_attributes = bool(vertex_attributes)
+        self.vertex_attributes = None
+
+    def _ensure(self, n_paths, nv, ni):
+        super()._ensure(n_paths, nv, ni)
+        if self.want_vertex_attributes:
+            need = 4 * self.n_attributes * max(nv, 1, self.vertices.nbytes // 8)
+            if self.vertex_attributes is None or self.vertex_attributes.nbytes < need:
+                self.vertex_attributes = DeviceBuffer(need)
 
     def run(self, db: DeviceBatch, vertex_offset: int = 0, check=True) -> BatchResult:
         self._ensure(db.n, 0, 0)
@@ -219,7 +230,9 @@ class StrokeVWBatcher(_Batcher):
                               vertex_offset, 0, 0, self.status.ptr)
             st = B.lib().lyg_stroke_batch_vw(_Ctx.get(), C.byref(cin), self.attributes.ptr,
                                              self.n_attributes, self.width_attribute,
-                                             db.opts.ptr, db.n_opts, C.byref(out))
+                                             db.opts.ptr, db.n_opts, C.byref(out),
+                                             self.vertex_attributes.ptr
+                                             if self.vertex_attributes is not None else None)
             if st == LYG_ERR_CAPACITY:
                 self._ensure(db.n, out.n_vertices, out.n_indices)
                 continue

@@ -24,7 +24,7 @@ struct lyg_ctx {
     std::string last_error;
     // grow-only device scratch
     DevBuf in_verbs, in_points, in_offs, in_opts, counts_v, counts_i, offs_v, offs_i, out_v,
-        out_i, misc, arcs, flat_kinds, flat_ctrl, flat_runs, flat_poly, in_attrs;
+        out_i, misc, arcs, flat_kinds, flat_ctrl, flat_runs, flat_poly, in_attrs, out_attrs;
     uint32_t* d_counters = nullptr;  // [0..3] work-queue heads, [4] overflow flag
     uint64_t* d_totals = nullptr;    // [0] vertices, [1] indices
     void* h_pinned = nullptr;        // small pinned staging area for totals

@@ -551,7 +551,7 @@ void lyg_ctx_destroy(lyg_ctx* c) {
     DevBuf* bufs[] = {&c->in_verbs, &c->in_points, &c->in_offs, &c->in_opts, &c->counts_v,
                       &c->counts_i, &c->offs_v,    &c->offs_i,  &c->out_v,   &c->out_i,
                       &c->misc,     &c->arcs,      &c->flat_kinds, &c->flat_ctrl,
-                      &c->flat_runs, &c->flat_poly, &c->in_attrs};
+                      &c->flat_runs, &c->flat_poly, &c->in_attrs, &c->out_attrs};
     for (DevBuf* b : bufs) free_buf(*b);
     free_buf(c->f_bound);
     free_buf(c->f_status);

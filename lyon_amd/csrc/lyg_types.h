@@ -191,6 +191,11 @@ struct StrokeVWLaunch {
     uint32_t vertex_offset;
     uint32_t* err;
     uint32_t paths_per_wave;  // lanes of a wave that take paths (1-64); the rest idle
+    // optional: every vertex's interpolated custom attributes (n_attrs per vertex) from the
+    // attribute table (n_attrs per point of the batch)
+    const float* attrs;
+    uint32_t n_attrs;
+    float* out_attrs;
 };
 
 struct FlattenLaunch {

@@ -1419,10 +1419,17 @@ struct Pt {
     Side sp[2];
     bool fold[2];
     bool flat;
+    // VertexSource: endpoint (point index sf) or the point at st on the edge sf -> sto
+    uint32_t sf, sto;
+    float st;
+    bool sedge;
 };
 
 __device__ __forceinline__ void pt_init(Pt& e, vec2 pos, float hw, int join, bool flat) {
     const float nan = __builtin_nanf("");
+    e.sf = e.sto = 0;
+    e.st = 0.0f;
+    e.sedge = false;
     e.pos = pos;
     e.hw = hw;
     e.adv = nan;
@@ -1450,7 +1457,39 @@ struct Stroker {
     bool may_need_empty_cap;
     float sub_path_start_adv;
     uint32_t status;
+    // interpolated attributes (emit pass, when asked for): the current vertex source and
+    // the first vertex id not yet given its row
+    const float* attrs;
+    uint32_t n_attrs;
+    float* attr_out;  // row of the path's first vertex
+    uint32_t sf, sto, filled;
+    float st;
+    bool sedge;
 };
+
+// StrokeVertex::interpolated_attributes (stroke.rs:2728-2746) for the vertices emitted
+// since the last source change, then the new source (where lyon sets vertex.src).
+__device__ __forceinline__ void flush_attrs(Stroker& S) {
+    if (!S.attr_out) return;
+    const Sink& k = *S.s;
+    for (uint32_t v = S.filled; v < k.vid; ++v) {
+        const uint32_t row = v - k.vid0;
+        if (row >= k.vlim) break;
+        const float* a = S.attrs + (size_t)S.sf * S.n_attrs;
+        const float* b = S.attrs + (size_t)S.sto * S.n_attrs;
+        float* o = S.attr_out + (size_t)row * S.n_attrs;
+        for (uint32_t q = 0; q < S.n_attrs; ++q)
+            o[q] = S.sedge ? a[q] * (1.0f - S.st) + b[q] * S.st : a[q];
+    }
+    S.filled = k.vid;
+}
+__device__ __forceinline__ void set_src(Stroker& S, const Pt& p) {
+    flush_attrs(S);
+    S.sf = p.sf;
+    S.sto = p.sto;
+    S.st = p.st;
+    S.sedge = p.sedge;
+}
 
 // Slots are addressed with constant indices only (the step is instantiated per count), so
 // the points stay in registers.
@@ -1633,6 +1672,7 @@ __device__ __forceinline__ bool step_n(Stroker& S, Pt next) {
     if (C > 1) {
         Pt& prev = S.buf[C > 1 ? C - 2 : 0];
         Pt& j = S.buf[C > 1 ? C - 1 : 0];
+        set_src(S, j);
         S.on_path = j.pos;
         S.vhw = j.hw;
         bool fast = false;
@@ -1691,6 +1731,7 @@ __device__ __forceinline__ void round_cap(Stroker& S, vec2 center, float radius,
 // stroke.rs:1282-1316, 2526-2609
 __device__ __forceinline__ void empty_cap(Stroker& S) {
     Pt& p = S.buf[0];
+    set_src(S, p);
     S.vhw = p.hw;
     S.on_path = p.pos;
     if (S.start_cap == LYG_CAP_SQUARE) {
@@ -1714,6 +1755,7 @@ __device__ __forceinline__ void empty_cap(Stroker& S) {
 __device__ __forceinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool is_first) {
     vec2 v = sub(p1.pos, p0.pos);
     p1.adv = p0.adv + vlen(v);
+    set_src(S, p1);
     S.on_path = p1.pos;
     S.vhw = p1.hw;
     for (int side = 0; side < 2; ++side) {
@@ -1737,6 +1779,7 @@ __device__ __forceinline__ void last_edge(Stroker& S, const Pt& p0, Pt& p1, bool
 
 // stroke.rs:2208-2276
 __device__ __forceinline__ void first_edge(Stroker& S, Pt& f, const Pt& second) {
+    set_src(S, f);
     S.on_path = f.pos;
     S.vhw = f.hw;
     for (int side = 0; side < 2; ++side) {
@@ -1789,6 +1832,7 @@ __device__ __forceinline__ void close_sub(Stroker& S) {
         // here count is 3: both steps kept or replaced a point after the two firsts
         Pt& p0 = S.buf[1];
         Pt& p1 = S.buf[2];
+        set_src(S, p0);
         S.on_path = p0.pos;
         S.vhw = p0.hw;
         for (int side = 0; side < 2; ++side) {
@@ -1812,7 +1856,8 @@ __device__ __forceinline__ void end_sub(Stroker& S, bool close) {
 // points with the half width interpolated at their t on the whole curve
 // (t_whole = t0 + t * t_scale), as quadratic_bezier_to / cubic_bezier_to do.
 __device__ __forceinline__ void curve_piece(Stroker& S, const VRec& r, bool b, bool cubic,
-                                            float t0, float t_scale, float w0, float w1) {
+                                            float t0, float t_scale, float w0, float w1,
+                                            uint32_t id0, uint32_t id1) {
     const uint32_t n = b ? r.nB : r.nA;
     const float stp = b ? r.sB : r.sA;
     if (n > MAX_CURVE_POINTS) {
@@ -1834,6 +1879,11 @@ __device__ __forceinline__ void curve_piece(Stroker& S, const VRec& r, bool b, b
         const float tw = t_scale == 1.0f && t0 == 0.0f ? t1 : t0 + t1 * t_scale;
         Pt e;
         pt_init(e, pos, (w0 * (1.0f - tw) + w1 * tw) * 0.5f, (int)S.line_join, t1 != 1.0f);
+        // the endpoint at t == 1, else the edge (stroke.rs:1005-1013, 1041-1050)
+        e.sedge = tw != 1.0f;
+        e.sf = e.sedge ? id0 : id1;
+        e.sto = id1;
+        e.st = tw;
         step(S, e);
     }
 }
@@ -1892,6 +1942,15 @@ __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
         S.may_need_empty_cap = false;
         S.sub_path_start_adv = 0.0f;
         S.status = 0;
+        S.attrs = L.attrs;
+        S.n_attrs = L.n_attrs;
+        S.attr_out = EMIT && L.out_attrs ? L.out_attrs + (size_t)L.vtx_off[path] * L.n_attrs
+                                         : nullptr;
+        S.sf = S.sto = 0;
+        S.st = 0.0f;
+        S.sedge = false;
+        S.filled = sk.vid;
+        uint32_t cur_id = 0;  // point index (in the batch) of the current endpoint
         auto width = [&](uint32_t i) {
             return i < npts ? L.widths[(size_t)(pb + i) * L.widths_stride]
                             : __builtin_nanf("");
@@ -1908,12 +1967,16 @@ __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
                 Pt e;
                 pt_init(e, r.toA, o.line_width * cur_w * 0.5f, (int)o.line_join, false);
                 e.adv = S.sub_path_start_adv;
+                cur_id = pb + p;
+                e.sf = e.sto = cur_id;
                 step(S, e);
                 p += 1;
             } else if (verb == LYG_VERB_LINE) {
                 cur_w = width(p);
                 Pt e;
                 pt_init(e, r.toA, o.line_width * cur_w * 0.5f, (int)o.line_join, false);
+                cur_id = pb + p;
+                e.sf = e.sto = cur_id;
                 step(S, e);
                 p += 1;
             } else if (verb == LYG_VERB_QUADRATIC) {
@@ -1922,17 +1985,23 @@ __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
                 const float w1 = o.line_width * cur_w;
                 vec2 from = p > 0 ? ldp(pts, npts, p - 1) : mk(__builtin_nanf(""), __builtin_nanf(""));
                 const float ts = find_sharp_turn(from, ldp(pts, npts, p), ldp(pts, npts, p + 1));
+                const uint32_t id0 = cur_id;
+                cur_id = pb + p + 1;
                 if (ts > 0.0f) {  // flatten_quad's two halves (stroke.rs:2755-2779)
-                    curve_piece(S, r, false, false, 0.0f, ts, w0, w1);
-                    if (S.status == 0) curve_piece(S, r, true, false, ts, 1.0f - ts, w0, w1);
+                    curve_piece(S, r, false, false, 0.0f, ts, w0, w1, id0, cur_id);
+                    if (S.status == 0)
+                        curve_piece(S, r, true, false, ts, 1.0f - ts, w0, w1, id0, cur_id);
                 } else {
-                    curve_piece(S, r, false, false, 0.0f, 1.0f, w0, w1);
+                    curve_piece(S, r, false, false, 0.0f, 1.0f, w0, w1, id0, cur_id);
                 }
                 p += 2;
             } else if (verb == LYG_VERB_CUBIC) {
                 const float w0 = o.line_width * cur_w;
                 cur_w = width(p + 2);
-                curve_piece(S, r, false, true, 0.0f, 1.0f, w0, o.line_width * cur_w);
+                const uint32_t id0 = cur_id;
+                cur_id = pb + p + 2;
+                curve_piece(S, r, false, true, 0.0f, 1.0f, w0, o.line_width * cur_w, id0,
+                            cur_id);
                 p += 3;
             } else {
                 const bool close = verb == LYG_VERB_CLOSE;
@@ -1940,6 +2009,7 @@ __global__ void __launch_bounds__(64) stroke_vw_kernel(StrokeVWLaunch L) {
                 if (close) p += 1;
             }
         }
+        flush_attrs(S);
         if (!EMIT) {
             const bool ok = S.status == 0;
             L.path_status[path] = S.status;

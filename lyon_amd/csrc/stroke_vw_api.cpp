@@ -95,7 +95,8 @@ extern "C" {
 // (stroke.rs:653-767) for a batch in device memory.
 int32_t lyg_stroke_batch_vw(lyg_ctx* c, const lyg_batch_in* in, const float* attributes,
                             uint32_t n_attributes, uint32_t width_attribute,
-                            const lyg_stroke_options* opts, uint32_t n_opts, lyg_batch_out* out) {
+                            const lyg_stroke_options* opts, uint32_t n_opts, lyg_batch_out* out,
+                            float* out_attributes) {
     if (!c || !in || !opts || !out || !attributes) return LYG_ERR_INVALID_ARGUMENT;
     if (n_opts != 1 && n_opts != in->n_paths) return LYG_ERR_INVALID_ARGUMENT;
     if (width_attribute >= n_attributes) return LYG_ERR_INVALID_ARGUMENT;
@@ -114,6 +115,9 @@ int32_t lyg_stroke_batch_vw(lyg_ctx* c, const lyg_batch_in* in, const float* att
     L.opts_stride = n_opts == 1 ? 0 : 1;
     L.widths = attributes + width_attribute;
     L.widths_stride = n_attributes;
+    L.attrs = attributes;
+    L.n_attrs = n_attributes;
+    L.out_attrs = out_attributes;
     uint64_t tv = 0, ti = 0;
     std::vector<uint32_t> status;
     int32_t st = stroke_vw_run(
@@ -140,7 +144,8 @@ int32_t lyg_stroke_batch_vw(lyg_ctx* c, const lyg_batch_in* in, const float* att
 // attributes: n_points x n_attributes floats.
 int32_t lyg_stroke_path_vw(lyg_ctx* c, const lyg_path_view* path, const float* attributes,
                            uint32_t n_attributes, uint32_t width_attribute,
-                           const lyg_stroke_options* opts, lyg_output* out) {
+                           const lyg_stroke_options* opts, lyg_output* out,
+                           float* out_attributes) {
     if (!c || !path || !opts || !out) return LYG_ERR_INVALID_ARGUMENT;
     if (!valid_opts(*opts) || width_attribute >= n_attributes) return LYG_ERR_INVALID_ARGUMENT;
     if ((path->n_verbs && !path->verbs) || (path->n_points && !path->points) ||
@@ -179,6 +184,8 @@ int32_t lyg_stroke_path_vw(lyg_ctx* c, const lyg_path_view* path, const float* a
     L.opts_stride = 0;
     L.widths = as<float>(c->in_attrs) + width_attribute;
     L.widths_stride = n_attributes;
+    L.attrs = as<float>(c->in_attrs);
+    L.n_attrs = n_attributes;
     uint64_t tv = 0, ti = 0;
     std::vector<uint32_t> status;
     int32_t st = stroke_vw_run(
@@ -189,8 +196,10 @@ int32_t lyg_stroke_path_vw(lyg_ctx* c, const lyg_path_view* path, const float* a
                 out->n_indices = (uint32_t)ni;
                 return LYG_ERR_CAPACITY;
             }
-            if (!ensure(c, c->out_v, 8 * nv) || !ensure(c, c->out_i, 4 * ni))
+            if (!ensure(c, c->out_v, 8 * nv) || !ensure(c, c->out_i, 4 * ni) ||
+                (out_attributes && !ensure(c, c->out_attrs, 4ull * n_attributes * nv)))
                 return LYG_ERR_DEVICE;
+            L.out_attrs = out_attributes ? as<float>(c->out_attrs) : nullptr;
             *v = as<float>(c->out_v);
             *i = as<uint32_t>(c->out_i);
             return LYG_OK;
@@ -201,6 +210,11 @@ int32_t lyg_stroke_path_vw(lyg_ctx* c, const lyg_path_view* path, const float* a
     if (tv && !check(c, hipMemcpy(out->vertices, c->out_v.p, 8 * tv, hipMemcpyDeviceToHost), "D2H"))
         return LYG_ERR_DEVICE;
     if (ti && !check(c, hipMemcpy(out->indices, c->out_i.p, 4 * ti, hipMemcpyDeviceToHost), "D2H"))
+        return LYG_ERR_DEVICE;
+    if (out_attributes && tv &&
+        !check(c, hipMemcpy(out_attributes, c->out_attrs.p, 4ull * n_attributes * tv,
+                            hipMemcpyDeviceToHost),
+               "D2H"))
         return LYG_ERR_DEVICE;
     out->n_vertices = (uint32_t)tv;
     out->n_indices = (uint32_t)ti;

@@ -301,7 +301,7 @@ class StrokeTessellator:
 
             def call(ctx, pv, copts, out):
                 return B.lib().lyg_stroke_path_vw(ctx, pv, attrs.ctypes.data_as(B.F32P),
-                                                  attrs.shape[1], idx, copts, out)
+                                                  attrs.shape[1], idx, copts, out, None)
             _run(call, path, options.to_c(), builder)
             return
         _run(B.lib().lyg_stroke_path, path, options.to_c(), builder)

@@ -21,6 +21,7 @@ struct orc_result {
     std::vector<V2> vertices;
     std::vector<uint32_t> indices;
     std::vector<uint32_t> offsets;  // flatten: per-segment point offsets
+    std::vector<float> attrs;       // variable-width stroke: interpolated vertex attributes
     int status = 0;
 };
 
@@ -61,10 +62,18 @@ orc_result* orc_stroke_path(const uint8_t* verbs, uint32_t n_verbs, const float*
 
 // The same with StrokeOptions::variable_line_width: widths[i] is the width factor of the
 // endpoint stored at point i (tessellate_with_ids_vw, stroke.rs:653-767).
+size_t orc_result_n_attrs(const orc_result* r) { return r->attrs.size(); }
+void orc_result_copy_attrs(const orc_result* r, float* out) {
+    if (!r->attrs.empty()) std::memcpy(out, r->attrs.data(), r->attrs.size() * 4);
+}
+
+// attrs (optional): n_points x n_attrs custom attributes; each vertex's interpolated
+// attributes (StrokeVertex::interpolated_attributes) are then kept in the result.
 orc_result* orc_stroke_path_vw(const uint8_t* verbs, uint32_t n_verbs, const float* points,
                                uint32_t n_points, const float* widths, float line_width,
                                float miter_limit, float tolerance, int start_cap, int end_cap,
-                               int line_join, uint32_t vertex_offset) {
+                               int line_join, uint32_t vertex_offset, const float* attrs,
+                               uint32_t n_attrs) {
     orc_result* r = new orc_result();
     refpath::PathView pv{verbs, n_verbs, points, n_points};
     refstroke::StrokeOptions o;
@@ -75,6 +84,11 @@ orc_result* orc_stroke_path_vw(const uint8_t* verbs, uint32_t n_verbs, const flo
     o.end_cap = end_cap;
     o.line_join = line_join;
     refstroke::Output out{&r->vertices, &r->indices, vertex_offset};
+    if (attrs && n_attrs) {
+        out.attrs = attrs;
+        out.n_attrs = n_attrs;
+        out.attr_out = &r->attrs;
+    }
     r->status = refstroke::stroke_path_vw(pv, o, widths, out);
     return r;
 }

@@ -38,8 +38,11 @@ def lib():
         L.orc_stroke_path.restype = C.c_void_p
         L.orc_stroke_path_vw.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, F32P, C.c_float,
                                          C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
-                                         C.c_uint32]
+                                         C.c_uint32, F32P, C.c_uint32]
         L.orc_stroke_path_vw.restype = C.c_void_p
+        L.orc_result_n_attrs.argtypes = [C.c_void_p]
+        L.orc_result_n_attrs.restype = C.c_size_t
+        L.orc_result_copy_attrs.argtypes = [C.c_void_p, F32P]
         L.orc_fill_path.argtypes = [U8P, C.c_uint32, F32P, C.c_uint32, C.c_float, C.c_int,
                                     C.c_int, C.c_int, C.c_uint32]
         L.orc_fill_path.restype = C.c_void_p
@@ -89,20 +92,34 @@ def _collect(h):
 
 
 def stroke(path, line_width=1.0, miter_limit=4.0, tolerance=0.1, start_cap=0, end_cap=0,
-           line_join=0, vertex_offset=0, widths=None):
+           line_join=0, vertex_offset=0, widths=None, attributes=None):
     """StrokeTessellator::tessellate_path on the CPU.  Returns (status, vertices, indices).
     widths: per point of the path, the width factor of the endpoint stored there
-    (StrokeOptions::variable_line_width); None strokes at the fixed width."""
+    (StrokeOptions::variable_line_width); None strokes at the fixed width.
+    attributes (with widths): per point, the custom attributes (n_points x k); the result
+    then has a fourth item, every vertex's interpolated attributes (n_vertices x k)."""
     v = np.ascontiguousarray(path.verbs, np.uint8)
     p = np.ascontiguousarray(path.points, np.float32)
     if widths is not None:
         w = np.ascontiguousarray(widths, np.float32)
         assert len(w) == len(p)
+        a = None
+        if attributes is not None:
+            a = np.ascontiguousarray(attributes, np.float32).reshape(len(p), -1)
         h = lib().orc_stroke_path_vw(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p),
                                      _ptr(w, F32P), line_width, miter_limit, tolerance,
-                                     start_cap, end_cap, line_join, vertex_offset)
+                                     start_cap, end_cap, line_join, vertex_offset,
+                                     _ptr(a, F32P) if a is not None else None,
+                                     0 if a is None else a.shape[1])
+        attrs = None
+        if a is not None:
+            na = lib().orc_result_n_attrs(h)
+            attrs = np.zeros(na, np.float32)
+            if na:
+                lib().orc_result_copy_attrs(h, _ptr(attrs, F32P))
+            attrs = attrs.reshape(-1, a.shape[1])
         st, vv, ii, _ = _collect(h)
-        return st, vv, ii
+        return (st, vv, ii) if a is None else (st, vv, ii, attrs)
     h = lib().orc_stroke_path(_ptr(v, U8P), len(v), _ptr(p, F32P), len(p), line_width,
                               miter_limit, tolerance, start_cap, end_cap, line_join,
                               vertex_offset)

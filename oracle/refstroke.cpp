@@ -456,6 +456,7 @@ struct Builder {
         if (count > 1) {
             EndpointData& prev = point_buffer.second_last();
             EndpointData& join = point_buffer.last_of_two();
+            vertex.src = join.src;
             vertex.position_on_path = join.position;
             vertex.half_width = join.half_width;
             bool fast_path = false;
@@ -575,6 +576,7 @@ struct Builder {
     // stroke.rs:1282-1316
     bool tessellate_empty_cap() {
         EndpointData& p = point_buffer.get(0);
+        vertex.src = p.src;
         vertex.half_width = p.half_width;
         if (options.start_cap == CAP_SQUARE) return tessellate_empty_square_cap(p.position);
         if (options.start_cap == CAP_ROUND) return tessellate_empty_round_cap(p.position);
@@ -585,6 +587,7 @@ struct Builder {
     bool tessellate_last_edge(const EndpointData& p0, EndpointData& p1, bool is_first_edge) {
         V2 v = p1.position - p0.position;
         p1.advancement = p0.advancement + length(v);
+        vertex.src = p1.src;
         vertex.position_on_path = p1.position;
         vertex.half_width = p1.half_width;
         for (int side = 0; side < 2; ++side) {
@@ -618,6 +621,7 @@ struct Builder {
 
     // stroke.rs:2208-2276
     bool tessellate_first_edge(EndpointData& first, const EndpointData& second) {
+        vertex.src = first.src;
         vertex.position_on_path = first.position;
         vertex.half_width = first.half_width;
         for (int side = 0; side < 2; ++side) {
@@ -693,6 +697,7 @@ struct Builder {
             if (!step_any(p2, &dummy)) return false;
             EndpointData& p0 = point_buffer.second_last();
             EndpointData& p1 = point_buffer.last_of_two();
+            vertex.src = p0.src;
             vertex.position_on_path = p0.position;
             vertex.half_width = p0.half_width;
             for (int side = 0; side < 2; ++side) {
@@ -740,12 +745,14 @@ struct Builder {
     // quadratic_bezier_to with widths (stroke.rs:997-1030) over flatten_quad
     // (stroke.rs:2755-2779): the half width at each flattened point interpolates the
     // endpoints' widths at its t on the whole curve.
-    void quadratic_to_vw(V2 from, V2 ctrl, V2 to, float w0, float w1) {
+    void quadratic_to_vw(V2 from, V2 ctrl, V2 to, float w0, float w1, uint32_t id0,
+                         uint32_t id1) {
         float t_split;
         float tol = options.tolerance;
         auto at = [&](V2 p, float t, bool flat) {
             EndpointData e = make_point(p, flat);
             e.half_width = (w0 * (1.0f - t) + w1 * t) * 0.5f;
+            e.src = edge_src(id0, id1, t);
             step(e);
         };
         if (find_sharp_turn(from, ctrl, to, &t_split)) {
@@ -764,13 +771,31 @@ struct Builder {
     }
 
     // cubic_bezier_to with widths (stroke.rs:1032-1067)
-    void cubic_to_vw(V2 from, V2 c1, V2 c2, V2 to, float w0, float w1) {
+    void cubic_to_vw(V2 from, V2 c1, V2 c2, V2 to, float w0, float w1, uint32_t id0,
+                     uint32_t id1) {
         cubic_flatten_with_t(from, c1, c2, to, options.tolerance,
                              [&](V2, V2 p, float, float t1) {
                                  EndpointData e = make_point(p, t1 != 1.0f);
                                  e.half_width = (w0 * (1.0f - t1) + w1 * t1) * 0.5f;
+                                 e.src = edge_src(id0, id1, t1);
                                  step(e);
                              });
+    }
+
+    // quadratic_bezier_to / cubic_bezier_to (stroke.rs:1005-1013, 1041-1050): the
+    // endpoint at t == 1, else the edge at t
+    static Source edge_src(uint32_t id0, uint32_t id1, float t) {
+        Source r;
+        r.from = t == 1.0f ? id1 : id0;
+        r.to = id1;
+        r.t = t;
+        r.edge = t != 1.0f;
+        return r;
+    }
+    static Source endpoint_src(uint32_t id) {
+        Source r;
+        r.from = r.to = id;
+        return r;
     }
 
     // cubic_bezier_to_fw (stroke.rs:1145-1177)
@@ -827,13 +852,14 @@ int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output
 // endpoint's half width is line_width * its width factor * 0.5; curves interpolate the
 // full widths of their two endpoints.
 int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
-                   const float* widths, Output& out) {
+                   const float* widths, Output& out, uint32_t point_base) {
     Builder b(opts, out);
     b.vw = true;
     refpath::Iter it(path);
     refpath::Event e;
     V2 current{NAN, NAN};
     float cur_w = NAN;  // width factor of the current endpoint
+    uint32_t cur_id = 0;  // its point index (+ point_base: the ids of the attribute rows)
     auto width_at = [&](uint32_t pi) { return pi < path.n_points ? widths[pi] : NAN; };
     while (it.next(e)) {
         const uint32_t pi = it.pi - 1;  // the event's endpoint (Begin, Line, Quad, Cubic)
@@ -845,6 +871,8 @@ int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
                 EndpointData p = b.make_point(e.to, false);
                 p.half_width = opts.line_width * cur_w * 0.5f;
                 p.advancement = b.sub_path_start_advancement;
+                cur_id = point_base + pi;
+                p.src = Builder::endpoint_src(cur_id);
                 b.step(p);
                 break;
             }
@@ -853,6 +881,8 @@ int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
                 cur_w = width_at(pi);
                 EndpointData p = b.make_point(e.to, false);
                 p.half_width = opts.line_width * cur_w * 0.5f;
+                cur_id = point_base + pi;
+                p.src = Builder::endpoint_src(cur_id);
                 b.step(p);
                 break;
             }
@@ -861,7 +891,9 @@ int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
                 current = e.to;
                 const float w0 = opts.line_width * cur_w;
                 cur_w = width_at(pi);
-                b.quadratic_to_vw(from, e.ctrl1, e.to, w0, opts.line_width * cur_w);
+                const uint32_t id0 = cur_id;
+                cur_id = point_base + pi;
+                b.quadratic_to_vw(from, e.ctrl1, e.to, w0, opts.line_width * cur_w, id0, cur_id);
                 break;
             }
             case refpath::EV_CUBIC: {
@@ -869,7 +901,10 @@ int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
                 current = e.to;
                 const float w0 = opts.line_width * cur_w;
                 cur_w = width_at(pi);
-                b.cubic_to_vw(from, e.ctrl1, e.ctrl2, e.to, w0, opts.line_width * cur_w);
+                const uint32_t id0 = cur_id;
+                cur_id = point_base + pi;
+                b.cubic_to_vw(from, e.ctrl1, e.ctrl2, e.to, w0, opts.line_width * cur_w, id0,
+                              cur_id);
                 break;
             }
             case refpath::EV_END:

@@ -31,6 +31,11 @@ struct Output {
     std::vector<uint32_t>* indices;
     uint32_t vertex_offset = 0;
     size_t first_vertex = 0, first_index = 0;
+    // custom attributes per point (n_attrs each) and, when set, the interpolated
+    // attributes of every vertex
+    const float* attrs = nullptr;
+    uint32_t n_attrs = 0;
+    std::vector<float>* attr_out = nullptr;
     void begin_geometry() {
         first_vertex = vertices->size();
         first_index = indices->size();
@@ -51,6 +56,7 @@ struct Output {
     void abort_geometry() {
         vertices->resize(first_vertex);
         indices->resize(first_index);
+        if (attr_out) attr_out->resize(first_vertex * n_attrs);
     }
 };
 
@@ -63,10 +69,19 @@ struct SidePoints {
 
 // EndpointData (stroke.rs:320-352); vertex sources are omitted because they never
 // influence positions or topology (advancement does, through the variable-width joins).
+// VertexSource (stroke.rs / lib.rs): an endpoint (a point index) or a point at t on the
+// edge between two endpoints.
+struct Source {
+    uint32_t from = 0xffffffffu, to = 0xffffffffu;
+    float t = 0.0f;
+    bool edge = false;
+};
+
 struct EndpointData {
     V2 position{0, 0};
     float half_width = NAN;
     float advancement = NAN;
+    Source src;
     int line_join = JOIN_MITER;
     SidePoints side_points[2];
     bool fold[2] = {false, false};
@@ -79,12 +94,21 @@ struct VertexData {
     float half_width = 0;
     V2 normal{0, 0};
     int side = SIDE_NEGATIVE;
+    Source src;
 };
 
-// StrokeVertex::position (stroke.rs:2676-2678)
+// StrokeVertex::position (stroke.rs:2676-2678); with attributes to interpolate, also
+// StrokeVertex::interpolated_attributes (stroke.rs:2728-2746) into out.attr_out.
 inline bool emit_vertex(Output& out, const VertexData& v, uint32_t* id) {
     V2 pos = v.position_on_path + v.normal * v.half_width;
-    return out.add_vertex(pos, id);
+    if (!out.add_vertex(pos, id)) return false;
+    if (out.attr_out) {
+        const float* a = out.attrs + (size_t)v.src.from * out.n_attrs;
+        const float* b = out.attrs + (size_t)v.src.to * out.n_attrs;
+        for (uint32_t k = 0; k < out.n_attrs; ++k)
+            out.attr_out->push_back(v.src.edge ? a[k] * (1.0f - v.src.t) + b[k] * v.src.t : a[k]);
+    }
+    return true;
 }
 
 // PointBuffer (stroke.rs:2371-2457)
@@ -115,8 +139,10 @@ struct PointBuffer {
 int stroke_path(const refpath::PathView& path, const StrokeOptions& opts, Output& out);
 // StrokeOptions::variable_line_width: widths[i] is the line-width factor attribute of the
 // endpoint stored at point i (control points' entries are not read).
+// With out.attr_out set, the interpolated custom attributes of every vertex go there;
+// out.attrs is the attribute table, point_base the row of the path's first point.
 int stroke_path_vw(const refpath::PathView& path, const StrokeOptions& opts,
-                   const float* widths, Output& out);
+                   const float* widths, Output& out, uint32_t point_base = 0);
 bool find_sharp_turn_export(float fx, float fy, float cx, float cy, float tx, float ty,
                             float* t_out);
 uint32_t arc_subdivisions_export(float diff, float radius, float tolerance);

@@ -226,3 +226,40 @@ def test_constant_width_covers_the_fixed_width_stroke(join):
         loose += differ > 0.001 * either
     assert compared >= 12 and same_count >= compared - 4, (compared, same_count)
     assert loose <= 2, loose
+
+
+def test_vertex_source_attributes():
+    """stroke_vertex_source_01 (stroke.rs:3047-3126): a (0,0) [1], b (10,10) [2], a
+    quadratic to c (0,20) [3], closed, variable_line_width(0).  Every vertex's interpolated
+    attribute is its source's (stroke.rs:2728-2746): an endpoint's own value, or for points
+    of the quadratic b -> c strictly between 2 and 3 — and every source occurs."""
+    b = PathBuilder(1)
+    b.begin((0, 0), [1.0])
+    b.line_to((10, 10), [2.0])
+    b.quadratic_bezier_to((10, 20), (0, 20), [3.0])
+    b.end(True)
+    p = b.build()
+    st, v, i, a = O.stroke(p, widths=p.attributes[:, 0], attributes=p.attributes)
+    assert st == 0 and a.shape == (len(v), 1)
+    vals = a[:, 0]
+    on_end = np.isin(vals, [1.0, 2.0, 3.0])
+    assert np.all(on_end | ((vals > 2.0) & (vals < 3.0)))
+    for x in (1.0, 2.0, 3.0):
+        assert np.any(vals == x)
+    assert np.any(~on_end)
+
+
+def test_edge_attributes_interpolate_along_the_curve():
+    """On a cubic from width factor 1 to 3 carrying a second attribute from 10 to 20, the
+    vertices of each flattened point share one source: both attributes are the same affine
+    function of t (a2 = 10 + 5 (a1 - 1) up to rounding), and they grow along the curve."""
+    b = PathBuilder(2)
+    b.begin((0, 0), [1.0, 10.0])
+    b.cubic_bezier_to((100, 0), (100, 100), (200, 100), [3.0, 20.0])
+    b.end(False)
+    p = b.build()
+    st, v, i, a = O.stroke(p, line_width=2.0, tolerance=0.05, widths=p.attributes[:, 0],
+                           attributes=p.attributes)
+    assert st == 0
+    assert np.allclose(a[:, 1], 10.0 + 5.0 * (a[:, 0] - 1.0), atol=1e-4)
+    assert a[:, 0].min() == 1.0 and a[:, 0].max() == 3.0
